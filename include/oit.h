@@ -1,0 +1,195 @@
+/*
+ * oit.h - C ABI of the MI355X (gfx950) order-independent-transparency
+ * fragment-store and composite path.
+ *
+ * The reference (nvpro-samples/vk_order_independent_transparency) has no C
+ * ABI: each algorithm is a method pair on its Sample class,
+ * clearTransparent<Alg>() / drawTransparent<Alg>() (oit.h:379-425), dispatched
+ * by switch(m_state.algorithm) in Sample::onRender (oitRender.cpp:43-66,
+ * 125-148), with all buffers owned by Sample (oit.h:173-214) and sized in
+ * Sample::createFrameImages (oit.cpp:84-163). Each entry point below names
+ * the reference code it replaces.
+ *
+ * Conventions
+ *  - Every function returns an int status (OIT_OK == 0, negative on error)
+ *    and never aborts. Overflow of an A-buffer is not an error: it is
+ *    tail-blended as in the reference and reported by oit_stats().
+ *  - Pointers in oit_fragments and output pointers are DEVICE pointers.
+ *  - `stream` is a hipStream_t passed as void* (NULL = the null stream).
+ *    Calls are stream-ordered; a context is not thread-safe.
+ *  - A "target" is one A-buffer list: sample * width * height + y * width + x
+ *    (the reference's per-sample image-array addressing,
+ *    oitColorDepthDefines.glsl:433-441, oitCompositeDefines.glsl:282-290).
+ *  - Fragment ids are assigned in append order: the i-th fragment of the
+ *    k-th oit_append call of a frame has id = (fragments appended earlier in
+ *    the frame) + i. Ids are the canonical draw order and break depth ties.
+ */
+#ifndef OIT_MI355X_H
+#define OIT_MI355X_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define OIT_ABI_VERSION 1
+
+/* Algorithms; values equal the reference's shaders/common.h:44-51. */
+enum oit_algorithm {
+  OIT_SIMPLE     = 0,
+  OIT_LINKEDLIST = 1,
+  OIT_LOOP       = 2, /* not provided: OIT_ERR_UNSUPPORTED */
+  OIT_LOOP64     = 3,
+  OIT_SPINLOCK   = 4,
+  OIT_INTERLOCK  = 5, /* not provided: OIT_ERR_UNSUPPORTED */
+  OIT_WEIGHTED   = 6
+};
+
+/* Status codes. */
+enum oit_status {
+  OIT_OK               = 0,
+  OIT_ERR_INVALID_ARG  = -1,
+  OIT_ERR_OOM          = -2,
+  OIT_ERR_OVER_BUDGET  = -3,
+  OIT_ERR_DEVICE       = -4,
+  OIT_ERR_UNSUPPORTED  = -5,
+  OIT_ERR_CAPACITY     = -6,
+  OIT_ERR_STATE        = -7,
+  OIT_ERR_NOT_READY    = -8
+};
+
+/* Renderer configuration; mirrors the reference's State (oit.h:64-116) and
+ * the per-frame SceneData contract (shaders/common.h:77-92). */
+typedef struct oit_config {
+  uint32_t algorithm;          /* State::algorithm (oit.h:66) */
+  uint32_t layers;             /* OIT_LAYERS = State::oitLayers (oit.h:67): 1,2,4,8,16,32 */
+  uint32_t tail_blend;         /* OIT_TAILBLEND = State::tailBlend (oit.h:70) */
+  uint32_t width;              /* viewport width  (SceneData.viewport.x) */
+  uint32_t height;             /* viewport height (SceneData.viewport.y) */
+  uint32_t samples;            /* sample planes: 1, or the MSAA count with sample shading */
+  uint32_t ll_nodes_per_pixel; /* State::linkedListAllocatedPerElement (oit.h:68), used when ll_pool_nodes == 0 */
+  uint32_t reserved0;
+  uint64_t ll_pool_nodes;      /* Linked List capacity in nodes INCLUDING terminator node 0
+                                  (SceneData.linkedListAllocatedPerElement for OIT_LINKEDLIST,
+                                  oit.cpp:125); 0 = ll_nodes_per_pixel * width * height * samples */
+  uint64_t max_fragments;      /* per-frame fragment capacity of the tail lists; 0 = grow on demand */
+  float    background[4];      /* premultiplied RGBA the colour pass starts from (opaque gray 0.5 by default) */
+  float    wboit_view_near;    /* WBOIT: view distance = near + depth * span ... */
+  float    wboit_view_span;    /* ... stands in for -viewZ (object.vert.glsl:35) */
+} oit_config;
+
+/* A batch of fragments in draw order (structure of arrays, device memory). */
+typedef struct oit_fragments {
+  const uint32_t* target; /* list index, see conventions above */
+  const float*    depth;  /* gl_FragCoord.z (any finite float; order-preserving key) */
+  const float*    rgba;   /* premultiplied linear RGBA, 4 floats per fragment */
+  uint64_t        count;
+} oit_fragments;
+
+/* Memory footprint, like the reference GUI's "Object Sizes" (oitGui.cpp:327-336). */
+typedef struct oit_sizes {
+  uint64_t abuffer_bytes;     /* A-buffer / node pool / WBOIT accumulation targets */
+  uint64_t aux_bytes;         /* per-target counters, heads, locks, depth bounds */
+  uint64_t tail_bytes;        /* per-fragment tail-list structures currently allocated */
+  uint64_t total_bytes;
+  uint64_t ll_capacity;       /* LL nodes including the terminator */
+  uint32_t bytes_per_element; /* per stored fragment */
+  uint32_t targets;           /* width * height * samples */
+} oit_sizes;
+
+/* Counters of the last frame (synchronises `stream`). */
+typedef struct oit_stats {
+  uint64_t fragments;         /* appended this frame (the LL counter value, oitLinkedList.frag.glsl:55) */
+  uint64_t ll_overflow;       /* LL fragments that found the pool full and were tail-blended */
+  uint64_t tail_fragments;    /* fragments recorded in the colour-pass tail lists */
+  uint64_t appends;           /* oit_append calls this frame */
+} oit_stats;
+
+typedef struct oit_ctx oit_ctx;
+
+/* Creates a context on HIP device `device`. Replaces Sample::onAttach's
+ * resource set-up (main.cpp:74-128). */
+int oit_create(oit_ctx** out, int device);
+/* Frees every device buffer. Replaces Sample::onDetach/destroy* (oit.cpp). */
+int oit_destroy(oit_ctx* ctx);
+/* Validates `cfg`, (re)allocates the A-buffer, aux targets and pool.
+ * Replaces updateRendererFromState + createFrameImages (main.cpp:130-250,
+ * oit.cpp:45-220). */
+int oit_configure(oit_ctx* ctx, const oit_config* cfg);
+/* Reports the allocation sizes of the current configuration. */
+int oit_query_sizes(const oit_ctx* ctx, oit_sizes* out);
+/* Starts a frame: clears counters, heads, locks and k-buffers to the
+ * reference's clear values. Replaces clearTransparent<Alg> (oitRender.cpp:
+ * 156-174, 201-216, 303-311, 337-356) and the WBOIT clear values
+ * (oitRender.cpp:394-397). */
+int oit_clear(oit_ctx* ctx, void* stream);
+/* Fragment-store ("color") pass for a batch of fragments. Replaces the
+ * COLOR draw of drawTransparent<Alg> (oitRender.cpp:176-186, 218-227,
+ * 313-322, 358-368, 384-414) running oitSimple/oitLinkedList/oitLoop64/
+ * oitSpinlock/oitWeighted.frag.glsl with PASS == PASS_COLOR. The fragment
+ * buffers must stay valid until oit_resolve has been enqueued when the
+ * algorithm is OIT_LOOP64 with tail blending (the resolve re-reads them). */
+int oit_append(oit_ctx* ctx, const oit_fragments* frags, void* stream);
+/* Composite pass: writes the final premultiplied RGBA per target
+ * (composite over tail-blended colour buffer over background) to
+ * out_rgba[4 * targets]. Replaces the COMPOSITE full-screen draw
+ * (oitRender.cpp:191-198, 232-239, 327-334, 373-381, 416-424) with its
+ * PREMULTIPLIED / WEIGHTED_COMPOSITE framebuffer blend (main.cpp:548-558,
+ * 578-588). */
+int oit_resolve(oit_ctx* ctx, float* out_rgba, void* stream);
+/* Like oit_resolve, and additionally writes per target the sorted front-K
+ * depth keys and values (fragment ids; packed colours for OIT_LOOP64) to
+ * frontk_key/frontk_val[targets * layers] (0xFFFFFFFF padding) and the
+ * algorithm's per-target count to counts[targets]. Any of the three extra
+ * pointers may be NULL. For parity tests; not a reference entry point. */
+int oit_resolve_debug(oit_ctx* ctx, float* out_rgba, uint32_t* frontk_key,
+                      uint32_t* frontk_val, uint32_t* counts, void* stream);
+/* Copies the frame counters to the host (synchronises `stream`). */
+int oit_get_stats(oit_ctx* ctx, oit_stats* out, void* stream);
+/* Human-readable name of a status code. */
+const char* oit_status_string(int status);
+
+/* ---- Stage timing (replaces the reference's per-pass GPU timer sections,
+ * e.g. "LinkedListColor" / "LinkedListComposite", oitRender.cpp:223,235).
+ * Stages: 0 clear, 1 store (colour pass), 2 link (LL head insertion),
+ * 3 classify (Loop64 tail classification), 4 tail link, 5 resolve. ---- */
+#define OIT_PROFILE_STAGES 6
+typedef struct oit_profile {
+  double   ms[OIT_PROFILE_STAGES];
+  uint64_t launches[OIT_PROFILE_STAGES];
+} oit_profile;
+
+/* Brackets every later kernel launch of the context with HIP events. */
+int oit_profile_enable(oit_ctx* ctx, int enable);
+/* Waits for the recorded events and returns the accumulated stage times. */
+int oit_profile_read(oit_ctx* ctx, oit_profile* out);
+/* Zeroes the accumulated stage times. */
+int oit_profile_reset(oit_ctx* ctx);
+
+/* ---- Synthetic scene (stands in for the reference's rasterised sphere
+ * mesh, main.cpp:334-417): analytic spheres, orthographic, each covered
+ * pixel (or sample) centre emits a front and a back fragment. ---------- */
+typedef struct oit_scene_params {
+  uint64_t seed;
+  uint32_t width, height;
+  uint32_t samples;           /* 1, or 4 for the standard 4x sample positions */
+  uint32_t spheres;
+  float    radius_min, radius_max;
+  float    alpha_min, alpha_max;
+  float    cluster_fraction;  /* fraction of spheres packed in a cluster_extent-wide square */
+  float    cluster_extent;
+} oit_scene_params;
+
+/* Number of fragments the scene emits (host computation). */
+int oit_scene_count(const oit_scene_params* p, uint64_t* out_count);
+/* Writes the fragment stream in draw order into HOST arrays of `capacity`
+ * fragments (target[], depth[], rgba[4*]). */
+int oit_scene_generate_host(const oit_scene_params* p, uint32_t* target, float* depth,
+                            float* rgba, uint64_t capacity, uint64_t* out_count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* OIT_MI355X_H */

@@ -1,0 +1,664 @@
+// oit_api.hip - host runtime behind include/oit.h: configuration, A-buffer
+// sizing and ownership, per-frame pass order, and kernel dispatch.
+//
+// Reference counterparts: State and its rebuild logic (oit.h:64-116,
+// main.cpp:130-250), createFrameImages sizing (oit.cpp:84-163), the
+// clear/draw method pairs dispatched by Sample::onRender (oitRender.cpp:
+// 28-154, 156-425).
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <vector>
+
+#include "oit.h"
+#include "oit_kernels.hpp"
+
+using oit::kEmpty;
+
+namespace {
+
+constexpr uint32_t kMaxFrameFragments = 0x7FFFFFFEu;  // ids and tail links fit in 31 bits
+constexpr uint64_t kMaxPoolNodes = 0xFFFFFFFEull;      // node indices stay below the kUnset sentinel
+
+enum Stage { kStClear = 0, kStStore, kStLink, kStClassify, kStTailLink, kStResolve, kStCount };
+
+struct EventPair {
+  hipEvent_t a, b;
+  int stage;
+};
+
+// --- sRGB8 codec tables (host) ---------------------------------------------
+// Encode curve of unPremultLinearToSRGB (shaderCommon.glsl:60-69) and the
+// decode curve of unPremultSRGBToLinear (shaderCommon.glsl:84-94), evaluated
+// in double; thresholds are found by bisection over binary32 bit patterns.
+double srgb_curve(double c) { return c < (double)0.0031308f ? c * 12.92 : std::pow(c, 1.0 / 2.4) * 1.055 - 0.055; }
+
+int srgb_code(float c) {
+  double s = srgb_curve((double)c);
+  if (!(s > 0.0)) s = 0.0;
+  if (s > 1.0) s = 1.0;
+  return (int)std::floor(s * 255.0 + 0.5);
+}
+
+float from_bits(uint32_t b) {
+  float f;
+  std::memcpy(&f, &b, 4);
+  return f;
+}
+
+void build_codec(oit::CodecTables* t) {
+  t->thr[0] = -INFINITY;
+  t->thr[256] = INFINITY;
+  for (int q = 1; q <= 255; ++q) {
+    uint32_t lo = 0u, hi = 0x3F800000u;
+    while (hi - lo > 1u) {
+      const uint32_t mid = lo + (hi - lo) / 2u;
+      if (srgb_code(from_bits(mid)) >= q) hi = mid; else lo = mid;
+    }
+    t->thr[q] = from_bits(hi);
+  }
+  for (int u = 0; u < 256; ++u) {
+    const double f = (double)u / 255.0;
+    const double lin = f < (double)0.04045f ? f / 12.92 : std::pow((f + 0.055) / 1.055, 2.4);
+    t->dec[u] = (float)lin;
+    t->alpha[u] = (float)u / 255.0f;
+  }
+}
+
+bool valid_layers(uint32_t k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16 || k == 32; }
+
+}  // namespace
+
+struct oit_ctx {
+  int device = 0;
+  bool configured = false;
+  oit_config cfg{};
+  uint32_t T = 0;
+  oit::CodecTables* d_tab = nullptr;
+  uint32_t* d_counters = nullptr;  // [0] tail records, [1] LL overflow
+  // Linked List
+  uint32_t* ll_head = nullptr;
+  uint64_t* ll_cd = nullptr;
+  uint32_t* ll_next = nullptr;
+  uint64_t ll_cap = 0;
+  // Simple / Spinlock
+  uint32_t* kb_count = nullptr;
+  uint32_t* kb_lock = nullptr;
+  uint32_t* kb_depth = nullptr;
+  uint32_t* kb_col = nullptr;
+  uint32_t* kb_key = nullptr;
+  uint32_t* kb_id = nullptr;
+  // Loop64
+  uint64_t* l64 = nullptr;
+  // WBOIT
+  float4* wb_accum = nullptr;
+  float* wb_reveal = nullptr;
+  // colour-pass tail lists
+  uint32_t* tl_head = nullptr;
+  uint32_t* tl_next = nullptr;
+  float4* tl_rgba = nullptr;
+  uint64_t* tl_rec = nullptr;
+  uint32_t tl_cap = 0;
+  bool tl_head_valid = false;  // cleared for the current frame
+  // frame state: oit_clear opens a frame; the first resolve links the tail
+  // lists, after which the frame takes no more appends.
+  bool frame_open = false;
+  bool resolved = false;
+  uint64_t frame_frags = 0;
+  uint64_t appends = 0;
+  std::vector<oit::FragView> segments;
+  // sizes
+  uint64_t bytes_abuf = 0, bytes_aux = 0;
+  // profiling
+  bool profiling = false;
+  std::vector<EventPair> events;
+  std::vector<hipEvent_t> event_pool;
+  double prof_ms[kStCount] = {0};
+  uint64_t prof_launches[kStCount] = {0};
+};
+
+namespace {
+
+struct DeviceGuard {
+  int prev = -1;
+  explicit DeviceGuard(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceGuard() {
+    if (prev >= 0) (void)hipSetDevice(prev);
+  }
+};
+
+template <typename T>
+int dev_alloc(T** p, uint64_t count) {
+  *p = nullptr;
+  if (count == 0) return OIT_OK;
+  void* q = nullptr;
+  if (hipMalloc(&q, count * sizeof(T)) != hipSuccess) {
+    (void)hipGetLastError();
+    return OIT_ERR_OOM;
+  }
+  *p = static_cast<T*>(q);
+  return OIT_OK;
+}
+
+template <typename T>
+void dev_free(T** p) {
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+}
+
+void free_frame_buffers(oit_ctx* c) {
+  dev_free(&c->ll_head);
+  dev_free(&c->ll_cd);
+  dev_free(&c->ll_next);
+  dev_free(&c->kb_count);
+  dev_free(&c->kb_lock);
+  dev_free(&c->kb_depth);
+  dev_free(&c->kb_col);
+  dev_free(&c->kb_key);
+  dev_free(&c->kb_id);
+  dev_free(&c->l64);
+  dev_free(&c->wb_accum);
+  dev_free(&c->wb_reveal);
+  dev_free(&c->tl_head);
+  dev_free(&c->tl_next);
+  dev_free(&c->tl_rgba);
+  dev_free(&c->tl_rec);
+  c->tl_cap = 0;
+  c->tl_head_valid = false;
+  c->ll_cap = 0;
+  c->bytes_abuf = c->bytes_aux = 0;
+  c->configured = false;
+}
+
+int status_of(hipError_t e) {
+  if (e == hipSuccess) return OIT_OK;
+  if (e == hipErrorOutOfMemory) return OIT_ERR_OOM;
+  if (e == hipErrorInvalidValue) return OIT_ERR_INVALID_ARG;
+  return OIT_ERR_DEVICE;
+}
+
+hipEvent_t take_event(oit_ctx* c) {
+  if (!c->event_pool.empty()) {
+    hipEvent_t e = c->event_pool.back();
+    c->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Runs `launch` bracketed by events when profiling is on.
+template <typename F>
+hipError_t staged(oit_ctx* c, int stage, hipStream_t s, F&& launch) {
+  if (!c->profiling) return launch();
+  hipEvent_t a = take_event(c), b = take_event(c);
+  if (!a || !b) return launch();
+  (void)hipEventRecord(a, s);
+  const hipError_t e = launch();
+  (void)hipEventRecord(b, s);
+  c->events.push_back({a, b, stage});
+  return e;
+}
+
+oit::Common common_of(const oit_ctx* c) {
+  oit::Common k;
+  k.T = c->T;
+  k.K = c->cfg.layers;
+  k.tail = c->cfg.tail_blend ? 1 : 0;
+  k.background = make_float4(c->cfg.background[0], c->cfg.background[1], c->cfg.background[2], c->cfg.background[3]);
+  k.tab = c->d_tab;
+  return k;
+}
+
+oit::TailBuf tail_of(const oit_ctx* c) {
+  oit::TailBuf t;
+  const bool on = c->tl_head != nullptr && c->tl_head_valid;
+  t.head = on ? c->tl_head : nullptr;
+  t.next = c->tl_next;
+  t.rgba = c->tl_rgba;
+  t.rec = c->tl_rec;
+  t.counter = c->d_counters;
+  t.cap = c->tl_cap;
+  return t;
+}
+
+bool uses_tail_lists(const oit_ctx* c) { return c->cfg.tail_blend && c->cfg.algorithm != OIT_WEIGHTED; }
+
+// Makes the per-fragment tail arrays hold `need` fragments, keeping this
+// frame's records (stream-ordered copies; the old buffers are freed after a
+// stream synchronisation).
+int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
+  if (!c->tl_head) {
+    int rc = dev_alloc(&c->tl_head, c->T);
+    if (rc) return rc;
+    c->tl_head_valid = false;
+  }
+  if (!c->tl_head_valid) {
+    hipError_t e = oit::launch_fill_u32(c->tl_head, c->T, kEmpty, s);
+    if (e != hipSuccess) return status_of(e);
+    c->tl_head_valid = true;
+  }
+  if (need <= c->tl_cap) return OIT_OK;
+  uint64_t cap = c->tl_cap ? c->tl_cap : 1024;
+  while (cap < need) cap *= 2;
+  if (c->cfg.max_fragments && cap > c->cfg.max_fragments) cap = c->cfg.max_fragments;
+  if (cap < need) return OIT_ERR_CAPACITY;
+  if (cap > kMaxFrameFragments) cap = kMaxFrameFragments;
+  uint32_t* nn = nullptr;
+  float4* nr = nullptr;
+  uint64_t* nrec = nullptr;
+  int rc = dev_alloc(&nn, cap);
+  if (!rc) rc = dev_alloc(&nr, cap);
+  if (!rc) rc = dev_alloc(&nrec, cap);
+  if (rc) {
+    dev_free(&nn);
+    dev_free(&nr);
+    dev_free(&nrec);
+    return rc;
+  }
+  if (c->tl_cap) {
+    (void)hipMemcpyAsync(nn, c->tl_next, (uint64_t)c->tl_cap * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(nr, c->tl_rgba, (uint64_t)c->tl_cap * sizeof(float4), hipMemcpyDeviceToDevice, s);
+    (void)hipMemcpyAsync(nrec, c->tl_rec, (uint64_t)c->tl_cap * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+    (void)hipStreamSynchronize(s);
+  }
+  dev_free(&c->tl_next);
+  dev_free(&c->tl_rgba);
+  dev_free(&c->tl_rec);
+  c->tl_next = nn;
+  c->tl_rgba = nr;
+  c->tl_rec = nrec;
+  c->tl_cap = (uint32_t)cap;
+  return OIT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* oit_status_string(int status) {
+  switch (status) {
+    case OIT_OK: return "ok";
+    case OIT_ERR_INVALID_ARG: return "invalid argument";
+    case OIT_ERR_OOM: return "out of device memory";
+    case OIT_ERR_OVER_BUDGET: return "over memory budget";
+    case OIT_ERR_DEVICE: return "device error";
+    case OIT_ERR_UNSUPPORTED: return "unsupported configuration";
+    case OIT_ERR_CAPACITY: return "capacity exceeded";
+    case OIT_ERR_STATE: return "invalid call order";
+    case OIT_ERR_NOT_READY: return "not ready";
+    default: return "unknown status";
+  }
+}
+
+int oit_create(oit_ctx** out, int device) {
+  if (!out) return OIT_ERR_INVALID_ARG;
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    (void)hipGetLastError();
+    return OIT_ERR_DEVICE;
+  }
+  oit_ctx* c = new (std::nothrow) oit_ctx();
+  if (!c) return OIT_ERR_OOM;
+  c->device = device;
+  DeviceGuard g(device);
+  oit::CodecTables host;
+  build_codec(&host);
+  int rc = dev_alloc(&c->d_tab, 1);
+  if (!rc) rc = dev_alloc(&c->d_counters, 16);
+  if (!rc && hipMemcpy(c->d_tab, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess) rc = OIT_ERR_DEVICE;
+  if (!rc && hipMemset(c->d_counters, 0, 16 * sizeof(uint32_t)) != hipSuccess) rc = OIT_ERR_DEVICE;
+  if (rc) {
+    dev_free(&c->d_tab);
+    dev_free(&c->d_counters);
+    delete c;
+    return rc;
+  }
+  *out = c;
+  return OIT_OK;
+}
+
+int oit_destroy(oit_ctx* c) {
+  if (!c) return OIT_ERR_INVALID_ARG;
+  DeviceGuard g(c->device);
+  (void)hipDeviceSynchronize();
+  free_frame_buffers(c);
+  dev_free(&c->d_tab);
+  dev_free(&c->d_counters);
+  for (auto& p : c->events) {
+    (void)hipEventDestroy(p.a);
+    (void)hipEventDestroy(p.b);
+  }
+  for (auto e : c->event_pool) (void)hipEventDestroy(e);
+  delete c;
+  return OIT_OK;
+}
+
+int oit_configure(oit_ctx* c, const oit_config* cfg) {
+  if (!c || !cfg) return OIT_ERR_INVALID_ARG;
+  const uint32_t alg = cfg->algorithm;
+  if (alg == OIT_LOOP || alg == OIT_INTERLOCK) return OIT_ERR_UNSUPPORTED;
+  if (alg != OIT_SIMPLE && alg != OIT_LINKEDLIST && alg != OIT_LOOP64 && alg != OIT_SPINLOCK && alg != OIT_WEIGHTED)
+    return OIT_ERR_INVALID_ARG;
+  if (alg != OIT_WEIGHTED && !valid_layers(cfg->layers)) return OIT_ERR_INVALID_ARG;
+  if (cfg->width == 0 || cfg->height == 0) return OIT_ERR_INVALID_ARG;
+  const uint32_t S = cfg->samples ? cfg->samples : 1;
+  if (S != 1 && S != 2 && S != 4 && S != 8) return OIT_ERR_INVALID_ARG;
+  const uint64_t T = (uint64_t)cfg->width * cfg->height * S;
+  if (T >= 0xFFFFFFFFull) return OIT_ERR_INVALID_ARG;
+  for (int k = 0; k < 4; ++k)
+    if (!std::isfinite(cfg->background[k])) return OIT_ERR_INVALID_ARG;
+  DeviceGuard g(c->device);
+  (void)hipDeviceSynchronize();
+  free_frame_buffers(c);
+  c->cfg = *cfg;
+  c->cfg.samples = S;
+  if (alg == OIT_WEIGHTED && !valid_layers(c->cfg.layers)) c->cfg.layers = 1;
+  c->T = (uint32_t)T;
+  const uint32_t K = c->cfg.layers;
+  int rc = OIT_OK;
+  switch (alg) {
+    case OIT_LINKEDLIST: {  // oit.cpp:119-126
+      uint64_t cap = cfg->ll_pool_nodes;
+      if (cap == 0) cap = (uint64_t)(cfg->ll_nodes_per_pixel ? cfg->ll_nodes_per_pixel : 10) * T;
+      if (cap < 1) cap = 1;
+      if (cap > kMaxPoolNodes) return OIT_ERR_CAPACITY;
+      c->ll_cap = cap;
+      rc = dev_alloc(&c->ll_head, T);
+      if (!rc) rc = dev_alloc(&c->ll_cd, cap);
+      if (!rc) rc = dev_alloc(&c->ll_next, cap);
+      c->bytes_abuf = cap * 12u;
+      c->bytes_aux = T * 4u;
+      break;
+    }
+    case OIT_SIMPLE:
+    case OIT_SPINLOCK: {  // oit.cpp:102-118
+      rc = dev_alloc(&c->kb_count, T);
+      if (!rc) rc = dev_alloc(&c->kb_col, T * K);
+      if (!rc) rc = dev_alloc(&c->kb_key, T * K);
+      if (!rc) rc = dev_alloc(&c->kb_id, T * K);
+      if (!rc && alg == OIT_SPINLOCK) rc = dev_alloc(&c->kb_lock, T);
+      if (!rc && alg == OIT_SPINLOCK) rc = dev_alloc(&c->kb_depth, T);
+      c->bytes_abuf = T * K * 12u;
+      c->bytes_aux = T * (alg == OIT_SPINLOCK ? 12u : 4u);
+      break;
+    }
+    case OIT_LOOP64: {  // oit.cpp:134-140
+      rc = dev_alloc(&c->l64, T * K);
+      c->bytes_abuf = T * K * 8u;
+      break;
+    }
+    case OIT_WEIGHTED: {  // oit.h:234-235 (binary32 here)
+      rc = dev_alloc(&c->wb_accum, T);
+      if (!rc) rc = dev_alloc(&c->wb_reveal, T);
+      c->bytes_abuf = T * 20u;
+      break;
+    }
+  }
+  if (!rc && uses_tail_lists(c)) {
+    rc = dev_alloc(&c->tl_head, T);
+    c->tl_head_valid = false;
+    if (!rc && cfg->max_fragments) {
+      const uint64_t cap = cfg->max_fragments < kMaxFrameFragments ? cfg->max_fragments : kMaxFrameFragments;
+      rc = dev_alloc(&c->tl_next, cap);
+      if (!rc) rc = dev_alloc(&c->tl_rgba, cap);
+      if (!rc) rc = dev_alloc(&c->tl_rec, cap);
+      if (!rc) c->tl_cap = (uint32_t)cap;
+    }
+  }
+  if (rc) {
+    free_frame_buffers(c);
+    return rc;
+  }
+  c->configured = true;
+  c->frame_open = false;
+  c->resolved = false;
+  c->frame_frags = 0;
+  c->appends = 0;
+  c->segments.clear();
+  return OIT_OK;
+}
+
+int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
+  if (!c || !out) return OIT_ERR_INVALID_ARG;
+  if (!c->configured) return OIT_ERR_STATE;
+  std::memset(out, 0, sizeof *out);
+  out->abuffer_bytes = c->bytes_abuf;
+  out->aux_bytes = c->bytes_aux + (c->tl_head ? (uint64_t)c->T * 4u : 0u);
+  out->tail_bytes = (uint64_t)c->tl_cap * (4u + 16u + 8u);
+  out->total_bytes = out->abuffer_bytes + out->aux_bytes + out->tail_bytes;
+  out->ll_capacity = c->ll_cap;
+  switch (c->cfg.algorithm) {
+    case OIT_LINKEDLIST: out->bytes_per_element = 12; break;
+    case OIT_SIMPLE:
+    case OIT_SPINLOCK: out->bytes_per_element = 12; break;
+    case OIT_LOOP64: out->bytes_per_element = 8; break;
+    default: out->bytes_per_element = 20; break;
+  }
+  out->targets = c->T;
+  return OIT_OK;
+}
+
+int oit_clear(oit_ctx* c, void* stream) {
+  if (!c) return OIT_ERR_INVALID_ARG;
+  if (!c->configured) return OIT_ERR_STATE;
+  DeviceGuard g(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t T = c->T, K = c->cfg.layers;
+  hipError_t e = staged(c, kStClear, s, [&]() -> hipError_t {
+    hipError_t r = oit::launch_fill_u32(c->d_counters, 16, 0u, s);
+    switch (c->cfg.algorithm) {
+      case OIT_LINKEDLIST:  // head and counter to 0, oitRender.cpp:206-212
+        if (r == hipSuccess) r = oit::launch_fill_u32(c->ll_head, T, 0u, s);
+        break;
+      case OIT_SIMPLE:  // count to 0, oitRender.cpp:161-169
+        if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_count, T, 0u, s);
+        break;
+      case OIT_SPINLOCK:  // count, spin to 0, depth to ~0, oitRender.cpp:343-354
+        if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_count, T, 0u, s);
+        if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_lock, T, 0u, s);
+        if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_depth, T, kEmpty, s);
+        break;
+      case OIT_LOOP64:  // every byte to 0xFF, oitRender.cpp:307
+        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->l64), 2 * T * K, kEmpty, s);
+        break;
+      case OIT_WEIGHTED:  // accum 0, reveal 1, oitRender.cpp:394-397
+        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_accum), 4 * T, 0u, s);
+        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_reveal), T, 0x3F800000u, s);
+        break;
+    }
+    if (r == hipSuccess && c->tl_head) r = oit::launch_fill_u32(c->tl_head, T, kEmpty, s);
+    return r;
+  });
+  c->tl_head_valid = c->tl_head != nullptr;
+  c->frame_frags = 0;
+  c->appends = 0;
+  c->segments.clear();
+  c->frame_open = e == hipSuccess;
+  c->resolved = false;
+  return status_of(e);
+}
+
+int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
+  if (!c || !f) return OIT_ERR_INVALID_ARG;
+  if (!c->configured) return OIT_ERR_STATE;
+  if (!c->frame_open || c->resolved) return OIT_ERR_STATE;
+  if (f->count == 0) return OIT_OK;
+  if (!f->target || !f->depth || !f->rgba) return OIT_ERR_INVALID_ARG;
+  if (f->count > kMaxFrameFragments || c->frame_frags + f->count > kMaxFrameFragments) return OIT_ERR_CAPACITY;
+  DeviceGuard g(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint64_t end = c->frame_frags + f->count;
+  const uint32_t alg = c->cfg.algorithm;
+  if (uses_tail_lists(c)) {
+    const bool need = alg != OIT_LINKEDLIST || end + 1 > c->ll_cap;  // LL: only when the pool can overflow
+    if (need) {
+      int rc = ensure_tail_capacity(c, end, s);
+      if (rc) return rc;
+    }
+  }
+  oit::FragView v;
+  v.target = f->target;
+  v.depth = f->depth;
+  v.rgba = reinterpret_cast<const float4*>(f->rgba);
+  v.count = (uint32_t)f->count;
+  v.id_base = (uint32_t)c->frame_frags;
+  const oit::Common k = common_of(c);
+  const oit::TailBuf tb = tail_of(c);
+  hipError_t e = hipSuccess;
+  switch (alg) {
+    case OIT_LINKEDLIST: {
+      oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1};
+      e = staged(c, kStStore, s, [&] { return oit::launch_ll_store(k, v, ll, tb, s); });
+      if (e == hipSuccess) e = staged(c, kStLink, s, [&] { return oit::launch_ll_link(k, v, ll, s); });
+      break;
+    }
+    case OIT_SIMPLE: {
+      oit::KBuf kb{c->kb_count, nullptr, nullptr, c->kb_col, c->kb_key, c->kb_id};
+      e = staged(c, kStStore, s, [&] { return oit::launch_simple_store(k, v, kb, tb, s); });
+      break;
+    }
+    case OIT_SPINLOCK: {
+      oit::KBuf kb{c->kb_count, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
+      e = staged(c, kStStore, s, [&] { return oit::launch_spin_store(k, v, kb, tb, s); });
+      break;
+    }
+    case OIT_LOOP64: {
+      e = staged(c, kStStore, s, [&] { return oit::launch_loop64_store(k, v, c->l64, s); });
+      if (c->cfg.tail_blend) c->segments.push_back(v);
+      break;
+    }
+    case OIT_WEIGHTED: {
+      oit::WBBuf wb{c->wb_accum, c->wb_reveal};
+      e = staged(c, kStStore, s, [&] {
+        return oit::launch_wboit_store(k, v, wb, c->cfg.wboit_view_near, c->cfg.wboit_view_span, s);
+      });
+      break;
+    }
+  }
+  if (e != hipSuccess) return status_of(e);
+  c->frame_frags = end;
+  c->appends++;
+  return OIT_OK;
+}
+
+int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* fk_val, uint32_t* counts,
+                      void* stream) {
+  if (!c || !out_rgba) return OIT_ERR_INVALID_ARG;
+  if (!c->configured || !c->frame_open) return OIT_ERR_STATE;
+  DeviceGuard g(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const uint32_t alg = c->cfg.algorithm;
+  const bool link_tails = !c->resolved;
+  if (link_tails && alg == OIT_LOOP64 && c->cfg.tail_blend && !c->segments.empty()) {
+    int rc = ensure_tail_capacity(c, c->frame_frags, s);
+    if (rc) return rc;
+  }
+  const oit::Common k = common_of(c);
+  const oit::TailBuf tb = tail_of(c);
+  oit::ResolveOut o{reinterpret_cast<float4*>(out_rgba), fk_key, fk_val, counts};
+  hipError_t e = hipSuccess;
+  if (link_tails && alg == OIT_LOOP64 && c->cfg.tail_blend && tb.head) {
+    for (const auto& v : c->segments) {
+      e = staged(c, kStClassify, s, [&] { return oit::launch_loop64_classify(k, v, c->l64, tb, s); });
+      if (e != hipSuccess) return status_of(e);
+    }
+  }
+  if (link_tails && tb.head && tb.next) {
+    e = staged(c, kStTailLink, s, [&] { return oit::launch_tail_link(tb, s); });
+    if (e != hipSuccess) return status_of(e);
+  }
+  c->resolved = true;
+  oit::TailBuf tbr = tb;
+  if (!tb.next) tbr.head = nullptr;  // no tail records can exist
+  switch (alg) {
+    case OIT_LINKEDLIST: {
+      oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1};
+      e = staged(c, kStResolve, s, [&] { return oit::launch_ll_resolve(k, ll, tbr, o, s); });
+      break;
+    }
+    case OIT_SIMPLE:
+    case OIT_SPINLOCK: {
+      oit::KBuf kb{c->kb_count, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
+      e = staged(c, kStResolve, s, [&] { return oit::launch_kbuf_resolve(k, kb, tbr, o, s); });
+      break;
+    }
+    case OIT_LOOP64:
+      e = staged(c, kStResolve, s, [&] { return oit::launch_loop64_resolve(k, c->l64, tbr, o, s); });
+      break;
+    case OIT_WEIGHTED: {
+      oit::WBBuf wb{c->wb_accum, c->wb_reveal};
+      e = staged(c, kStResolve, s, [&] { return oit::launch_wboit_resolve(k, wb, o, s); });
+      break;
+    }
+  }
+  return status_of(e);
+}
+
+int oit_resolve(oit_ctx* c, float* out_rgba, void* stream) {
+  return oit_resolve_debug(c, out_rgba, nullptr, nullptr, nullptr, stream);
+}
+
+int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
+  if (!c || !out) return OIT_ERR_INVALID_ARG;
+  DeviceGuard g(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  uint32_t h[16] = {0};
+  if (hipMemcpyAsync(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess) return OIT_ERR_DEVICE;
+  if (hipStreamSynchronize(s) != hipSuccess) return OIT_ERR_DEVICE;
+  std::memset(out, 0, sizeof *out);
+  out->fragments = c->frame_frags;
+  out->ll_overflow = h[1];
+  out->tail_fragments = h[0];
+  out->appends = c->appends;
+  return OIT_OK;
+}
+
+int oit_profile_enable(oit_ctx* c, int enable) {
+  if (!c) return OIT_ERR_INVALID_ARG;
+  c->profiling = enable != 0;
+  return OIT_OK;
+}
+
+int oit_profile_read(oit_ctx* c, oit_profile* out) {
+  if (!c || !out) return OIT_ERR_INVALID_ARG;
+  DeviceGuard g(c->device);
+  for (auto& p : c->events) {
+    if (hipEventSynchronize(p.b) != hipSuccess) return OIT_ERR_DEVICE;
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, p.a, p.b) == hipSuccess) {
+      c->prof_ms[p.stage] += ms;
+      c->prof_launches[p.stage] += 1;
+    }
+    c->event_pool.push_back(p.a);
+    c->event_pool.push_back(p.b);
+  }
+  c->events.clear();
+  std::memset(out, 0, sizeof *out);
+  for (int i = 0; i < kStCount && i < OIT_PROFILE_STAGES; ++i) {
+    out->ms[i] = c->prof_ms[i];
+    out->launches[i] = c->prof_launches[i];
+  }
+  return OIT_OK;
+}
+
+int oit_profile_reset(oit_ctx* c) {
+  if (!c) return OIT_ERR_INVALID_ARG;
+  oit_profile tmp;
+  int rc = oit_profile_read(c, &tmp);
+  for (int i = 0; i < kStCount; ++i) {
+    c->prof_ms[i] = 0;
+    c->prof_launches[i] = 0;
+  }
+  return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,214 @@
+// oit_device.hpp - device-side building blocks of the gfx950 OIT kernels:
+// depth keys, the sRGB8 colour codec, premultiplied "over", wave-level
+// aggregation and the lock-free id-ordered per-target lists.
+//
+// Every kernel translation unit is compiled with -ffp-contract=off, so each
+// float expression below rounds once per operation, exactly like the
+// single-threaded restatement the parity tests compare against.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace oit {
+
+constexpr uint32_t kEmpty = 0xFFFFFFFFu;      // empty head / slot / list end
+constexpr uint32_t kUnset = 0xFFFFFFFEu;      // link word not yet written during a list build
+constexpr uint64_t kEmpty64 = 0xFFFFFFFFFFFFFFFFull;
+
+// Colour codec tables, filled on the host at context creation.
+//  thr[q]  (1 <= q <= 255): smallest binary32 c whose sRGB8 code is >= q,
+//          for the curve of unPremultLinearToSRGB (shaderCommon.glsl:60-69)
+//          followed by packUnorm4x8 rounding; thr[0] = -inf, thr[256] = +inf.
+//  dec[u]: correctly rounded unPremultSRGBToLinear(u / 255) (shaderCommon.glsl:84-94).
+//  alpha[u]: u / 255 (unpackUnorm4x8).
+struct CodecTables {
+  float thr[257];
+  float dec[256];
+  float alpha[256];
+};
+
+// Order-preserving u32 key of a binary32 depth. floatBitsToUint (used by
+// oitLoop64.frag.glsl:79 and oitSpinlock.frag.glsl:60,68) orders only
+// non-negative depths; this key orders every float and never produces the
+// empty sentinel 0xFFFFFFFF (oitRender.cpp:307).
+__device__ __forceinline__ uint32_t depth_key(float d) {
+  const uint32_t b = __float_as_uint(d);
+  const uint32_t k = (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+  return k == kEmpty ? 0xFFFFFFFEu : k;
+}
+
+// sRGB8 code of an unpremultiplied linear channel: the unique q with
+// thr[q] <= c < thr[q+1]. A fast approximation of the curve picks the start,
+// the table makes it exact (NaN and negatives give 0).
+__device__ __forceinline__ uint32_t code_rgb(float c, const float* __restrict__ thr) {
+  float s;
+  if (c < 0.0031308f) {
+    s = c * 12.92f;
+  } else {
+    s = 1.055f * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(c) * (1.0f / 2.4f)) - 0.055f;
+  }
+  int q = (int)fminf(fmaxf(s * 255.0f + 0.5f, 0.0f), 255.0f);
+  while (q > 0 && !(c >= thr[q])) --q;
+  while (q < 255 && c >= thr[q + 1]) ++q;
+  return (uint32_t)q;
+}
+
+// shading() -> unPremultLinearToSRGB -> packUnorm4x8 (oitLinkedList.frag.glsl:
+// 57,75-77). The fragment stream carries premultiplied colour, so rgb is
+// divided by alpha first. Alpha: floor(clamp(a,0,1) * 255 + 0.5).
+__device__ __forceinline__ uint32_t encode_color(float4 p, const CodecTables* __restrict__ tab) {
+  const float a = p.w;
+  float r = 0.0f, g = 0.0f, b = 0.0f;
+  if (a > 0.0f) {
+    r = p.x / a;
+    g = p.y / a;
+    b = p.z / a;
+  }
+  const float ac = fminf(fmaxf(a, 0.0f), 1.0f);
+  const uint32_t qa = (uint32_t)floorf(ac * 255.0f + 0.5f);
+  return code_rgb(r, tab->thr) | (code_rgb(g, tab->thr) << 8) | (code_rgb(b, tab->thr) << 16) | (qa << 24);
+}
+
+// unpackUnorm4x8 -> unPremultSRGBToLinear -> premultiply
+// (first half of doBlendPacked, shaderCommon.glsl:117-124).
+__device__ __forceinline__ float4 decode_premult(uint32_t c, const CodecTables* __restrict__ tab) {
+  const float a = tab->alpha[c >> 24];
+  return make_float4(tab->dec[c & 255u] * a, tab->dec[(c >> 8) & 255u] * a, tab->dec[(c >> 16) & 255u] * a, a);
+}
+
+// top over base: doBlend (shaderCommon.glsl:108-112), which is also the
+// PREMULTIPLIED framebuffer blend dst = src + (1 - src.a) * dst
+// (main.cpp:548-558, oit.h:55) with src = top, dst = base.
+__device__ __forceinline__ float4 over(float4 top, float4 base) {
+  const float t = 1.0f - top.w;
+  return make_float4(top.x + t * base.x, top.y + t * base.y, top.z + t * base.z, top.w + t * base.w);
+}
+
+// --------------------------------------------------------------------------
+// Wave helpers (64-wide wavefronts)
+// --------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t lane_id() {
+  return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
+}
+
+// Number of set bits of `mask` below this lane.
+__device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+// Wave-aggregated allocation: every active lane with `want` gets a unique
+// index from *counter with ONE atomicAdd per wave (ballot + mbcnt).
+// Must be reached by all active lanes of the wave.
+__device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, bool want) {
+  const uint64_t mask = __ballot(want);
+  uint32_t base = 0;
+  if (mask) {
+    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
+    if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__builtin_popcountll(mask));
+    base = __shfl(base, (int)leader);
+  }
+  return base + lanes_below(mask);
+}
+
+// --------------------------------------------------------------------------
+// Relaxed agent-scope atomics (coherent across XCDs, no L2 writeback)
+// --------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t ld_relaxed(const uint32_t* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(uint32_t* p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t max_relaxed(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_max(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint32_t min_relaxed(uint32_t* p, uint32_t v) {
+  return __hip_atomic_fetch_min(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// --------------------------------------------------------------------------
+// Id-ordered lock-free lists
+//
+// A list is a chain of link words: the head word per target, then next[x]
+// for each element x. Element values are ordered (descending for the
+// Linked List, whose walk order is newest first as in the reference's
+// head-exchange lists, oitLinkedList.frag.glsl:72; ascending for the
+// colour-pass tail lists, which blend oldest first). Insertion of a block
+// of consecutive values [lo, hi] (a run of same-target lanes):
+//   old = atomic_max/min(link, hi/lo)  -- publishes the block
+//   if old is beyond the block: next[last of block] = old  (one store)
+//   else: walk to the successor of `old` and retry there.
+// next[] words of elements that are about to be inserted hold kUnset, written
+// by an EARLIER kernel; a walker that reaches a published element whose link
+// is still kUnset waits one loop trip for the owner's store. Only relaxed
+// atomics are used: no release fences (which would write back L2) are needed
+// because no other data is read during the build.
+// The loop is written so that one trip advances every active lane by one
+// step; a lane's store of its own link happens in the same trip as its
+// successful atomic, so lanes of one wave never wait on each other forever.
+// --------------------------------------------------------------------------
+
+// Descending list (head holds the largest value, 0 terminates).
+// Inserts block [lo, hi] (hi >= lo, consecutive values already chained
+// hi -> hi-1 -> ... -> lo by the caller) at link word `p0`.
+__device__ __forceinline__ void list_insert_desc(uint32_t* p0, uint32_t* next, uint32_t lo, uint32_t hi) {
+  uint32_t* p = p0;
+  bool first = true;
+  for (;;) {
+    uint32_t v = 0;
+    bool try_max = first;
+    if (!first) {
+      v = ld_relaxed(p);
+      if (v != kUnset) {
+        if (v > hi) {
+          p = &next[v];
+        } else {
+          try_max = true;
+        }
+      }
+    }
+    first = false;
+    if (try_max) {
+      const uint32_t old = max_relaxed(p, hi);
+      if (old < lo) {
+        st_relaxed(&next[lo], old);
+        return;
+      }
+      p = &next[old];  // a larger element got there first: continue behind it
+    }
+  }
+}
+
+// Ascending list (head holds the smallest value, kEmpty terminates).
+// Inserts the single value x at link word p0.
+__device__ __forceinline__ void list_insert_asc(uint32_t* p0, uint32_t* next, uint32_t x, uint32_t next_index) {
+  uint32_t* p = p0;
+  bool first = true;
+  for (;;) {
+    bool try_min = first;
+    if (!first) {
+      const uint32_t v = ld_relaxed(p);
+      if (v != kUnset) {
+        if (v < x) {
+          p = &next[v >> 1];
+        } else {
+          try_min = true;
+        }
+      }
+    }
+    first = false;
+    if (try_min) {
+      const uint32_t old = min_relaxed(p, x);
+      if (old > x) {
+        st_relaxed(&next[next_index], old);
+        return;
+      }
+      p = &next[old >> 1];
+    }
+  }
+}
+
+}  // namespace oit

@@ -1,0 +1,716 @@
+// oit_kernels.hip - gfx950 kernels of the OIT fragment-store and composite
+// passes. One thread per fragment for the store passes, one thread per
+// target (pixel or sample) for the composite passes; 256-thread blocks of
+// four 64-wide waves.
+//
+// Built with: hipcc --offload-arch=gfx950 -O3 -ffp-contract=off
+//             -Xclang -target-feature -Xclang -packed-fp32-ops
+#include "oit_kernels.hpp"
+
+namespace oit {
+
+namespace {
+
+constexpr uint32_t kBlock = 256;
+
+inline uint32_t blocks_for(uint64_t n) { return (uint32_t)((n + kBlock - 1) / kBlock); }
+
+// A run of consecutive lanes that carry the same target. Fragment ids grow
+// with the lane, so a run is a block of consecutive ids of one target, and
+// no fragment outside the wave can have an id inside it.
+struct Run {
+  uint32_t lane;
+  uint32_t start_lane;
+  uint32_t end_lane;
+  bool start;
+  bool end;
+};
+
+// Must be reached by all 64 lanes (callers keep every lane alive and give
+// lanes without a fragment a target of kEmpty).
+__device__ __forceinline__ Run same_target_run(uint32_t t) {
+  Run r;
+  r.lane = lane_id();
+  const uint32_t tp = __shfl_up(t, 1u);
+  const uint32_t tn = __shfl_down(t, 1u);
+  r.start = (r.lane == 0) || (tp != t);
+  r.end = (r.lane == 63) || (tn != t);
+  const uint64_t starts = __ballot(r.start);
+  const uint64_t ends = __ballot(r.end);
+  const uint64_t le = (2ull << r.lane) - 1ull;  // lanes <= this lane
+  r.start_lane = 63u - (uint32_t)__builtin_clzll(starts & le);
+  r.end_lane = r.lane + (uint32_t)__builtin_ctzll(ends >> r.lane);
+  return r;
+}
+
+// Records a colour-pass tail event for frame-local fragment `lid` (phase 1 of
+// the tail lists): payload colour, kUnset link, and a compact record whose
+// slot comes from one atomicAdd per wave. Callable from divergent code.
+__device__ __forceinline__ void tail_record(bool want, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
+                                            const TailBuf& tb) {
+  const uint32_t m = wave_alloc(tb.counter, want);
+  if (want && m < tb.cap && lid < tb.cap) {
+    tb.rgba[lid] = payload;
+    tb.next[lid] = kUnset;
+    tb.rec[m] = ((uint64_t)t << 32) | ((lid << 1) | flag);
+  }
+}
+
+// Sorted register array of K (sort key, colour) pairs, ascending; inserting x
+// returns the element pushed out of the back (x itself if it does not fit):
+// insertionSortTail, oitCompositeDefines.glsl:116-139, unrolled into selects.
+template <int K>
+struct FrontK {
+  uint64_t sk[K];
+  uint32_t col[K];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      sk[k] = kEmpty64;
+      col[k] = 0u;
+    }
+  }
+  __device__ __forceinline__ void insert(uint64_t x, uint32_t xc, uint64_t& ev, uint32_t& evc) {
+    const bool fits = x < sk[K - 1];
+    ev = fits ? sk[K - 1] : x;
+    evc = fits ? col[K - 1] : xc;
+#pragma unroll
+    for (int j = K - 1; j > 0; --j) {
+      const bool before = x < sk[j - 1];
+      const bool here = x < sk[j];
+      sk[j] = before ? sk[j - 1] : (here ? x : sk[j]);
+      col[j] = before ? col[j - 1] : (here ? xc : col[j]);
+    }
+    const bool first = x < sk[0];
+    sk[0] = first ? x : sk[0];
+    col[0] = first ? xc : col[0];
+  }
+};
+
+// Front-to-back blend of the first n sorted packed colours
+// (oitLinkedList.frag.glsl:162-166, oitSimple.frag.glsl:161-165).
+template <int K>
+__device__ __forceinline__ float4 blend_front(const FrontK<K>& a, uint32_t n, const CodecTables* tab) {
+  float4 c = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((uint32_t)k < n) c = over(c, decode_premult(a.col[k], tab));
+  return c;
+}
+
+// Colour buffer after the colour pass: the tail fragments of target t blended
+// over the background in ascending id (the ROP's primitive order,
+// main.cpp:548-558). `skip` boundary-flagged entries are in the A-buffer and
+// not blended (Loop64 identical keys, see k_loop64_classify).
+__device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float4 bg, uint32_t skip) {
+  float4 d = bg;
+  uint32_t link = tb.head[t];
+  while (link != kEmpty) {
+    const uint32_t id = link >> 1;
+    const uint32_t nx = tb.next[id];
+    if ((link & 1u) && skip > 0u) {
+      --skip;
+    } else {
+      d = over(tb.rgba[id], d);
+    }
+    link = nx;
+  }
+  return d;
+}
+
+template <int K>
+__device__ __forceinline__ void write_debug(const ResolveOut& o, uint32_t t, const FrontK<K>& a, uint32_t n,
+                                            uint32_t count, bool ids_from_nodes, bool vals_are_colors) {
+  if (o.fk_key) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) o.fk_key[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)(a.sk[k] >> 32) : kEmpty;
+  }
+  if (o.fk_val) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      uint32_t v = kEmpty;
+      if ((uint32_t)k < n) {
+        v = vals_are_colors ? a.col[k] : (uint32_t)a.sk[k];
+        if (ids_from_nodes) v -= 1u;  // node n holds fragment id n - 1
+      }
+      o.fk_val[(uint64_t)t * K + k] = v;
+    }
+  }
+  if (o.counts) o.counts[t] = count;
+}
+
+// ---------------------------------------------------------------------------
+// Fill
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, uint64_t n, uint32_t v) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t n4 = n / 4;
+  uint4* p4 = reinterpret_cast<uint4*>(p);
+  const uint4 v4 = make_uint4(v, v, v, v);
+  for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += stride) p4[i] = v4;
+  for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Linked List: oitLinkedList.frag.glsl
+// ---------------------------------------------------------------------------
+
+// Store half of the colour pass (oitLinkedList.frag.glsl:51-85 minus the head
+// exchange): node n = id + 1 (the reference's counter, :55, advances once per
+// fragment in draw order; here the allocation is the fragment's own index, so
+// no global counter is contended), overflow -> tail (:59-68), node payload
+// (:75-82). The link word is set to kUnset for the link kernel.
+template <bool kTail>
+__global__ __launch_bounds__(kBlock) void k_ll_store(Common c, FragView f, LLBuf ll, TailBuf tb) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const bool valid = i < f.count;
+  const uint32_t lid = f.id_base + i;
+  const uint64_t n = (uint64_t)lid + 1u;
+  bool ovf = false;
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t t = kEmpty;
+  if (valid) {
+    col = f.rgba[i];
+    if (n >= ll.cap) {
+      t = f.target[i];
+      ovf = t < c.T;
+    } else {
+      const uint32_t key = depth_key(f.depth[i]);
+      ll.cd[n] = ((uint64_t)key << 32) | encode_color(col, c.tab);
+      ll.next[n] = kUnset;
+    }
+  }
+  const uint64_t om = __ballot(ovf);
+  if (om && lane_id() == (uint32_t)__builtin_ctzll(om)) atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
+  if constexpr (kTail) tail_record(ovf, t, lid, 0u, col, tb);
+}
+
+// Link half: the head exchange of oitLinkedList.frag.glsl:72, done as an
+// ordered insert so that every list is in descending id (newest first: the
+// order a sequential draw produces). Same-target runs in a wave are chained
+// locally and inserted with one atomic.
+__global__ __launch_bounds__(kBlock) void k_ll_link(Common c, FragView f, LLBuf ll) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t n64 = (uint64_t)f.id_base + i + 1u;
+  bool valid = i < f.count && n64 < ll.cap;
+  uint32_t t = valid ? f.target[i] : kEmpty;
+  if (t >= c.T) {
+    valid = false;
+    t = kEmpty;
+  }
+  const Run r = same_target_run(t);
+  if (!valid) return;
+  const uint32_t n = (uint32_t)n64;
+  // chain inside the run; the run start lane's link is written by the insert
+  if (!r.start) st_relaxed(&ll.next[n], n - 1u);
+  if (r.end) list_insert_desc(&ll.head[t], ll.next, n - (r.lane - r.start_lane), n);
+}
+
+// Composite pass, oitLinkedList.frag.glsl:106-172: walk the list (newest
+// first), keep the K nearest in registers, tail-blend the ones pushed out
+// (:131-143), blend front to back (:162-166), over the tail (:170), then the
+// PREMULTIPLIED framebuffer blend onto the colour buffer (oit.cpp:723-726).
+template <int K, bool kTail, bool kColorTail, bool kDebug>
+__global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailBuf tb, ResolveOut o) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= c.T) return;
+  FrontK<K> a;
+  a.init();
+  float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t p = ll.head[t];
+  uint32_t cnt = 0;
+  while (p != 0u) {
+    const uint64_t cd = ll.cd[p];
+    const uint32_t nx = ll.next[p];
+    const uint64_t x = (cd & 0xFFFFFFFF00000000ull) | p;  // (depth key, id + 1)
+    uint64_t ev;
+    uint32_t evc;
+    a.insert(x, (uint32_t)cd, ev, evc);
+    if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult(evc, c.tab));
+    ++cnt;
+    p = nx;
+  }
+  const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
+  float4 col = blend_front<K>(a, n, c.tab);
+  col = over(col, tailc);
+  float4 d = c.background;
+  if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+  o.out[t] = over(col, d);
+  if (kDebug) write_debug<K>(o, t, a, n, cnt, true, false);
+}
+
+// ---------------------------------------------------------------------------
+// Simple: oitSimple.frag.glsl
+// ---------------------------------------------------------------------------
+
+// Colour pass (oitSimple.frag.glsl:50-91): slot = atomicAdd(count), slot < K
+// stores (colour, key, id) in layer `slot`, else the fragment tail-blends.
+// A run of same-target lanes takes its slots with one atomicAdd.
+template <bool kTail>
+__global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, KBuf kb, TailBuf tb) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool valid = i < f.count;
+  uint32_t t = valid ? f.target[i] : kEmpty;
+  if (t >= c.T) {
+    valid = false;
+    t = kEmpty;
+  }
+  const uint32_t lid = f.id_base + i;
+  const Run r = same_target_run(t);
+  uint32_t base = 0;
+  if (valid && r.start) base = atomicAdd(&kb.count[t], r.end_lane - r.lane + 1u);
+  base = __shfl(base, (int)r.start_lane);
+  const uint32_t slot = base + (r.lane - r.start_lane);
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+  bool tail = false;
+  if (valid) {
+    col = f.rgba[i];
+    if (slot < c.K) {
+      const uint64_t idx = (uint64_t)slot * c.T + t;
+      kb.col[idx] = encode_color(col, c.tab);
+      kb.key[idx] = depth_key(f.depth[i]);
+      kb.id[idx] = lid;
+    } else {
+      tail = true;
+    }
+  }
+  if constexpr (kTail) tail_record(tail, t, lid, 0u, col, tb);
+}
+
+// ---------------------------------------------------------------------------
+// Spinlock: oitSpinlock.frag.glsl
+// ---------------------------------------------------------------------------
+
+// Colour pass (oitSpinlock.frag.glsl:49-129). The critical section runs under
+// a per-target lock taken with an atomic exchange inside a `done` loop, the
+// shape that lets one lane of a wave hold the lock while its neighbours
+// retry on the next trip. Elements are ordered by (depth key, id); the
+// furthest is found by a scan with first-index-wins (:96-107), replaced when
+// the newcomer is nearer (:109-117). Evicted and rejected fragments are
+// recorded for the tail with their packed sRGB8 colour.
+template <int K, bool kTail>
+__global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBuf kb, TailBuf tb) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool valid = i < f.count;
+  uint32_t t = valid ? f.target[i] : kEmpty;
+  if (t >= c.T) valid = false;
+  const uint32_t lid = f.id_base + i;
+  bool tail = false;
+  uint32_t tail_id = lid;
+  uint32_t tail_col = 0u;
+  if (valid) {
+    const uint32_t key = depth_key(f.depth[i]);
+    const uint32_t col = encode_color(f.rgba[i], c.tab);
+    tail_col = col;
+    if (key <= ld_relaxed(&kb.depth[t])) {  // :66-69 early depth test
+      bool done = false;
+      while (!done) {
+        if (__hip_atomic_exchange(&kb.lock[t], 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+          const uint32_t cnt = ld_relaxed(&kb.count[t]);  // :86-87
+          st_relaxed(&kb.count[t], cnt + 1u);
+          if (cnt < (uint32_t)K) {                          // :89-93
+            const uint64_t idx = (uint64_t)cnt * c.T + t;
+            st_relaxed(&kb.col[idx], col);
+            st_relaxed(&kb.key[idx], key);
+            st_relaxed(&kb.id[idx], lid);
+          } else {
+            uint32_t fur = 0;                               // :96-107
+            uint64_t mx = 0;
+#pragma unroll
+            for (int j = 0; j < K; ++j) {
+              const uint64_t idx = (uint64_t)j * c.T + t;
+              const uint64_t v = ((uint64_t)ld_relaxed(&kb.key[idx]) << 32) | ld_relaxed(&kb.id[idx]);
+              if (v > mx) {
+                mx = v;
+                fur = (uint32_t)j;
+              }
+            }
+            const uint64_t me = ((uint64_t)key << 32) | lid;
+            if (mx > me) {                                  // :109-117
+              const uint64_t idx = (uint64_t)fur * c.T + t;
+              tail_col = ld_relaxed(&kb.col[idx]);
+              tail_id = (uint32_t)mx;
+              st_relaxed(&kb.col[idx], col);
+              st_relaxed(&kb.key[idx], key);
+              st_relaxed(&kb.id[idx], lid);
+              st_relaxed(&kb.depth[t], (uint32_t)(mx >> 32));
+            }
+            tail = true;
+          }
+          __hip_atomic_exchange(&kb.lock[t], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+          done = true;
+        }
+      }
+    } else {
+      tail = true;
+    }
+  }
+  if constexpr (kTail) {
+    float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (tail) payload = decode_premult(tail_col, c.tab);
+    tail_record(tail, valid ? t : kEmpty, tail_id, 0u, payload, tb);
+  }
+}
+
+// Composite of Simple and Spinlock (oitSimple.frag.glsl:115-169 ==
+// oitSpinlock.frag.glsl:153-207): n = min(count, K) elements, sorted,
+// blended front to back, over the colour buffer.
+template <int K, bool kColorTail, bool kDebug>
+__global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, TailBuf tb, ResolveOut o) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= c.T) return;
+  const uint32_t count = kb.count[t];
+  const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
+  FrontK<K> a;
+  a.init();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if ((uint32_t)k < n) {
+      const uint64_t idx = (uint64_t)k * c.T + t;
+      uint64_t ev;
+      uint32_t evc;
+      a.insert(((uint64_t)kb.key[idx] << 32) | kb.id[idx], kb.col[idx], ev, evc);
+    }
+  }
+  const float4 col = blend_front<K>(a, n, c.tab);
+  float4 d = c.background;
+  if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+  o.out[t] = over(col, d);
+  if (kDebug) write_debug<K>(o, t, a, n, count, false, false);
+}
+
+// ---------------------------------------------------------------------------
+// Loop64: oitLoop64.frag.glsl
+// ---------------------------------------------------------------------------
+
+// Colour pass (oitLoop64.frag.glsl:65-141): key = depth << 32 | colour (:79),
+// the two pretests of USE_EARLYDEPTH (:82-101, plain loads: slots only
+// decrease within a frame, so a stale value only lets more fragments try),
+// then the atomicMin chain (:109-121). Slots end as the K smallest keys for
+// any arrival order; evictions are classified afterwards.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, uint64_t* __restrict__ slots) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= f.count) return;
+  const uint32_t t = f.target[i];
+  if (t >= c.T) return;
+  uint64_t z = ((uint64_t)depth_key(f.depth[i]) << 32) | encode_color(f.rgba[i], c.tab);
+  if (z > slots[(uint64_t)(K - 1) * c.T + t]) return;
+  int j = 0;
+  if (z > slots[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
+  for (; j < K; ++j) {
+    const uint64_t old =
+        __hip_atomic_fetch_min(&slots[(uint64_t)j * c.T + t], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == kEmpty64) break;
+    z = old > z ? old : z;
+  }
+}
+
+// Tail classification: a fragment whose key is above the final K-th key was
+// evicted or rejected (the colours oitLoop64.frag.glsl:131-136 tail-blends);
+// one equal to it is "boundary": identical (depth, colour) keys are the same
+// bits in the A-buffer, and the resolve keeps the lowest ids of them in front.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f, const uint64_t* __restrict__ slots,
+                                                            TailBuf tb) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool valid = i < f.count;
+  uint32_t t = valid ? f.target[i] : kEmpty;
+  if (t >= c.T) valid = false;
+  bool tail = false;
+  uint32_t flag = 0;
+  float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid) {
+    const uint32_t col = encode_color(f.rgba[i], c.tab);
+    const uint64_t z = ((uint64_t)depth_key(f.depth[i]) << 32) | col;
+    const uint64_t s = slots[(uint64_t)(K - 1) * c.T + t];
+    if (z >= s) {
+      tail = true;
+      flag = z == s ? 1u : 0u;
+      payload = decode_premult(col, c.tab);
+    }
+  }
+  tail_record(tail, t, f.id_base + i, flag, payload, tb);
+}
+
+// Composite (oitLoop64.frag.glsl:162-183): blend slots front to back until
+// the first empty one.
+template <int K, bool kColorTail, bool kDebug>
+__global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint64_t* __restrict__ slots, TailBuf tb,
+                                                           ResolveOut o) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= c.T) return;
+  uint64_t s[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) s[k] = slots[(uint64_t)k * c.T + t];
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t n = 0;
+  bool alive = true;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    alive = alive && (uint32_t)(s[k] >> 32) != kEmpty;
+    if (alive) {
+      col = over(col, decode_premult((uint32_t)s[k], c.tab));
+      ++n;
+    }
+  }
+  float4 d = c.background;
+  if (kColorTail) {
+    uint32_t skip = 0;
+    if (n == (uint32_t)K) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) skip += s[k] == s[K - 1] ? 1u : 0u;
+    }
+    d = tail_fold(tb, t, c.background, skip);
+  }
+  o.out[t] = over(col, d);
+  if (kDebug) {
+    if (o.fk_key) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) o.fk_key[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)(s[k] >> 32) : kEmpty;
+    }
+    if (o.fk_val) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) o.fk_val[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)s[k] : kEmpty;
+    }
+    if (o.counts) o.counts[t] = n;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Weighted, blended OIT: oitWeighted.frag.glsl
+// ---------------------------------------------------------------------------
+
+// Colour pass (oitWeighted.frag.glsl:53-79) with the two framebuffer blends
+// (main.cpp:559-577): accum += premult * w (ONE, ONE) as float atomics,
+// reveal *= (1 - a) (ZERO, ONE_MINUS_SRC_COLOR) as a compare-and-swap loop.
+__global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WBBuf wb, float vnear, float vspan) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= f.count) return;
+  const uint32_t t = f.target[i];
+  if (t >= c.T) return;
+  const float4 col = f.rgba[i];
+  const float view = vnear + f.depth[i] * vspan;   // stands in for -viewZ (object.vert.glsl:35)
+  const float depthZ = view * 10.0f;               // :65
+  const float x = depthZ / 200.0f;
+  const float x2 = x * x;
+  const float x4 = x2 * x2;                        // pow(depthZ / 200, 4.0)
+  float dw = 0.03f / (1e-5f + x4);                 // :67
+  dw = fminf(fmaxf(dw, 1e-2f), 3e3f);
+  float aw = fminf(1.0f, fmaxf(fmaxf(col.x, col.y), fmaxf(col.z, col.w)) * 40.0f + 0.01f);  // :69
+  aw = aw * aw;                                    // :70
+  const float w = aw * dw;                         // :72
+  float* acc = reinterpret_cast<float*>(&wb.accum[t]);
+  unsafeAtomicAdd(acc + 0, col.x * w);
+  unsafeAtomicAdd(acc + 1, col.y * w);
+  unsafeAtomicAdd(acc + 2, col.z * w);
+  unsafeAtomicAdd(acc + 3, col.w * w);
+  const float om = 1.0f - col.w;
+  uint32_t* rp = reinterpret_cast<uint32_t*>(&wb.reveal[t]);
+  uint32_t old = ld_relaxed(rp);
+  for (;;) {
+    const uint32_t nv = __float_as_uint(__uint_as_float(old) * om);
+    const uint32_t prev = atomicCAS(rp, old, nv);
+    if (prev == old) break;
+    old = prev;
+  }
+}
+
+// Composite (oitWeighted.frag.glsl:98-109) with the WEIGHTED_COMPOSITE blend
+// (main.cpp:578-588): dst = src * (1 - reveal) + dst * reveal.
+__global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, ResolveOut o) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= c.T) return;
+  const float4 A = wb.accum[t];
+  const float r = wb.reveal[t];
+  const float den = fmaxf(A.w, 1e-5f);
+  const float4 s = make_float4(A.x / den, A.y / den, A.z / den, r);
+  const float om = 1.0f - r;
+  const float4 bg = c.background;
+  o.out[t] = make_float4(s.x * om + bg.x * r, s.y * om + bg.y * r, s.z * om + bg.z * r, s.w * om + bg.w * r);
+  if (o.counts) o.counts[t] = 0u;
+}
+
+// ---------------------------------------------------------------------------
+// Tail lists, phase 2: link the frame's records into per-target lists in
+// ascending id (the blend order of the colour-pass tail).
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb) {
+  uint32_t nrec = *tb.counter;
+  if (nrec > tb.cap) nrec = tb.cap;
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < nrec; m += stride) {
+    const uint64_t rec = tb.rec[m];
+    const uint32_t t = (uint32_t)(rec >> 32);
+    const uint32_t link = (uint32_t)rec;
+    list_insert_asc(&tb.head[t], tb.next, link, link >> 1);
+  }
+}
+
+template <template <int> class F, typename... A>
+hipError_t dispatch_k(uint32_t K, A&&... args) {
+  switch (K) {
+    case 1: return F<1>::run(args...);
+    case 2: return F<2>::run(args...);
+    case 4: return F<4>::run(args...);
+    case 8: return F<8>::run(args...);
+    case 16: return F<16>::run(args...);
+    case 32: return F<32>::run(args...);
+    default: return hipErrorInvalidValue;
+  }
+}
+
+template <int K>
+struct LLResolve {
+  static hipError_t run(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
+    const bool dbg = o.fk_key || o.fk_val || o.counts;
+    const bool ctail = tb.head != nullptr;
+    const dim3 g(blocks_for(c.T)), b(kBlock);
+#define OIT_LL(TAIL, CT, DBG) hipLaunchKernelGGL((k_ll_resolve<K, TAIL, CT, DBG>), g, b, 0, s, c, ll, tb, o)
+    if (c.tail) {
+      if (ctail) { if (dbg) OIT_LL(true, true, true); else OIT_LL(true, true, false); }
+      else { if (dbg) OIT_LL(true, false, true); else OIT_LL(true, false, false); }
+    } else {
+      if (dbg) OIT_LL(false, false, true); else OIT_LL(false, false, false);
+    }
+#undef OIT_LL
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct KBufResolve {
+  static hipError_t run(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
+    const bool dbg = o.fk_key || o.fk_val || o.counts;
+    const bool ctail = c.tail && tb.head != nullptr;
+    const dim3 g(blocks_for(c.T)), b(kBlock);
+#define OIT_KB(CT, DBG) hipLaunchKernelGGL((k_kbuf_resolve<K, CT, DBG>), g, b, 0, s, c, kb, tb, o)
+    if (ctail) { if (dbg) OIT_KB(true, true); else OIT_KB(true, false); }
+    else { if (dbg) OIT_KB(false, true); else OIT_KB(false, false); }
+#undef OIT_KB
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct SpinStore {
+  static hipError_t run(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s) {
+    const dim3 g(blocks_for(f.count)), b(kBlock);
+    if (c.tail && tb.head) hipLaunchKernelGGL((k_spin_store<K, true>), g, b, 0, s, c, f, kb, tb);
+    else hipLaunchKernelGGL((k_spin_store<K, false>), g, b, 0, s, c, f, kb, tb);
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct Loop64Store {
+  static hipError_t run(const Common& c, const FragView& f, uint64_t* slots, hipStream_t s) {
+    hipLaunchKernelGGL((k_loop64_store<K>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, slots);
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct Loop64Classify {
+  static hipError_t run(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb, hipStream_t s) {
+    hipLaunchKernelGGL((k_loop64_classify<K>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, slots, tb);
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct Loop64Resolve {
+  static hipError_t run(const Common& c, const uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
+                        hipStream_t s) {
+    const bool dbg = o.fk_key || o.fk_val || o.counts;
+    const bool ctail = c.tail && tb.head != nullptr;
+    const dim3 g(blocks_for(c.T)), b(kBlock);
+#define OIT_L64(CT, DBG) hipLaunchKernelGGL((k_loop64_resolve<K, CT, DBG>), g, b, 0, s, c, slots, tb, o)
+    if (ctail) { if (dbg) OIT_L64(true, true); else OIT_L64(true, false); }
+    else { if (dbg) OIT_L64(false, true); else OIT_L64(false, false); }
+#undef OIT_L64
+    return hipGetLastError();
+  }
+};
+
+}  // namespace
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t want = (n / 4 + kBlock - 1) / kBlock;
+  const uint32_t g = (uint32_t)(want < 4096 ? (want ? want : 1) : 4096);
+  hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_ll_store(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  const dim3 g(blocks_for(f.count)), b(kBlock);
+  if (c.tail && tb.head) hipLaunchKernelGGL((k_ll_store<true>), g, b, 0, s, c, f, ll, tb);
+  else hipLaunchKernelGGL((k_ll_store<false>), g, b, 0, s, c, f, ll, tb);
+  return hipGetLastError();
+}
+
+hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_ll_link, dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, ll);
+  return hipGetLastError();
+}
+
+hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
+  return dispatch_k<LLResolve>(c.K, c, ll, tb, o, s);
+}
+
+hipError_t launch_simple_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  const dim3 g(blocks_for(f.count)), b(kBlock);
+  if (c.tail && tb.head) hipLaunchKernelGGL((k_simple_store<true>), g, b, 0, s, c, f, kb, tb);
+  else hipLaunchKernelGGL((k_simple_store<false>), g, b, 0, s, c, f, kb, tb);
+  return hipGetLastError();
+}
+
+hipError_t launch_spin_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  return dispatch_k<SpinStore>(c.K, c, f, kb, tb, s);
+}
+
+hipError_t launch_kbuf_resolve(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
+  return dispatch_k<KBufResolve>(c.K, c, kb, tb, o, s);
+}
+
+hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  return dispatch_k<Loop64Store>(c.K, c, f, slots, s);
+}
+
+hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb,
+                                  hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  return dispatch_k<Loop64Classify>(c.K, c, f, slots, tb, s);
+}
+
+hipError_t launch_loop64_resolve(const Common& c, const uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
+                                 hipStream_t s) {
+  return dispatch_k<Loop64Resolve>(c.K, c, slots, tb, o, s);
+}
+
+hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& wb, float view_near, float view_span,
+                              hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_wboit_store, dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, wb, view_near, view_span);
+  return hipGetLastError();
+}
+
+hipError_t launch_wboit_resolve(const Common& c, const WBBuf& wb, const ResolveOut& o, hipStream_t s) {
+  hipLaunchKernelGGL(k_wboit_resolve, dim3(blocks_for(c.T)), dim3(kBlock), 0, s, c, wb, o);
+  return hipGetLastError();
+}
+
+hipError_t launch_tail_link(const TailBuf& tb, hipStream_t s) {
+  hipLaunchKernelGGL(k_tail_link, dim3(1024), dim3(kBlock), 0, s, tb);
+  return hipGetLastError();
+}
+
+}  // namespace oit

@@ -1,0 +1,90 @@
+// oit_kernels.hpp - host-side launch interface of the gfx950 OIT kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "oit_device.hpp"
+
+namespace oit {
+
+// Device view of one appended fragment batch.
+struct FragView {
+  const uint32_t* target;
+  const float* depth;
+  const float4* rgba;
+  uint32_t count;    // fragments in this batch
+  uint32_t id_base;  // frame-local id of the first fragment
+};
+
+// Colour-pass tail lists (per target, ascending fragment id).
+struct TailBuf {
+  uint32_t* head;     // [T] kEmpty = empty
+  uint32_t* next;     // [Fcap] link words, indexed by frame-local id
+  float4* rgba;       // [Fcap] premultiplied colour the tail blend uses
+  uint64_t* rec;      // [Fcap] (target << 32) | (id << 1 | boundary)
+  uint32_t* counter;  // number of records
+  uint32_t cap;       // Fcap
+};
+
+// Per-algorithm A-buffer views.
+struct LLBuf {
+  uint32_t* head;     // [T] 0 = empty (node 0 is the terminator)
+  uint64_t* cd;       // [cap] (depth key << 32) | packed colour
+  uint32_t* next;     // [cap]
+  uint64_t cap;       // nodes including the terminator
+  uint32_t* overflow; // LL fragments that found the pool full
+};
+
+struct KBuf {          // Simple / Spinlock
+  uint32_t* count;    // [T]
+  uint32_t* lock;     // [T] Spinlock only
+  uint32_t* depth;    // [T] Spinlock early-depth bound
+  uint32_t* col;      // [K][T]
+  uint32_t* key;      // [K][T]
+  uint32_t* id;       // [K][T]
+};
+
+struct WBBuf {
+  float4* accum;      // [T]
+  float* reveal;      // [T]
+};
+
+struct ResolveOut {
+  float4* out;        // [T]
+  uint32_t* fk_key;   // [T][K] or null
+  uint32_t* fk_val;   // [T][K] or null
+  uint32_t* counts;   // [T] or null
+};
+
+struct Common {
+  uint32_t T;              // targets
+  uint32_t K;              // layers
+  int tail;                // tail blending on
+  float4 background;
+  const CodecTables* tab;
+};
+
+hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+
+hipError_t launch_ll_store(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s);
+hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, hipStream_t s);
+hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s);
+
+hipError_t launch_simple_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s);
+hipError_t launch_spin_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s);
+hipError_t launch_kbuf_resolve(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o, hipStream_t s);
+
+hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, hipStream_t s);
+hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb,
+                                  hipStream_t s);
+hipError_t launch_loop64_resolve(const Common& c, const uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
+                                 hipStream_t s);
+
+hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& wb, float view_near, float view_span,
+                              hipStream_t s);
+hipError_t launch_wboit_resolve(const Common& c, const WBBuf& wb, const ResolveOut& o, hipStream_t s);
+
+hipError_t launch_tail_link(const TailBuf& tb, hipStream_t s);
+
+}  // namespace oit

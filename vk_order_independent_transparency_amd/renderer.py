@@ -1,0 +1,223 @@
+"""Host-side mirror of the reference's renderer interface, over the C ABI.
+
+``State`` carries the reference's settings under its own names (oit.h:64-116)
+and derives the antialiasing fields exactly as
+``State::recomputeAntialiasingSettings`` does (oit.h:84-115). ``Renderer``
+is the per-frame surface of ``Sample``: ``clear`` (clearTransparent<Alg>),
+``append`` (the COLOR draw of drawTransparent<Alg>), ``resolve`` (the
+COMPOSITE draw), dispatched on ``State.algorithm`` inside the native library.
+
+Device memory and streams come from PyTorch tensors (plumbing only): every
+call passes raw device pointers and torch's current HIP stream to the
+library, so torch events recorded on that stream bracket the kernels.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+from . import _abi
+
+ALGORITHM_NAMES = ("simple", "linkedlist", "loop", "loop64", "spinlock", "interlock", "weighted")
+
+
+@dataclasses.dataclass
+class State:
+    """Renderer settings; field names and defaults follow oit.h:64-76."""
+
+    algorithm: int = _abi.OIT_SPINLOCK
+    oitLayers: int = 8
+    linkedListAllocatedPerElement: int = 10
+    tailBlend: bool = True
+    aaType: int = _abi.AA_NONE
+    # implicit, recomputed from aaType (oit.h:78-82)
+    msaa: int = 1
+    sampleShading: bool = False
+    supersample: int = 1
+
+    def coverageShading(self) -> bool:  # oit.h:82
+        return self.msaa > 1 and not self.sampleShading
+
+    def recomputeAntialiasingSettings(self) -> None:  # oit.h:84-115
+        self.sampleShading = False
+        self.supersample = 1
+        if self.aaType == _abi.AA_NONE:
+            self.msaa = 1
+        elif self.aaType == _abi.AA_MSAA_4X:
+            self.msaa = 4
+        elif self.aaType == _abi.AA_SSAA_4X:
+            self.msaa = 4
+            self.sampleShading = True
+        elif self.aaType == _abi.AA_SUPER_4X:
+            self.msaa = 1
+            self.supersample = 2
+        elif self.aaType == _abi.AA_MSAA_8X:
+            self.msaa = 8
+        elif self.aaType == _abi.AA_SSAA_8X:
+            self.msaa = 8
+            self.sampleShading = True
+        else:
+            raise ValueError(f"antialiasing mode {self.aaType} not implemented")
+
+
+@dataclasses.dataclass
+class FrameConfig:
+    """Everything oit_configure needs, from a State plus the viewport."""
+
+    algorithm: int
+    layers: int
+    tail_blend: bool
+    width: int
+    height: int
+    samples: int = 1
+    ll_nodes_per_pixel: int = 10
+    ll_pool_nodes: int = 0
+    max_fragments: int = 0
+    background: tuple[float, float, float, float] = (0.5, 0.5, 0.5, 1.0)
+    wboit_view_near: float = 2.0
+    wboit_view_span: float = 8.0
+
+    @classmethod
+    def from_state(cls, state: State, width: int, height: int, **kw) -> "FrameConfig":
+        state.recomputeAntialiasingSettings()
+        if state.coverageShading():
+            raise ValueError("coverage shading (MSAA without sample shading) is not provided; use an SSAA mode")
+        w, h = width * state.supersample, height * state.supersample
+        return cls(
+            algorithm=state.algorithm,
+            layers=state.oitLayers,
+            tail_blend=state.tailBlend,
+            width=w,
+            height=h,
+            samples=state.msaa if state.sampleShading else 1,
+            ll_nodes_per_pixel=state.linkedListAllocatedPerElement,
+            **kw,
+        )
+
+    @property
+    def targets(self) -> int:
+        return self.width * self.height * self.samples
+
+    def to_c(self) -> _abi.OitConfig:
+        c = _abi.OitConfig()
+        c.algorithm = self.algorithm
+        c.layers = self.layers
+        c.tail_blend = 1 if self.tail_blend else 0
+        c.width = self.width
+        c.height = self.height
+        c.samples = self.samples
+        c.ll_nodes_per_pixel = self.ll_nodes_per_pixel
+        c.ll_pool_nodes = self.ll_pool_nodes
+        c.max_fragments = self.max_fragments
+        for i in range(4):
+            c.background[i] = self.background[i]
+        c.wboit_view_near = self.wboit_view_near
+        c.wboit_view_span = self.wboit_view_span
+        return c
+
+
+def _stream_ptr(stream) -> int:
+    import torch
+
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return int(s.cuda_stream)
+
+
+def _dptr(t) -> int:
+    if not t.is_cuda:
+        raise ValueError("expected a device tensor")
+    if not t.is_contiguous():
+        raise ValueError("expected a contiguous tensor")
+    return int(t.data_ptr())
+
+
+class Renderer:
+    """One native OIT context (oit_ctx) on one device."""
+
+    def __init__(self, device: int = 0):
+        self._lib = _abi.lib()
+        self._ctx = ctypes.c_void_p(0)
+        _abi.check("oit_create", self._lib.oit_create(ctypes.byref(self._ctx), int(device)))
+        self.device = device
+        self.config: FrameConfig | None = None
+
+    def close(self) -> None:
+        if self._ctx:
+            self._lib.oit_destroy(self._ctx)
+            self._ctx = ctypes.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def configure(self, cfg: FrameConfig) -> None:
+        c = cfg.to_c()
+        _abi.check("oit_configure", self._lib.oit_configure(self._ctx, ctypes.byref(c)))
+        self.config = cfg
+
+    def sizes(self) -> _abi.OitSizes:
+        out = _abi.OitSizes()
+        _abi.check("oit_query_sizes", self._lib.oit_query_sizes(self._ctx, ctypes.byref(out)))
+        return out
+
+    def clear(self, stream=None) -> None:
+        _abi.check("oit_clear", self._lib.oit_clear(self._ctx, _stream_ptr(stream)))
+
+    def append(self, target, depth, rgba, stream=None) -> None:
+        """target u32/int32 [F], depth f32 [F], rgba f32 [F, 4] premultiplied; device tensors."""
+        n = int(target.shape[0])
+        if depth.shape[0] != n or rgba.shape[0] != n or (n and tuple(rgba.shape[1:]) != (4,)):
+            raise ValueError("fragment arrays disagree in length")
+        f = _abi.OitFragments(_dptr(target) if n else None, _dptr(depth) if n else None,
+                              _dptr(rgba) if n else None, n)
+        _abi.check("oit_append", self._lib.oit_append(self._ctx, ctypes.byref(f), _stream_ptr(stream)))
+
+    def resolve(self, out, stream=None) -> None:
+        """out f32 [targets, 4] device tensor."""
+        self._check_out(out)
+        _abi.check("oit_resolve", self._lib.oit_resolve(self._ctx, _dptr(out), _stream_ptr(stream)))
+
+    def resolve_debug(self, out, frontk_key=None, frontk_val=None, counts=None, stream=None) -> None:
+        self._check_out(out)
+        p = lambda t: _dptr(t) if t is not None else None  # noqa: E731
+        _abi.check(
+            "oit_resolve_debug",
+            self._lib.oit_resolve_debug(self._ctx, _dptr(out), p(frontk_key), p(frontk_val), p(counts),
+                                        _stream_ptr(stream)),
+        )
+
+    def stats(self, stream=None) -> _abi.OitStats:
+        out = _abi.OitStats()
+        _abi.check("oit_get_stats", self._lib.oit_get_stats(self._ctx, ctypes.byref(out), _stream_ptr(stream)))
+        return out
+
+    def profile_enable(self, on: bool = True) -> None:
+        _abi.check("oit_profile_enable", self._lib.oit_profile_enable(self._ctx, 1 if on else 0))
+
+    def profile_reset(self) -> None:
+        _abi.check("oit_profile_reset", self._lib.oit_profile_reset(self._ctx))
+
+    def profile_read(self) -> dict:
+        out = _abi.OitProfile()
+        _abi.check("oit_profile_read", self._lib.oit_profile_read(self._ctx, ctypes.byref(out)))
+        return {name: (out.ms[i], out.launches[i]) for i, name in enumerate(_abi.PROFILE_STAGE_NAMES)}
+
+    def render(self, target, depth, rgba, out, stream=None) -> None:
+        """One frame: clear, one append, resolve (the reference's onRender order, oitRender.cpp:43-148)."""
+        self.clear(stream)
+        self.append(target, depth, rgba, stream)
+        self.resolve(out, stream)
+
+    def _check_out(self, out) -> None:
+        if self.config is None:
+            raise RuntimeError("configure() first")
+        if tuple(out.shape) != (self.config.targets, 4):
+            raise ValueError(f"out must have shape ({self.config.targets}, 4)")

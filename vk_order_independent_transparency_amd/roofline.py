@@ -1,0 +1,66 @@
+"""Algorithmic (compulsory) HBM bytes of each kernel and of a whole frame.
+
+One place for the formulas that bench.py's `roofline` block uses.
+F = fragments of the frame, P = targets (pixels x samples), K = layers,
+N = fragments that end in the colour-pass tail.
+
+Linked List (this build's layout: 8-byte (depth key, colour) node word plus
+a 4-byte link word per node, 4-byte head per target):
+  clear    : 4P                      (head words)
+  store    : 20F in (depth 4 + premultiplied RGBA 16) + 12F out (node + kUnset link)
+  link     : 4F in (target) + 8F (head read-modify-write, one per fragment
+             without run merging) + 4F out (link word)
+  resolve  : 4P (heads) + 12F (nodes) + 16P (RGBA out)
+Frame total 60F + 24P. The reference's layout (16-byte uvec4 nodes,
+oitLinkedList.frag.glsl:77-82) needs 64F + 24P for the same work; frame
+fractions are reported against that figure so the two layouts are compared
+on the same work.
+"""
+from __future__ import annotations
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def ll_stage_bytes(F: int, P: int) -> dict:
+    return {
+        "clear": 4 * P,
+        "store": 32 * F,
+        "link": 16 * F,
+        "resolve": 4 * P + 12 * F + 16 * P,
+    }
+
+
+def ll_frame_bytes_reference(F: int, P: int) -> int:
+    """64F + 24P: the reference node layout's compulsory traffic for one frame."""
+    return 64 * F + 24 * P
+
+
+def loop64_stage_bytes(F: int, P: int, K: int) -> dict:
+    return {
+        "clear": 8 * K * P,
+        "store": 20 * F + 16 * F,  # stream in + at least one slot read-modify-write
+        "resolve": 8 * K * P + 16 * P,
+    }
+
+
+def kbuffer_stage_bytes(F: int, P: int, K: int, spin: bool) -> dict:
+    aux = 12 if spin else 4
+    return {
+        "clear": aux * P,
+        "store": 24 * F + 12 * min(F, K * P) + 8 * F,
+        "resolve": 4 * P + 12 * min(F, K * P) + 16 * P,
+    }
+
+
+def wboit_stage_bytes(F: int, P: int) -> dict:
+    return {
+        "clear": 20 * P,
+        "store": 24 * F + 40 * F,  # stream in + accumulation read-modify-write
+        "resolve": 20 * P + 16 * P,
+    }
+
+
+def frac(bytes_: float, seconds: float) -> tuple[float, float]:
+    """Returns (GB/s, fraction of the HBM peak)."""
+    gbs = bytes_ / seconds / 1e9
+    return gbs, gbs / HBM_PEAK_GBS

@@ -1,0 +1,89 @@
+"""Synthetic sphere fragment streams (stand-in for the reference's rasterised
+sphere mesh, main.cpp:334-417) and the named workloads.
+
+The generator itself is native (csrc/oit_scene.cpp, exported through the C
+ABI); this module only describes workloads and copies results into numpy.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+
+import numpy as np
+
+from . import _abi
+
+
+@dataclasses.dataclass(frozen=True)
+class SceneParams:
+    seed: int
+    width: int
+    height: int
+    spheres: int
+    radius: tuple[float, float]
+    alpha: tuple[float, float]
+    samples: int = 1
+    cluster_fraction: float = 0.0
+    cluster_extent: float = 0.1
+
+    def to_c(self) -> _abi.OitSceneParams:
+        return _abi.OitSceneParams(
+            seed=self.seed,
+            width=self.width,
+            height=self.height,
+            samples=self.samples,
+            spheres=self.spheres,
+            radius_min=self.radius[0],
+            radius_max=self.radius[1],
+            alpha_min=self.alpha[0],
+            alpha_max=self.alpha[1],
+            cluster_fraction=self.cluster_fraction,
+            cluster_extent=self.cluster_extent,
+        )
+
+    @property
+    def targets(self) -> int:
+        return self.width * self.height * self.samples
+
+
+# Committed seed list; every reported result names its seed.
+SEEDS = (1, 2, 3, 4, 5, 6, 7, 8)
+
+# 256x256, 64 spheres, r in [0.02, 0.08], alpha in [0.2, 0.6]; Linked List K=8.
+SMALL_256 = SceneParams(seed=1, width=256, height=256, spheres=64, radius=(0.02, 0.08), alpha=(0.2, 0.6))
+# 1920x1080, 1000 spheres, r in [0.01, 0.05], alpha in [0.1, 0.5]; K in {8, 16}.
+HD_1080P = SceneParams(seed=1, width=1920, height=1080, spheres=1000, radius=(0.01, 0.05), alpha=(0.1, 0.5))
+# 3840x2160, 4 samples per pixel, 4000 spheres, r in [0.005, 0.03], alpha in [0.05, 0.4].
+UHD_4K_4X = SceneParams(
+    seed=1, width=3840, height=2160, samples=4, spheres=4000, radius=(0.005, 0.03), alpha=(0.05, 0.4)
+)
+
+
+def count(p: SceneParams) -> int:
+    n = ctypes.c_uint64(0)
+    cp = p.to_c()
+    _abi.check("oit_scene_count", _abi.lib().oit_scene_count(ctypes.byref(cp), ctypes.byref(n)))
+    return int(n.value)
+
+
+def generate(p: SceneParams) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
+    """Returns (target u32[F], depth f32[F], rgba f32[F,4]) in draw order."""
+    n = count(p)
+    target = np.empty(max(n, 1), dtype=np.uint32)
+    depth = np.empty(max(n, 1), dtype=np.float32)
+    rgba = np.empty((max(n, 1), 4), dtype=np.float32)
+    got = ctypes.c_uint64(0)
+    cp = p.to_c()
+    _abi.check(
+        "oit_scene_generate_host",
+        _abi.lib().oit_scene_generate_host(
+            ctypes.byref(cp),
+            target.ctypes.data,
+            depth.ctypes.data,
+            rgba.ctypes.data,
+            n,
+            ctypes.byref(got),
+        ),
+    )
+    assert got.value == n
+    return target[:n], depth[:n], rgba[:n]
