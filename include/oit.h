@@ -168,6 +168,28 @@ int oit_profile_read(oit_ctx* ctx, oit_profile* out);
 /* Zeroes the accumulated stage times. */
 int oit_profile_reset(oit_ctx* ctx);
 
+/* ---- Device plumbing for hosts without their own HIP binding (no
+ * reference counterpart: the reference allocates through VMA, oit.h:173-214,
+ * and times with its profiler). Thin wrappers of hipMalloc/hipMemcpy/
+ * streams/events on the library's own HIP runtime. ---------------------- */
+int oit_device_count(int* out);
+int oit_set_device(int device);
+int oit_device_alloc(void** out, uint64_t bytes);
+int oit_device_free(void* ptr);
+/* Stream-ordered copies (`stream` may be NULL); host memory is pageable, so
+ * they return after the copy has been staged. */
+int oit_memcpy_h2d(void* dst, const void* src, uint64_t bytes, void* stream);
+int oit_memcpy_d2h(void* dst, const void* src, uint64_t bytes, void* stream);
+int oit_memcpy_d2d(void* dst, const void* src, uint64_t bytes, void* stream);
+int oit_stream_create(void** out);
+int oit_stream_destroy(void* stream);
+int oit_stream_synchronize(void* stream);
+int oit_event_create(void** out);
+int oit_event_destroy(void* event);
+int oit_event_record(void* event, void* stream);
+int oit_event_synchronize(void* event);
+int oit_event_elapsed_ms(float* out, void* start, void* stop);
+
 /* ---- Synthetic scene (stands in for the reference's rasterised sphere
  * mesh, main.cpp:334-417): analytic spheres, orthographic, each covered
  * pixel (or sample) centre emits a front and a back fragment. ---------- */

@@ -131,6 +131,21 @@ SIGNATURES = {
     "oit_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "oit_profile_read": (ctypes.c_int, [_P, ctypes.POINTER(OitProfile)]),
     "oit_profile_reset": (ctypes.c_int, [_P]),
+    "oit_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
+    "oit_set_device": (ctypes.c_int, [ctypes.c_int]),
+    "oit_device_alloc": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_uint64]),
+    "oit_device_free": (ctypes.c_int, [_P]),
+    "oit_memcpy_h2d": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P]),
+    "oit_memcpy_d2h": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P]),
+    "oit_memcpy_d2d": (ctypes.c_int, [_P, _P, ctypes.c_uint64, _P]),
+    "oit_stream_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "oit_stream_destroy": (ctypes.c_int, [_P]),
+    "oit_stream_synchronize": (ctypes.c_int, [_P]),
+    "oit_event_create": (ctypes.c_int, [ctypes.POINTER(_P)]),
+    "oit_event_destroy": (ctypes.c_int, [_P]),
+    "oit_event_record": (ctypes.c_int, [_P, _P]),
+    "oit_event_synchronize": (ctypes.c_int, [_P]),
+    "oit_event_elapsed_ms": (ctypes.c_int, [ctypes.POINTER(ctypes.c_float), _P, _P]),
     "oit_scene_count": (ctypes.c_int, [ctypes.POINTER(OitSceneParams), ctypes.POINTER(ctypes.c_uint64)]),
     "oit_scene_generate_host": (
         ctypes.c_int,

@@ -7,9 +7,9 @@ is the per-frame surface of ``Sample``: ``clear`` (clearTransparent<Alg>),
 ``append`` (the COLOR draw of drawTransparent<Alg>), ``resolve`` (the
 COMPOSITE draw), dispatched on ``State.algorithm`` inside the native library.
 
-Device memory and streams come from PyTorch tensors (plumbing only): every
-call passes raw device pointers and torch's current HIP stream to the
-library, so torch events recorded on that stream bracket the kernels.
+Arguments are device buffers (``device.DeviceBuffer``) or raw device
+pointers, and streams are ``device.Stream`` objects or None (the null
+stream); all of it is allocated through the library's own C ABI.
 """
 from __future__ import annotations
 
@@ -17,6 +17,7 @@ import ctypes
 import dataclasses
 
 from . import _abi
+from .device import stream_ptr
 
 ALGORITHM_NAMES = ("simple", "linkedlist", "loop", "loop64", "spinlock", "interlock", "weighted")
 
@@ -117,18 +118,21 @@ class FrameConfig:
 
 
 def _stream_ptr(stream) -> int:
-    import torch
-
-    s = stream if stream is not None else torch.cuda.current_stream()
-    return int(s.cuda_stream)
+    return stream_ptr(stream)
 
 
 def _dptr(t) -> int:
-    if not t.is_cuda:
-        raise ValueError("expected a device tensor")
-    if not t.is_contiguous():
-        raise ValueError("expected a contiguous tensor")
-    return int(t.data_ptr())
+    """Device address of a DeviceBuffer (or anything with .ptr) or a raw int."""
+    if isinstance(t, int):
+        return t
+    p = getattr(t, "ptr", None)
+    if p is None:
+        raise TypeError("expected a DeviceBuffer or a device address")
+    return int(p)
+
+
+def _rows(t) -> int:
+    return int(t.shape[0])
 
 
 class Renderer:
@@ -171,11 +175,15 @@ class Renderer:
     def clear(self, stream=None) -> None:
         _abi.check("oit_clear", self._lib.oit_clear(self._ctx, _stream_ptr(stream)))
 
-    def append(self, target, depth, rgba, stream=None) -> None:
-        """target u32/int32 [F], depth f32 [F], rgba f32 [F, 4] premultiplied; device tensors."""
-        n = int(target.shape[0])
-        if depth.shape[0] != n or rgba.shape[0] != n or (n and tuple(rgba.shape[1:]) != (4,)):
-            raise ValueError("fragment arrays disagree in length")
+    def append(self, target, depth, rgba, stream=None, count=None) -> None:
+        """target u32 [F], depth f32 [F], rgba f32 [F, 4] premultiplied: DeviceBuffers, or raw
+        device addresses with `count`."""
+        if count is None:
+            n = _rows(target)
+            if _rows(depth) != n or _rows(rgba) != n or (n and tuple(rgba.shape[1:]) != (4,)):
+                raise ValueError("fragment arrays disagree in length")
+        else:
+            n = int(count)
         f = _abi.OitFragments(_dptr(target) if n else None, _dptr(depth) if n else None,
                               _dptr(rgba) if n else None, n)
         _abi.check("oit_append", self._lib.oit_append(self._ctx, ctypes.byref(f), _stream_ptr(stream)))
@@ -210,14 +218,14 @@ class Renderer:
         _abi.check("oit_profile_read", self._lib.oit_profile_read(self._ctx, ctypes.byref(out)))
         return {name: (out.ms[i], out.launches[i]) for i, name in enumerate(_abi.PROFILE_STAGE_NAMES)}
 
-    def render(self, target, depth, rgba, out, stream=None) -> None:
+    def render(self, target, depth, rgba, out, stream=None, count=None) -> None:
         """One frame: clear, one append, resolve (the reference's onRender order, oitRender.cpp:43-148)."""
         self.clear(stream)
-        self.append(target, depth, rgba, stream)
+        self.append(target, depth, rgba, stream, count)
         self.resolve(out, stream)
 
     def _check_out(self, out) -> None:
         if self.config is None:
             raise RuntimeError("configure() first")
-        if tuple(out.shape) != (self.config.targets, 4):
+        if not isinstance(out, int) and tuple(out.shape) != (self.config.targets, 4):
             raise ValueError(f"out must have shape ({self.config.targets}, 4)")
