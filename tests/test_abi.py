@@ -1,0 +1,123 @@
+"""The C ABI boundary without a GPU: the library loads, exports every function
+include/oit.h declares, and the ctypes structures have the C layout (checked
+against offsetof/sizeof from a C program compiled from the header)."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from vk_order_independent_transparency_amd import _abi
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "oit.h")
+
+
+def declared_functions():
+    text = open(HEADER).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(oit_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_every_declared_function_is_exported_and_bound():
+    lib = _abi.lib()
+    names = declared_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(lib, name), name
+        assert name in _abi.SIGNATURES, f"{name} has no ctypes signature"
+    assert set(_abi.SIGNATURES) == set(names)
+
+
+STRUCTS = {
+    "oit_config": _abi.OitConfig,
+    "oit_fragments": _abi.OitFragments,
+    "oit_sizes": _abi.OitSizes,
+    "oit_stats": _abi.OitStats,
+    "oit_profile": _abi.OitProfile,
+    "oit_scene_params": _abi.OitSceneParams,
+}
+
+
+def test_struct_layouts_match_the_header(tmp_path):
+    lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "oit.h"', "int main(void) {"]
+    for cname, cls in STRUCTS.items():
+        lines.append(f'  printf("{cname} size %zu\\n", sizeof({cname}));')
+        for fname, _ in cls._fields_:
+            lines.append(f'  printf("{cname}.{fname} %zu\\n", offsetof({cname}, {fname}));')
+    lines.append("  return 0; }")
+    src = tmp_path / "probe.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "probe"
+    subprocess.check_call(["gcc", "-std=c11", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe)])
+    got = {}
+    for line in subprocess.check_output([str(exe)]).decode().splitlines():
+        k, v = line.rsplit(" ", 1)
+        got[k] = int(v)
+    for cname, cls in STRUCTS.items():
+        assert got[f"{cname} size"] == ctypes.sizeof(cls), cname
+        for fname, _ in cls._fields_:
+            assert got[f"{cname}.{fname}"] == getattr(cls, fname).offset, f"{cname}.{fname}"
+
+
+def test_enums_match_reference_values():
+    # shaders/common.h:44-51 and :58-64 of the reference
+    assert (_abi.OIT_SIMPLE, _abi.OIT_LINKEDLIST, _abi.OIT_LOOP, _abi.OIT_LOOP64, _abi.OIT_SPINLOCK,
+            _abi.OIT_INTERLOCK, _abi.OIT_WEIGHTED) == tuple(range(7))
+    assert (_abi.AA_NONE, _abi.AA_MSAA_4X, _abi.AA_SSAA_4X, _abi.AA_SUPER_4X, _abi.AA_MSAA_8X,
+            _abi.AA_SSAA_8X) == tuple(range(6))
+    text = open(HEADER).read()
+    for name, value in (("OIT_SIMPLE", 0), ("OIT_LINKEDLIST", 1), ("OIT_LOOP64", 3), ("OIT_SPINLOCK", 4),
+                        ("OIT_WEIGHTED", 6)):
+        assert re.search(rf"{name}\s*=\s*{value}\b", text)
+
+
+def test_status_strings():
+    lib = _abi.lib()
+    assert lib.oit_status_string(0) == b"ok"
+    assert lib.oit_status_string(-6) == b"capacity exceeded"
+    assert lib.oit_status_string(12345) == b"unknown status"
+
+
+def test_null_arguments_are_errors_not_crashes():
+    lib = _abi.lib()
+    assert lib.oit_create(None, 0) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_destroy(None) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_configure(None, None) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_clear(None, None) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_append(None, None, None) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_resolve(None, None, None) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_device_alloc(None, 16) == _abi.OIT_ERR_INVALID_ARG
+
+
+def test_create_without_a_gpu_reports_device_error():
+    from vk_order_independent_transparency_amd import device_count
+
+    if device_count() > 0:
+        pytest.skip("a GPU is present")
+    ctx = ctypes.c_void_p()
+    assert _abi.lib().oit_create(ctypes.byref(ctx), 0) == _abi.OIT_ERR_DEVICE
+    assert not ctx.value
+
+
+def test_state_mirrors_reference_antialiasing_rules():
+    from vk_order_independent_transparency_amd import FrameConfig, State
+
+    s = State()
+    assert (s.algorithm, s.oitLayers, s.linkedListAllocatedPerElement, s.tailBlend) == (_abi.OIT_SPINLOCK, 8, 10,
+                                                                                       True)
+    for aa, (msaa, ss, sup) in {0: (1, False, 1), 1: (4, False, 1), 2: (4, True, 1), 3: (1, False, 2),
+                                4: (8, False, 1), 5: (8, True, 1)}.items():
+        s.aaType = aa
+        s.recomputeAntialiasingSettings()
+        assert (s.msaa, s.sampleShading, s.supersample) == (msaa, ss, sup)
+    s.aaType = _abi.AA_SSAA_4X
+    cfg = FrameConfig.from_state(s, 64, 32)
+    assert (cfg.width, cfg.height, cfg.samples) == (64, 32, 4)
+    s.aaType = _abi.AA_SUPER_4X
+    cfg = FrameConfig.from_state(s, 64, 32)
+    assert (cfg.width, cfg.height, cfg.samples) == (128, 64, 1)
+    s.aaType = _abi.AA_MSAA_4X
+    with pytest.raises(ValueError):
+        FrameConfig.from_state(s, 64, 32)

@@ -37,7 +37,7 @@ def _assert_same(name, got, want):
 @pytest.mark.parametrize("K", [1, 2, 4, 8, 16, 32])
 @pytest.mark.parametrize("tail", [0, 1])
 def test_linkedlist_bitexact(oracle, K, tail):
-    p, tg, d, c = deep_scene()
+    p, tg, d, c = deep_scene(spheres=160 if K <= 16 else 400)
     F = len(tg)
     cap = F + F // 2 + 1
     _say("LL", K, tail, F)
@@ -68,7 +68,7 @@ def test_linkedlist_pool_overflow_bitexact(oracle, cap_div):
 @pytest.mark.parametrize("K", [1, 4, 8, 16, 32])
 @pytest.mark.parametrize("tail", [0, 1])
 def test_loop64_bitexact(oracle, K, tail):
-    p, tg, d, c = deep_scene(seed=7)
+    p, tg, d, c = deep_scene(seed=7, spheres=160 if K <= 16 else 400)
     _say("Loop64", K, tail, len(tg))
     g = run_gpu(LOOP64, K, tail, p.width, p.height, tg, d, c)
     o = oracle.run(_oracle_cfg(oracle, oracle.LOOP64, K, tail, p), tg, d, c)
@@ -81,7 +81,7 @@ def test_loop64_bitexact(oracle, K, tail):
 @pytest.mark.parametrize("K", [1, 4, 8, 16, 32])
 @pytest.mark.parametrize("tail", [0, 1])
 def test_spinlock_bitexact(oracle, K, tail):
-    p, tg, d, c = deep_scene(seed=9)
+    p, tg, d, c = deep_scene(seed=9, spheres=160 if K <= 16 else 400)
     _say("Spinlock", K, tail, len(tg))
     g = run_gpu(SPIN, K, tail, p.width, p.height, tg, d, c)
     o = oracle.run(_oracle_cfg(oracle, oracle.SPINLOCK, K, tail, p), tg, d, c)

@@ -146,67 +146,84 @@ __device__ __forceinline__ uint32_t min_relaxed(uint32_t* p, uint32_t v) {
 // is still kUnset waits one loop trip for the owner's store. Only relaxed
 // atomics are used: no release fences (which would write back L2) are needed
 // because no other data is read during the build.
-// The loop is written so that one trip advances every active lane by one
-// step; a lane's store of its own link happens in the same trip as its
-// successful atomic, so lanes of one wave never wait on each other forever.
+// The loops are driven by a wave-wide ballot so that one trip advances every
+// active lane by one step; a lane's store of its own link happens in the same
+// trip as its successful atomic, so no lane ever waits on a store that sits
+// behind its own wait.
 // --------------------------------------------------------------------------
 
 // Descending list (head holds the largest value, 0 terminates).
 // Inserts block [lo, hi] (hi >= lo, consecutive values already chained
-// hi -> hi-1 -> ... -> lo by the caller) at link word `p0`.
-__device__ __forceinline__ void list_insert_desc(uint32_t* p0, uint32_t* next, uint32_t lo, uint32_t hi) {
-  uint32_t* p = p0;
+// hi -> hi-1 -> ... -> lo by the caller) at link word `p0` when `want`.
+// Called by every lane of the wave (lanes without a block pass want=false).
+// The loop condition is a wave-wide ballot, so the compiler cannot turn a
+// lane's wait for a kUnset link into an inner loop that the owner of that
+// link (possibly a lane of the same wave) would never leave: every trip is
+// one step for every active lane, and the owner's link store happens in the
+// trip of its successful atomic.
+__device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32_t* next, uint32_t lo,
+                                                 uint32_t hi) {
+  bool active = want;
   bool first = true;
-  for (;;) {
-    uint32_t v = 0;
-    bool try_max = first;
-    if (!first) {
-      v = ld_relaxed(p);
-      if (v != kUnset) {
-        if (v > hi) {
-          p = &next[v];
-        } else {
-          try_max = true;
+  uint32_t* p = p0;
+  while (__ballot(active)) {
+    if (active) {
+      bool try_max = first;
+      first = false;
+      if (!try_max) {
+        const uint32_t v = ld_relaxed(p);
+        if (v != kUnset) {
+          if (v > hi) {
+            p = &next[v];
+          } else {
+            try_max = true;
+          }
         }
       }
-    }
-    first = false;
-    if (try_max) {
-      const uint32_t old = max_relaxed(p, hi);
-      if (old < lo) {
-        st_relaxed(&next[lo], old);
-        return;
+      if (try_max) {
+        const uint32_t old = max_relaxed(p, hi);
+        if (old < lo) {
+          st_relaxed(&next[lo], old);
+          active = false;
+        } else {
+          p = &next[old];  // a larger element got there first: continue behind it
+        }
       }
-      p = &next[old];  // a larger element got there first: continue behind it
     }
   }
 }
 
 // Ascending list (head holds the smallest value, kEmpty terminates).
-// Inserts the single value x at link word p0.
-__device__ __forceinline__ void list_insert_asc(uint32_t* p0, uint32_t* next, uint32_t x, uint32_t next_index) {
-  uint32_t* p = p0;
+// Inserts the single value x at link word p0 when `want`; same trip
+// structure as list_insert_desc.
+__device__ __forceinline__ void list_insert_asc(bool want, uint32_t* p0, uint32_t* next, uint32_t x,
+                                                uint32_t next_index) {
+  bool active = want;
   bool first = true;
-  for (;;) {
-    bool try_min = first;
-    if (!first) {
-      const uint32_t v = ld_relaxed(p);
-      if (v != kUnset) {
-        if (v < x) {
-          p = &next[v >> 1];
-        } else {
-          try_min = true;
+  uint32_t* p = p0;
+  while (__ballot(active)) {
+    if (active) {
+      bool try_min = first;
+      first = false;
+      if (!try_min) {
+        const uint32_t v = ld_relaxed(p);
+        if (v != kUnset) {
+          if (v < x) {
+            p = &next[v >> 1];
+          } else {
+            try_min = true;
+          }
         }
       }
-    }
-    first = false;
-    if (try_min) {
-      const uint32_t old = min_relaxed(p, x);
-      if (old > x) {
-        st_relaxed(&next[next_index], old);
-        return;
+      if (try_min) {
+        const uint32_t old = min_relaxed(p, x);
+        if (old > x) {
+          st_relaxed(&next[next_index], old);
+          active = false;
+        } else {
+          p = &next[old >> 1];
+        }
       }
-      p = &next[old >> 1];
     }
   }
 }

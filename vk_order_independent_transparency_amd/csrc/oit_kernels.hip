@@ -200,11 +200,11 @@ __global__ __launch_bounds__(kBlock) void k_ll_link(Common c, FragView f, LLBuf 
     t = kEmpty;
   }
   const Run r = same_target_run(t);
-  if (!valid) return;
   const uint32_t n = (uint32_t)n64;
   // chain inside the run; the run start lane's link is written by the insert
-  if (!r.start) st_relaxed(&ll.next[n], n - 1u);
-  if (r.end) list_insert_desc(&ll.head[t], ll.next, n - (r.lane - r.start_lane), n);
+  if (valid && !r.start) st_relaxed(&ll.next[n], n - 1u);
+  const bool ins = valid && r.end;
+  list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n);
 }
 
 // Composite pass, oitLinkedList.frag.glsl:106-172: walk the list (newest
@@ -283,9 +283,9 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
 // ---------------------------------------------------------------------------
 
 // Colour pass (oitSpinlock.frag.glsl:49-129). The critical section runs under
-// a per-target lock taken with an atomic exchange inside a `done` loop, the
-// shape that lets one lane of a wave hold the lock while its neighbours
-// retry on the next trip. Elements are ordered by (depth key, id); the
+// a per-target lock taken with an atomic exchange; one lane of a wave holds
+// the lock while its neighbours retry on the next trip. Elements are ordered
+// by (depth key, id); the
 // furthest is found by a scan with first-index-wins (:96-107), replaced when
 // the newcomer is nearer (:109-117). Evicted and rejected fragments are
 // recorded for the tail with their packed sRGB8 colour.
@@ -297,53 +297,60 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   if (t >= c.T) valid = false;
   const uint32_t lid = f.id_base + i;
   bool tail = false;
+  bool pending = false;
   uint32_t tail_id = lid;
   uint32_t tail_col = 0u;
+  uint32_t my_key = 0u, my_col = 0u;
   if (valid) {
     const uint32_t key = depth_key(f.depth[i]);
     const uint32_t col = encode_color(f.rgba[i], c.tab);
     tail_col = col;
-    if (key <= ld_relaxed(&kb.depth[t])) {  // :66-69 early depth test
-      bool done = false;
-      while (!done) {
-        if (__hip_atomic_exchange(&kb.lock[t], 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-          const uint32_t cnt = ld_relaxed(&kb.count[t]);  // :86-87
-          st_relaxed(&kb.count[t], cnt + 1u);
-          if (cnt < (uint32_t)K) {                          // :89-93
-            const uint64_t idx = (uint64_t)cnt * c.T + t;
-            st_relaxed(&kb.col[idx], col);
-            st_relaxed(&kb.key[idx], key);
-            st_relaxed(&kb.id[idx], lid);
-          } else {
-            uint32_t fur = 0;                               // :96-107
-            uint64_t mx = 0;
+    pending = key <= ld_relaxed(&kb.depth[t]);  // :66-69 early depth test
+    if (!pending) tail = true;
+    my_key = key;
+    my_col = col;
+  }
+  // The lock loop (:75-123) is driven by a wave-wide ballot: a lane that takes
+  // the lock runs the critical section and releases it in the same trip, and
+  // the compiler cannot hoist the exchange into an inner spin that a lock
+  // holder of the same wave would wait behind.
+  while (__ballot(pending)) {
+    if (pending) {
+      if (__hip_atomic_exchange(&kb.lock[t], 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
+        const uint32_t cnt = ld_relaxed(&kb.count[t]);  // :86-87
+        st_relaxed(&kb.count[t], cnt + 1u);
+        if (cnt < (uint32_t)K) {                          // :89-93
+          const uint64_t idx = (uint64_t)cnt * c.T + t;
+          st_relaxed(&kb.col[idx], my_col);
+          st_relaxed(&kb.key[idx], my_key);
+          st_relaxed(&kb.id[idx], lid);
+        } else {
+          uint32_t fur = 0;                               // :96-107
+          uint64_t mx = 0;
 #pragma unroll
-            for (int j = 0; j < K; ++j) {
-              const uint64_t idx = (uint64_t)j * c.T + t;
-              const uint64_t v = ((uint64_t)ld_relaxed(&kb.key[idx]) << 32) | ld_relaxed(&kb.id[idx]);
-              if (v > mx) {
-                mx = v;
-                fur = (uint32_t)j;
-              }
+          for (int j = 0; j < K; ++j) {
+            const uint64_t idx = (uint64_t)j * c.T + t;
+            const uint64_t v = ((uint64_t)ld_relaxed(&kb.key[idx]) << 32) | ld_relaxed(&kb.id[idx]);
+            if (v > mx) {
+              mx = v;
+              fur = (uint32_t)j;
             }
-            const uint64_t me = ((uint64_t)key << 32) | lid;
-            if (mx > me) {                                  // :109-117
-              const uint64_t idx = (uint64_t)fur * c.T + t;
-              tail_col = ld_relaxed(&kb.col[idx]);
-              tail_id = (uint32_t)mx;
-              st_relaxed(&kb.col[idx], col);
-              st_relaxed(&kb.key[idx], key);
-              st_relaxed(&kb.id[idx], lid);
-              st_relaxed(&kb.depth[t], (uint32_t)(mx >> 32));
-            }
-            tail = true;
           }
-          __hip_atomic_exchange(&kb.lock[t], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-          done = true;
+          const uint64_t me = ((uint64_t)my_key << 32) | lid;
+          if (mx > me) {                                  // :109-117
+            const uint64_t idx = (uint64_t)fur * c.T + t;
+            tail_col = ld_relaxed(&kb.col[idx]);
+            tail_id = (uint32_t)mx;
+            st_relaxed(&kb.col[idx], my_col);
+            st_relaxed(&kb.key[idx], my_key);
+            st_relaxed(&kb.id[idx], lid);
+            st_relaxed(&kb.depth[t], (uint32_t)(mx >> 32));
+          }
+          tail = true;
         }
+        __hip_atomic_exchange(&kb.lock[t], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        pending = false;
       }
-    } else {
-      tail = true;
     }
   }
   if constexpr (kTail) {
@@ -541,11 +548,16 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb) {
   uint32_t nrec = *tb.counter;
   if (nrec > tb.cap) nrec = tb.cap;
   const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t m = blockIdx.x * kBlock + threadIdx.x; m < nrec; m += stride) {
-    const uint64_t rec = tb.rec[m];
+  // every lane runs the same number of trips so the inserts stay wave-wide
+  const uint32_t base0 = blockIdx.x * kBlock + threadIdx.x;
+  for (uint32_t m0 = blockIdx.x * kBlock; m0 < nrec; m0 += stride) {
+    const uint32_t m = m0 + (base0 - blockIdx.x * kBlock);
+    const bool want = m < nrec;
+    uint64_t rec = 0;
+    if (want) rec = tb.rec[m];
     const uint32_t t = (uint32_t)(rec >> 32);
     const uint32_t link = (uint32_t)rec;
-    list_insert_asc(&tb.head[t], tb.next, link, link >> 1);
+    list_insert_asc(want, want ? &tb.head[t] : nullptr, tb.next, link, link >> 1);
   }
 }
 
