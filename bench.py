@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--layers", type=int, default=16)
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--variant", type=int, default=0, help="OIT_VARIANT_* bits (kernel schedule; results identical)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the frame and request workloads "
@@ -89,7 +90,7 @@ def main():
     out = DeviceBuffer((P, 4), "float32")
     r = Renderer(0)
     r.configure(FrameConfig(algorithm=OIT_LINKEDLIST, layers=K, tail_blend=True, width=p.width,
-                            height=p.height, ll_pool_nodes=cap))
+                            height=p.height, ll_pool_nodes=cap, variant=args.variant))
 
     for _ in range(args.warmup):
         r.render(tg_d, d_d, c_d, out, stream)
@@ -117,7 +118,7 @@ def main():
     stream.synchronize()
     prof = r.profile_read()
     r.profile_enable(False)
-    stage_bytes = rl.ll_stage_bytes(F, P)
+    stage_bytes = rl.ll_stage_bytes(F, P, fused=not (args.variant & 1))
     stages = {}
     for name, (ms, n) in prof.items():
         if n:
@@ -161,6 +162,7 @@ def main():
             "fragments": F,
             "pool_nodes": cap,
             "tail_blend": True,
+            "variant": args.variant,
         },
         "roofline": {
             "bound": "hbm",

@@ -47,6 +47,11 @@ enum oit_algorithm {
   OIT_WEIGHTED   = 6
 };
 
+/* Kernel variant bits (oit_config.variant); results are identical for every
+ * variant, only the kernel schedule differs. */
+#define OIT_VARIANT_LL_SPLIT_APPEND  0x1u /* Linked List: separate store and link kernels */
+#define OIT_VARIANT_LL_SINGLE_WALK   0x2u /* Linked List: resolve walks one node per round trip */
+
 /* Status codes. */
 enum oit_status {
   OIT_OK               = 0,
@@ -70,7 +75,7 @@ typedef struct oit_config {
   uint32_t height;             /* viewport height (SceneData.viewport.y) */
   uint32_t samples;            /* sample planes: 1, or the MSAA count with sample shading */
   uint32_t ll_nodes_per_pixel; /* State::linkedListAllocatedPerElement (oit.h:68), used when ll_pool_nodes == 0 */
-  uint32_t reserved0;
+  uint32_t variant;            /* kernel variant bits for tuning (OIT_VARIANT_*); 0 = library default */
   uint64_t ll_pool_nodes;      /* Linked List capacity in nodes INCLUDING terminator node 0
                                   (SceneData.linkedListAllocatedPerElement for OIT_LINKEDLIST,
                                   oit.cpp:125); 0 = ll_nodes_per_pixel * width * height * samples */

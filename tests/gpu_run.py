@@ -8,7 +8,7 @@ import numpy as np
 
 
 def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, samples=1, ll_pool_nodes=0,
-            splits=1, background=(0.5, 0.5, 0.5, 1.0), max_fragments=0, renderer=None):
+            splits=1, background=(0.5, 0.5, 0.5, 1.0), max_fragments=0, renderer=None, variant=0):
     from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, Renderer
 
     T = width * height * samples
@@ -18,7 +18,7 @@ def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, s
     try:
         cfg = FrameConfig(algorithm=algorithm, layers=layers, tail_blend=bool(tail_blend), width=width,
                           height=height, samples=samples, ll_pool_nodes=ll_pool_nodes,
-                          max_fragments=max_fragments, background=background)
+                          max_fragments=max_fragments, background=background, variant=variant)
         r.configure(cfg)
         F = int(len(target))
         tg = DeviceBuffer.from_numpy(np.ascontiguousarray(target, dtype=np.uint32).reshape(F))

@@ -51,6 +51,57 @@ def test_linkedlist_bitexact(oracle, K, tail):
     assert (per_target_counts(tg, p.targets) > K).any(), "scene must engage the tail"
 
 
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("K", [4, 16])
+def test_linkedlist_kernel_variants_bitexact(oracle, variant, K):
+    """Fused or split append, pair or single walk: identical results."""
+    p, tg, d, c = deep_scene(seed=25, spheres=200)
+    cap = len(tg) + 1
+    g = run_gpu(LL, K, 1, p.width, p.height, tg, d, c, ll_pool_nodes=cap, variant=variant)
+    o = oracle.run(_oracle_cfg(oracle, oracle.LINKEDLIST, K, 1, p, cap), tg, d, c)
+    _assert_same("frontk_val", g["frontk_val"], o["frontk_val"])
+    _assert_same("out", g["out"], o["out"])
+
+
+def test_linkedlist_frames_reuse_pool(oracle):
+    """A frame left unresolved, then resolved frames of different sizes on one context:
+    every resolve matches the oracle (the pool is cleaned by the resolve or by the clear)."""
+    from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, Renderer
+
+    scenes = [deep_scene(seed=s, spheres=n) for s, n in ((31, 200), (32, 120), (33, 260), (34, 60))]
+    cap = max(len(x[1]) for x in scenes) + 1
+    W, H = scenes[0][0].width, scenes[0][0].height
+    with Renderer(0) as r:
+        r.configure(FrameConfig(algorithm=LL, layers=8, tail_blend=True, width=W, height=H, ll_pool_nodes=cap))
+        out = DeviceBuffer((W * H, 4), np.float32)
+        for idx, (p, tg, d, c) in enumerate(scenes):
+            bufs = [DeviceBuffer.from_numpy(np.ascontiguousarray(a)) for a in (tg, d, c)]
+            r.clear()
+            r.append(*bufs)
+            if idx == 0:
+                continue  # abandoned frame: never resolved
+            r.resolve(out)
+            want = oracle.run(_oracle_cfg(oracle, oracle.LINKEDLIST, 8, 1, p, cap), tg, d, c)["out"]
+            _assert_same(f"frame {idx}", out.numpy(), want)
+
+
+def test_second_resolve_of_a_frame_is_rejected():
+    from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, OitError, Renderer
+
+    p, tg, d, c = deep_scene(seed=35)
+    with Renderer(0) as r:
+        r.configure(FrameConfig(algorithm=LL, layers=8, tail_blend=True, width=p.width, height=p.height,
+                                ll_pool_nodes=len(tg) + 1))
+        out = DeviceBuffer((p.targets, 4), np.float32)
+        r.clear()
+        r.append(*[DeviceBuffer.from_numpy(np.ascontiguousarray(a)) for a in (tg, d, c)])
+        r.resolve(out)
+        with pytest.raises(OitError):
+            r.resolve(out)
+        with pytest.raises(OitError):
+            r.append(*[DeviceBuffer.from_numpy(np.ascontiguousarray(a)) for a in (tg, d, c)])
+
+
 @pytest.mark.parametrize("cap_div", [2, 5])
 def test_linkedlist_pool_overflow_bitexact(oracle, cap_div):
     p, tg, d, c = deep_scene(seed=5)

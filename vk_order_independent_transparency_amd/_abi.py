@@ -40,6 +40,9 @@ OIT_ERR_CAPACITY = -6
 OIT_ERR_STATE = -7
 OIT_ERR_NOT_READY = -8
 
+VARIANT_LL_SPLIT_APPEND = 0x1
+VARIANT_LL_SINGLE_WALK = 0x2
+
 OIT_PROFILE_STAGES = 6
 PROFILE_STAGE_NAMES = ("clear", "store", "link", "classify", "tail_link", "resolve")
 
@@ -53,7 +56,7 @@ class OitConfig(ctypes.Structure):
         ("height", ctypes.c_uint32),
         ("samples", ctypes.c_uint32),
         ("ll_nodes_per_pixel", ctypes.c_uint32),
-        ("reserved0", ctypes.c_uint32),
+        ("variant", ctypes.c_uint32),
         ("ll_pool_nodes", ctypes.c_uint64),
         ("max_fragments", ctypes.c_uint64),
         ("background", ctypes.c_float * 4),

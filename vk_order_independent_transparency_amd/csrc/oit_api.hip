@@ -78,12 +78,19 @@ struct oit_ctx {
   oit_config cfg{};
   uint32_t T = 0;
   oit::CodecTables* d_tab = nullptr;
-  uint32_t* d_counters = nullptr;  // [0] tail records, [1] LL overflow
+  uint32_t* d_counters = nullptr;  // [0] tail records, [1] LL overflow; [16..31] last frame (LL)
   // Linked List
   uint32_t* ll_head = nullptr;
   uint64_t* ll_cd = nullptr;
   uint32_t* ll_next = nullptr;
   uint64_t ll_cap = 0;
+  // Linked List pool state: "clean" = heads 0 and every link word kUnset, as
+  // the resetting resolve leaves them; dirty_upto bounds the nodes a frame
+  // that was not resolved may have touched.
+  bool ll_clean = false;
+  uint64_t ll_dirty_upto = 0;
+  bool counters_clean = false;
+  bool stats_moved = false;  // the last resolve moved the counters to [16..31]
   // Simple / Spinlock
   uint32_t* kb_count = nullptr;
   uint32_t* kb_lock = nullptr;
@@ -313,9 +320,9 @@ int oit_create(oit_ctx** out, int device) {
   oit::CodecTables host;
   build_codec(&host);
   int rc = dev_alloc(&c->d_tab, 1);
-  if (!rc) rc = dev_alloc(&c->d_counters, 16);
+  if (!rc) rc = dev_alloc(&c->d_counters, 32);
   if (!rc && hipMemcpy(c->d_tab, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess) rc = OIT_ERR_DEVICE;
-  if (!rc && hipMemset(c->d_counters, 0, 16 * sizeof(uint32_t)) != hipSuccess) rc = OIT_ERR_DEVICE;
+  if (!rc && hipMemset(c->d_counters, 0, 32 * sizeof(uint32_t)) != hipSuccess) rc = OIT_ERR_DEVICE;
   if (rc) {
     dev_free(&c->d_tab);
     dev_free(&c->d_counters);
@@ -414,10 +421,21 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
       if (!rc) c->tl_cap = (uint32_t)cap;
     }
   }
+  if (!rc && alg == OIT_LINKEDLIST) {  // pool starts clean: heads 0, links kUnset
+    hipError_t e = oit::launch_fill_u32(c->ll_head, T, 0u, nullptr);
+    if (e == hipSuccess) e = oit::launch_fill_u32(c->ll_next, c->ll_cap, oit::kUnset, nullptr);
+    if (e == hipSuccess) e = oit::launch_fill_u32(c->d_counters, 32, 0u, nullptr);
+    if (e == hipSuccess) e = hipDeviceSynchronize();
+    if (e != hipSuccess) rc = status_of(e);
+  }
   if (rc) {
     free_frame_buffers(c);
     return rc;
   }
+  c->ll_clean = alg == OIT_LINKEDLIST;
+  c->ll_dirty_upto = 0;
+  c->counters_clean = alg == OIT_LINKEDLIST;
+  c->stats_moved = false;
   c->configured = true;
   c->frame_open = false;
   c->resolved = false;
@@ -453,11 +471,31 @@ int oit_clear(oit_ctx* c, void* stream) {
   DeviceGuard g(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const uint64_t T = c->T, K = c->cfg.layers;
+  const bool ll = c->cfg.algorithm == OIT_LINKEDLIST;
+  const bool need_ll_fill = ll && !c->ll_clean;
+  const bool need_counters = !ll || !c->counters_clean;
+  const bool need_tail = c->tl_head && (!ll || c->tl_next);
+  if (!need_ll_fill && !need_counters && !need_tail) {
+    // steady state of the Linked List: the previous resolve left heads at 0,
+    // links at kUnset and the counters at 0 (the clear of oitRender.cpp:
+    // 206-212 is done by the reader)
+    c->tl_head_valid = c->tl_head != nullptr;
+    c->frame_frags = 0;
+    c->appends = 0;
+    c->segments.clear();
+    c->frame_open = true;
+    c->resolved = false;
+    c->stats_moved = false;
+    return OIT_OK;
+  }
   hipError_t e = staged(c, kStClear, s, [&]() -> hipError_t {
-    hipError_t r = oit::launch_fill_u32(c->d_counters, 16, 0u, s);
+    hipError_t r = hipSuccess;
+    if (need_counters) r = oit::launch_fill_u32(c->d_counters, 16, 0u, s);
     switch (c->cfg.algorithm) {
-      case OIT_LINKEDLIST:  // head and counter to 0, oitRender.cpp:206-212
-        if (r == hipSuccess) r = oit::launch_fill_u32(c->ll_head, T, 0u, s);
+      case OIT_LINKEDLIST:  // head and counter to 0, oitRender.cpp:206-212; links of a dirty range to kUnset
+        if (need_ll_fill && r == hipSuccess) r = oit::launch_fill_u32(c->ll_head, T, 0u, s);
+        if (need_ll_fill && r == hipSuccess)
+          r = oit::launch_fill_u32(c->ll_next, c->ll_dirty_upto ? c->ll_dirty_upto : c->ll_cap, oit::kUnset, s);
         break;
       case OIT_SIMPLE:  // count to 0, oitRender.cpp:161-169
         if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_count, T, 0u, s);
@@ -475,9 +513,15 @@ int oit_clear(oit_ctx* c, void* stream) {
         if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_reveal), T, 0x3F800000u, s);
         break;
     }
-    if (r == hipSuccess && c->tl_head) r = oit::launch_fill_u32(c->tl_head, T, kEmpty, s);
+    if (r == hipSuccess && need_tail) r = oit::launch_fill_u32(c->tl_head, T, kEmpty, s);
     return r;
   });
+  if (e == hipSuccess && ll) {
+    c->ll_clean = true;
+    c->ll_dirty_upto = 0;
+    c->counters_clean = true;
+  }
+  c->stats_moved = false;
   c->tl_head_valid = c->tl_head != nullptr;
   c->frame_frags = 0;
   c->appends = 0;
@@ -516,9 +560,17 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
   hipError_t e = hipSuccess;
   switch (alg) {
     case OIT_LINKEDLIST: {
-      oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1};
-      e = staged(c, kStStore, s, [&] { return oit::launch_ll_store(k, v, ll, tb, s); });
-      if (e == hipSuccess) e = staged(c, kStLink, s, [&] { return oit::launch_ll_link(k, v, ll, s); });
+      oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      c->ll_clean = false;
+      c->counters_clean = false;
+      const uint64_t touched = end + 1 < c->ll_cap ? end + 1 : c->ll_cap;
+      if (touched > c->ll_dirty_upto) c->ll_dirty_upto = touched;
+      if (c->cfg.variant & OIT_VARIANT_LL_SPLIT_APPEND) {
+        e = staged(c, kStStore, s, [&] { return oit::launch_ll_store(k, v, ll, tb, s); });
+        if (e == hipSuccess) e = staged(c, kStLink, s, [&] { return oit::launch_ll_link(k, v, ll, s); });
+      } else {
+        e = staged(c, kStStore, s, [&] { return oit::launch_ll_append(k, v, ll, tb, s); });
+      }
       break;
     }
     case OIT_SIMPLE: {
@@ -554,6 +606,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
                       void* stream) {
   if (!c || !out_rgba) return OIT_ERR_INVALID_ARG;
   if (!c->configured || !c->frame_open) return OIT_ERR_STATE;
+  if (c->resolved) return OIT_ERR_STATE;  // one composite per frame (the Linked List recycles its heads)
   DeviceGuard g(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const uint32_t alg = c->cfg.algorithm;
@@ -581,8 +634,15 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
   if (!tb.next) tbr.head = nullptr;  // no tail records can exist
   switch (alg) {
     case OIT_LINKEDLIST: {
-      oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1};
-      e = staged(c, kStResolve, s, [&] { return oit::launch_ll_resolve(k, ll, tbr, o, s); });
+      oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      const bool pair = !(c->cfg.variant & OIT_VARIANT_LL_SINGLE_WALK);
+      e = staged(c, kStResolve, s, [&] { return oit::launch_ll_resolve_reset(k, ll, tbr, o, pair, s); });
+      if (e == hipSuccess) {
+        c->ll_clean = true;
+        c->ll_dirty_upto = 0;
+        c->counters_clean = true;
+        c->stats_moved = true;
+      }
       break;
     }
     case OIT_SIMPLE:
@@ -612,7 +672,8 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
   DeviceGuard g(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   uint32_t h[16] = {0};
-  if (hipMemcpyAsync(h, c->d_counters, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess) return OIT_ERR_DEVICE;
+  const uint32_t* src = c->d_counters + (c->stats_moved ? 16 : 0);
+  if (hipMemcpyAsync(h, src, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess) return OIT_ERR_DEVICE;
   if (hipStreamSynchronize(s) != hipSuccess) return OIT_ERR_DEVICE;
   std::memset(out, 0, sizeof *out);
   out->fragments = c->frame_frags;

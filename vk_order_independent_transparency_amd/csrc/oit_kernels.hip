@@ -240,6 +240,113 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
   if (kDebug) write_debug<K>(o, t, a, n, cnt, true, false);
 }
 
+// Fused colour pass of the Linked List: store and ordered link in one
+// kernel (oitLinkedList.frag.glsl:51-85). Blocks loop over 256-fragment
+// chunks so the encode table is staged in LDS once per block. The link words
+// of the frame's nodes already hold kUnset (left by the previous frame's
+// resetting resolve, or by the clear), so a walker can never follow a stale
+// link; the run chain n -> n-1 and the inserts are relaxed atomics.
+template <bool kTail>
+__global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBuf ll, TailBuf tb) {
+  __shared__ float s_thr[257];
+  for (uint32_t k = threadIdx.x; k < 257u; k += kBlock) s_thr[k] = c.tab->thr[k];
+  __syncthreads();
+  const uint32_t nchunks = (f.count + kBlock - 1) / kBlock;
+  for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
+    const uint32_t i = chunk * kBlock + threadIdx.x;
+    const bool valid = i < f.count;
+    const uint32_t lid = f.id_base + i;
+    const uint64_t n64 = (uint64_t)lid + 1u;
+    uint32_t t = valid ? f.target[i] : kEmpty;
+    if (t >= c.T) t = kEmpty;
+    const bool in_frame = t != kEmpty;
+    const bool node = in_frame && n64 < ll.cap;
+    const bool ovf = in_frame && !node;
+    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (in_frame) col = f.rgba[i];
+    const uint32_t n = (uint32_t)n64;
+    if (node) ll.cd[n] = ((uint64_t)depth_key(f.depth[i]) << 32) | encode_color_lds(col, s_thr);
+    const Run r = same_target_run(node ? t : kEmpty);
+    if (node && !r.start) st_relaxed(&ll.next[n], n - 1u);
+    const bool ins = node && r.end;
+    list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n);
+    const uint64_t om = __ballot(ovf);
+    if (om && lane_id() == (uint32_t)__builtin_ctzll(om))
+      atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
+    if constexpr (kTail) tail_record(ovf, t, lid, 0u, col, tb);
+  }
+}
+
+// Composite of the Linked List (oitLinkedList.frag.glsl:106-172) that leaves
+// the structure clean for the next frame: head[t] = 0 (the clear of
+// oitRender.cpp:206-212 done by the reader) and next[p] = kUnset for every
+// walked node. With kPair, the loads for node p-1 go out with those for p:
+// consecutive fragments of one target (a sphere's front and back) are
+// consecutive nodes chained p -> p-1, so one round trip consumes both.
+template <int K, bool kTail, bool kColorTail, bool kDebug, bool kPair>
+__global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll, TailBuf tb, ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  for (uint32_t k = threadIdx.x; k < 256u; k += kBlock) {
+    s_dec[k] = c.tab->dec[k];
+    s_alpha[k] = c.tab->alpha[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
+    ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
+    ll.counters[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+    FrontK<K> a;
+    a.init();
+    float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t p = ll.head[t];
+    if (p != 0u) ll.head[t] = 0u;
+    uint32_t cnt = 0;
+    while (p != 0u) {
+      const uint64_t cd0 = ll.cd[p];
+      const uint32_t nx0 = ll.next[p];
+      uint64_t cd1 = 0;
+      uint32_t nx1 = 0;
+      const uint32_t q = p - 1u;
+      if (kPair) {
+        cd1 = ll.cd[q];
+        nx1 = ll.next[q];
+      }
+      ll.next[p] = kUnset;
+      {
+        uint64_t ev;
+        uint32_t evc;
+        a.insert((cd0 & 0xFFFFFFFF00000000ull) | p, (uint32_t)cd0, ev, evc);
+        if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
+        ++cnt;
+      }
+      if (kPair && nx0 == q && q != 0u) {
+        ll.next[q] = kUnset;
+        uint64_t ev;
+        uint32_t evc;
+        a.insert((cd1 & 0xFFFFFFFF00000000ull) | q, (uint32_t)cd1, ev, evc);
+        if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
+        ++cnt;
+        p = nx1;
+      } else {
+        p = nx0;
+      }
+    }
+    const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
+    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if ((uint32_t)k < n) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
+    col = over(col, tailc);
+    float4 d = c.background;
+    if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+    o.out[t] = over(col, d);
+    if (kDebug) write_debug<K>(o, t, a, n, cnt, true, false);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Simple: oitSimple.frag.glsl
 // ---------------------------------------------------------------------------
@@ -647,6 +754,32 @@ struct Loop64Resolve {
   }
 };
 
+template <int K>
+struct LLResolveReset {
+  static hipError_t run(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, bool pair,
+                        hipStream_t s) {
+    const bool dbg = o.fk_key || o.fk_val || o.counts;
+    const bool ctail = tb.head != nullptr;
+    uint32_t blocks = blocks_for(c.T);
+    if (blocks > 8192u) blocks = 8192u;
+    if (blocks == 0) blocks = 1;
+    const dim3 g(blocks), b(kBlock);
+#define OIT_LLR(TAIL, CT, DBG, PAIR) \
+  hipLaunchKernelGGL((k_ll_resolve_reset<K, TAIL, CT, DBG, PAIR>), g, b, 0, s, c, ll, tb, o)
+#define OIT_LLR_P(TAIL, CT, DBG) \
+  do { if (pair) OIT_LLR(TAIL, CT, DBG, true); else OIT_LLR(TAIL, CT, DBG, false); } while (0)
+    if (c.tail) {
+      if (ctail) { if (dbg) OIT_LLR_P(true, true, true); else OIT_LLR_P(true, true, false); }
+      else { if (dbg) OIT_LLR_P(true, false, true); else OIT_LLR_P(true, false, false); }
+    } else {
+      if (dbg) OIT_LLR_P(false, false, true); else OIT_LLR_P(false, false, false);
+    }
+#undef OIT_LLR_P
+#undef OIT_LLR
+    return hipGetLastError();
+  }
+};
+
 }  // namespace
 
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
@@ -673,6 +806,21 @@ hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, h
 
 hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
   return dispatch_k<LLResolve>(c.K, c, ll, tb, o, s);
+}
+
+hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  uint32_t blocks = blocks_for(f.count);
+  if (blocks > 16384u) blocks = 16384u;
+  const dim3 g(blocks), b(kBlock);
+  if (c.tail && tb.head) hipLaunchKernelGGL((k_ll_append<true>), g, b, 0, s, c, f, ll, tb);
+  else hipLaunchKernelGGL((k_ll_append<false>), g, b, 0, s, c, f, ll, tb);
+  return hipGetLastError();
+}
+
+hipError_t launch_ll_resolve_reset(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o,
+                                   bool pair_walk, hipStream_t s) {
+  return dispatch_k<LLResolveReset>(c.K, c, ll, tb, o, pair_walk, s);
 }
 
 hipError_t launch_simple_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s) {

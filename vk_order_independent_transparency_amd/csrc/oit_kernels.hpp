@@ -34,6 +34,7 @@ struct LLBuf {
   uint32_t* next;     // [cap]
   uint64_t cap;       // nodes including the terminator
   uint32_t* overflow; // LL fragments that found the pool full
+  uint32_t* counters; // frame counters [16]; the resetting resolve moves them to counters[16..31]
 };
 
 struct KBuf {          // Simple / Spinlock
@@ -70,6 +71,15 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_ll_store(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s);
 hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, hipStream_t s);
 hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s);
+// Fused colour pass: store + ordered link in one kernel. Requires next[] of
+// the frame's nodes to hold kUnset on entry (left so by a resetting resolve
+// or by the clear).
+hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s);
+// Composite that also returns heads to 0, walked nodes' links to kUnset and
+// moves the frame counters aside; pair_walk consumes node p-1 in the same
+// round trip as p when the list continues there.
+hipError_t launch_ll_resolve_reset(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o,
+                                   bool pair_walk, hipStream_t s);
 
 hipError_t launch_simple_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s);
 hipError_t launch_spin_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s);

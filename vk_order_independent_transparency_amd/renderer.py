@@ -77,6 +77,7 @@ class FrameConfig:
     background: tuple[float, float, float, float] = (0.5, 0.5, 0.5, 1.0)
     wboit_view_near: float = 2.0
     wboit_view_span: float = 8.0
+    variant: int = 0
 
     @classmethod
     def from_state(cls, state: State, width: int, height: int, **kw) -> "FrameConfig":
@@ -114,6 +115,7 @@ class FrameConfig:
             c.background[i] = self.background[i]
         c.wboit_view_near = self.wboit_view_near
         c.wboit_view_span = self.wboit_view_span
+        c.variant = self.variant
         return c
 
 

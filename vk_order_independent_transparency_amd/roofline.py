@@ -21,13 +21,15 @@ from __future__ import annotations
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
 
 
-def ll_stage_bytes(F: int, P: int) -> dict:
-    return {
-        "clear": 4 * P,
-        "store": 32 * F,
-        "link": 16 * F,
-        "resolve": 4 * P + 12 * F + 16 * P,
-    }
+def ll_stage_bytes(F: int, P: int, fused: bool = True) -> dict:
+    """Per-launch algorithmic bytes. Fused append (default): stream in 24F,
+    node word 8F, link word 4F, head read-modify-write 8F. The resetting
+    resolve: heads 4P read + 4P reset, nodes 12F read + 4F link reset, 16P out.
+    Split append: store 32F (stream 20F in, node 8F + kUnset link 4F out) and
+    link 16F (target 4F, head read-modify-write 8F, link 4F)."""
+    if fused:
+        return {"clear": 0, "store": 44 * F, "resolve": 8 * P + 16 * F + 16 * P}
+    return {"clear": 0, "store": 32 * F, "link": 16 * F, "resolve": 8 * P + 16 * F + 16 * P}
 
 
 def ll_frame_bytes_reference(F: int, P: int) -> int:
