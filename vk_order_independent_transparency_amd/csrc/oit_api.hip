@@ -478,8 +478,9 @@ int oit_clear(oit_ctx* c, void* stream) {
   if (!need_ll_fill && !need_counters && !need_tail) {
     // steady state of the Linked List: the previous resolve left heads at 0,
     // links at kUnset and the counters at 0 (the clear of oitRender.cpp:
-    // 206-212 is done by the reader)
-    c->tl_head_valid = c->tl_head != nullptr;
+    // 206-212 is done by the reader). Tail-list heads were not cleared: they
+    // are filled on first use (ensure_tail_capacity).
+    c->tl_head_valid = false;
     c->frame_frags = 0;
     c->appends = 0;
     c->segments.clear();
@@ -522,7 +523,7 @@ int oit_clear(oit_ctx* c, void* stream) {
     c->counters_clean = true;
   }
   c->stats_moved = false;
-  c->tl_head_valid = c->tl_head != nullptr;
+  c->tl_head_valid = e == hipSuccess && need_tail;
   c->frame_frags = 0;
   c->appends = 0;
   c->segments.clear();

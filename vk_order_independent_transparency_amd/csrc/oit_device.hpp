@@ -204,8 +204,8 @@ __device__ __forceinline__ uint32_t min_relaxed(uint32_t* p, uint32_t v) {
 // one step for every active lane, and the owner's link store happens in the
 // trip of its successful atomic.
 __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32_t* next, uint32_t lo,
-                                                 uint32_t hi) {
-  bool active = want;
+                                                 uint32_t hi, uint32_t cap) {
+  bool active = want && hi < cap;
   bool first = true;
   uint32_t* p = p0;
   while (__ballot(active)) {
@@ -216,7 +216,7 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
         const uint32_t v = ld_relaxed(p);
         if (v != kUnset) {
           if (v > hi) {
-            p = &next[v];
+            if (v < cap) p = &next[v]; else active = false;  // a link outside the pool: stop
           } else {
             try_max = true;
           }
@@ -227,8 +227,10 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
         if (old < lo) {
           st_relaxed(&next[lo], old);
           active = false;
-        } else {
+        } else if (old < cap) {
           p = &next[old];  // a larger element got there first: continue behind it
+        } else {
+          active = false;
         }
       }
     }
@@ -239,8 +241,8 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
 // Inserts the single value x at link word p0 when `want`; same trip
 // structure as list_insert_desc.
 __device__ __forceinline__ void list_insert_asc(bool want, uint32_t* p0, uint32_t* next, uint32_t x,
-                                                uint32_t next_index) {
-  bool active = want;
+                                                uint32_t next_index, uint32_t cap) {
+  bool active = want && next_index < cap;
   bool first = true;
   uint32_t* p = p0;
   while (__ballot(active)) {
@@ -251,7 +253,7 @@ __device__ __forceinline__ void list_insert_asc(bool want, uint32_t* p0, uint32_
         const uint32_t v = ld_relaxed(p);
         if (v != kUnset) {
           if (v < x) {
-            p = &next[v >> 1];
+            if ((v >> 1) < cap) p = &next[v >> 1]; else active = false;  // a link outside the arrays: stop
           } else {
             try_min = true;
           }
@@ -262,8 +264,10 @@ __device__ __forceinline__ void list_insert_asc(bool want, uint32_t* p0, uint32_
         if (old > x) {
           st_relaxed(&next[next_index], old);
           active = false;
-        } else {
+        } else if ((old >> 1) < cap) {
           p = &next[old >> 1];
+        } else {
+          active = false;
         }
       }
     }

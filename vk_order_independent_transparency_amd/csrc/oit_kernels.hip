@@ -105,7 +105,7 @@ __device__ __forceinline__ float4 blend_front(const FrontK<K>& a, uint32_t n, co
 __device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float4 bg, uint32_t skip) {
   float4 d = bg;
   uint32_t link = tb.head[t];
-  while (link != kEmpty) {
+  while (link != kEmpty && (link >> 1) < tb.cap) {  // a link outside the arrays ends the walk
     const uint32_t id = link >> 1;
     const uint32_t nx = tb.next[id];
     if ((link & 1u) && skip > 0u) {
@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_link(Common c, FragView f, LLBuf 
   // chain inside the run; the run start lane's link is written by the insert
   if (valid && !r.start) st_relaxed(&ll.next[n], n - 1u);
   const bool ins = valid && r.end;
-  list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n);
+  list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n, (uint32_t)ll.cap);
 }
 
 // Composite pass, oitLinkedList.frag.glsl:106-172: walk the list (newest
@@ -220,7 +220,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
   float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t p = ll.head[t];
   uint32_t cnt = 0;
-  while (p != 0u) {
+  while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
     const uint64_t cd = ll.cd[p];
     const uint32_t nx = ll.next[p];
     const uint64_t x = (cd & 0xFFFFFFFF00000000ull) | p;  // (depth key, id + 1)
@@ -269,7 +269,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBu
     const Run r = same_target_run(node ? t : kEmpty);
     if (node && !r.start) st_relaxed(&ll.next[n], n - 1u);
     const bool ins = node && r.end;
-    list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n);
+    list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n, (uint32_t)ll.cap);
     const uint64_t om = __ballot(ovf);
     if (om && lane_id() == (uint32_t)__builtin_ctzll(om))
       atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
@@ -304,7 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
     uint32_t p = ll.head[t];
     if (p != 0u) ll.head[t] = 0u;
     uint32_t cnt = 0;
-    while (p != 0u) {
+    while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
       const uint64_t cd0 = ll.cd[p];
       const uint32_t nx0 = ll.next[p];
       uint64_t cd1 = 0;
@@ -664,7 +664,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb) {
     if (want) rec = tb.rec[m];
     const uint32_t t = (uint32_t)(rec >> 32);
     const uint32_t link = (uint32_t)rec;
-    list_insert_asc(want, want ? &tb.head[t] : nullptr, tb.next, link, link >> 1);
+    list_insert_asc(want, want ? &tb.head[t] : nullptr, tb.next, link, link >> 1, tb.cap);
   }
 }
 
