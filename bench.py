@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="OIT_VARIANT_* bits (kernel schedule; results identical)")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
+                    help="also run one device-to-device copy of MB megabytes (a known-traffic kernel for counter scaling)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the frame and request workloads "
@@ -95,6 +97,14 @@ def main():
     for _ in range(args.warmup):
         r.render(tg_d, d_d, c_d, out, stream)
     stream.synchronize()
+    if args.copy_calibration:
+        nbytes = args.copy_calibration << 20
+        src, dst = DeviceBuffer((nbytes,), "uint8"), DeviceBuffer((nbytes,), "uint8")
+        from vk_order_independent_transparency_amd import _abi
+        _abi.check("oit_memcpy_d2d", _abi.lib().oit_memcpy_d2d(dst.handle, src.handle, nbytes, stream.handle))
+        stream.synchronize()
+        log(f"calibration copy: {nbytes} bytes device to device")
+        del src, dst
 
     # timed region: exactly `steps` frames, HIP events on the launch stream
     ev = [Event() for _ in range(args.steps + 1)]

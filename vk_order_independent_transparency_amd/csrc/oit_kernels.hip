@@ -85,7 +85,38 @@ struct FrontK {
     sk[0] = first ? x : sk[0];
     col[0] = first ? xc : col[0];
   }
+  // The same insert when at most w - 1 slots can be occupied (w is wave
+  // uniform): slots at and beyond w hold the +inf sentinel both before and
+  // after the insert, so their compare/select steps are skipped with scalar
+  // branches. Exact for any w >= (occupied slots) + 1.
+  __device__ __forceinline__ void insert_bounded(uint64_t x, uint32_t xc, uint64_t& ev, uint32_t& evc, int w) {
+    const bool fits = x < sk[K - 1];
+    ev = fits ? sk[K - 1] : x;
+    evc = fits ? col[K - 1] : xc;
+#pragma unroll
+    for (int j = K - 1; j > 0; --j) {
+      if (j < w) {
+        const bool before = x < sk[j - 1];
+        const bool here = x < sk[j];
+        sk[j] = before ? sk[j - 1] : (here ? x : sk[j]);
+        col[j] = before ? col[j - 1] : (here ? xc : col[j]);
+      }
+    }
+    const bool first = x < sk[0];
+    sk[0] = first ? x : sk[0];
+    col[0] = first ? xc : col[0];
+  }
 };
+
+// Largest value of v over the active lanes, as a wave-uniform (scalar) value.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) {
+    const uint32_t w = __shfl_xor(v, o);
+    v = w > v ? w : v;
+  }
+  return __builtin_amdgcn_readfirstlane(v);
+}
 
 // Front-to-back blend of the first n sorted packed colours
 // (oitLinkedList.frag.glsl:162-166, oitSimple.frag.glsl:161-165).
@@ -304,7 +335,13 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
     uint32_t p = ll.head[t];
     if (p != 0u) ll.head[t] = 0u;
     uint32_t cnt = 0;
+    // trips of this walk so far; every active lane has the same value, so
+    // readfirstlane makes it a scalar bound on the occupied slots (a trip
+    // consumes at most two nodes)
+    uint32_t trip = 0;
     while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
+      const int w = (int)__builtin_amdgcn_readfirstlane(kPair ? 2u * trip + 2u : trip + 1u);
+      ++trip;
       const uint64_t cd0 = ll.cd[p];
       const uint32_t nx0 = ll.next[p];
       uint64_t cd1 = 0;
@@ -318,7 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
       {
         uint64_t ev;
         uint32_t evc;
-        a.insert((cd0 & 0xFFFFFFFF00000000ull) | p, (uint32_t)cd0, ev, evc);
+        a.insert_bounded((cd0 & 0xFFFFFFFF00000000ull) | p, (uint32_t)cd0, ev, evc, w);
         if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
         ++cnt;
       }
@@ -326,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
         ll.next[q] = kUnset;
         uint64_t ev;
         uint32_t evc;
-        a.insert((cd1 & 0xFFFFFFFF00000000ull) | q, (uint32_t)cd1, ev, evc);
+        a.insert_bounded((cd1 & 0xFFFFFFFF00000000ull) | q, (uint32_t)cd1, ev, evc, w);
         if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
         ++cnt;
         p = nx1;
@@ -335,10 +372,11 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
       }
     }
     const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
+    const uint32_t nmax = wave_max_u32(n);
     float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
     for (int k = 0; k < K; ++k)
-      if ((uint32_t)k < n) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
+      if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
     col = over(col, tailc);
     float4 d = c.background;
     if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
