@@ -173,6 +173,17 @@ int oit_profile_read(oit_ctx* ctx, oit_profile* out);
 /* Zeroes the accumulated stage times. */
 int oit_profile_reset(oit_ctx* ctx);
 
+/* ---- Codec checks (no reference counterpart). ------------------------- */
+/* The library's sRGB8 tables: encode thresholds thr[0..256] (thr[0] = -inf,
+ * thr[256] = +inf), decode dec[0..255], alpha[0..255]. Host computation. */
+int oit_codec_tables(float* thr257, float* dec256, float* alpha256);
+/* Runs the device's branch-free encoder against its exact table search on
+ * every binary32 bit pattern in [lo_bits, hi_bits) and reports the number of
+ * differing codes and the first differing pattern (0xFFFFFFFF if none).
+ * Synchronous. */
+int oit_selftest_codec(oit_ctx* ctx, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches,
+                       uint32_t* first_bad);
+
 /* ---- Device plumbing for hosts without their own HIP binding (no
  * reference counterpart: the reference allocates through VMA, oit.h:173-214,
  * and times with its profiler). Thin wrappers of hipMalloc/hipMemcpy/

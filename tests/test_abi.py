@@ -121,3 +121,19 @@ def test_state_mirrors_reference_antialiasing_rules():
     s.aaType = _abi.AA_MSAA_4X
     with pytest.raises(ValueError):
         FrameConfig.from_state(s, 64, 32)
+
+
+def test_library_codec_tables_equal_the_oracle(oracle):
+    """The product computes its sRGB8 tables with its own host code; they must equal the
+    oracle's (which test_oracle_codec.py pins to the exact curve)."""
+    import numpy as np
+
+    thr = np.zeros(257, np.float32)
+    dec = np.zeros(256, np.float32)
+    alpha = np.zeros(256, np.float32)
+    assert _abi.lib().oit_codec_tables(thr.ctypes.data, dec.ctypes.data, alpha.ctypes.data) == 0
+    othr, odec, oalpha = oracle.tables()
+    assert thr[0] == -np.inf and thr[256] == np.inf
+    assert np.array_equal(thr[1:256].view(np.uint32), othr[1:256].view(np.uint32))
+    assert np.array_equal(dec.view(np.uint32), odec.view(np.uint32))
+    assert np.array_equal(alpha.view(np.uint32), oalpha.view(np.uint32))

@@ -19,6 +19,22 @@ pytestmark = pytest.mark.gpu
 LL, SIMPLE, LOOP64, SPIN, WB = 1, 0, 3, 4, 6
 
 
+@pytest.mark.parametrize("lo,hi", [
+    (0x00000000, 0x7F800001),  # every non-negative float up to +inf
+    (0x7F800001, 0x7F900000),  # NaNs
+    (0x80000000, 0x80900000),  # negative zero and small negatives
+    (0xBF000000, 0xBF900000),  # negatives around -0.5 .. -1.1
+])
+def test_branch_free_encode_is_exact_on_device(lo, hi):
+    """The kernels' branch-free sRGB8 encode (v_log_f32 / v_exp_f32 start, two
+    thresholds) equals the exact table search for every input in the range."""
+    from vk_order_independent_transparency_amd import Renderer
+
+    with Renderer(0) as r:
+        bad, first = r.selftest_codec(lo, hi)
+    assert bad == 0, f"{bad} mismatches, first at bits {first:#010x}"
+
+
 def _say(*a):
     print(*a, flush=True)
 

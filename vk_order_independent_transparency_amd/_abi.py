@@ -134,6 +134,9 @@ SIGNATURES = {
     "oit_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "oit_profile_read": (ctypes.c_int, [_P, ctypes.POINTER(OitProfile)]),
     "oit_profile_reset": (ctypes.c_int, [_P]),
+    "oit_codec_tables": (ctypes.c_int, [_P, _P, _P]),
+    "oit_selftest_codec": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(ctypes.c_uint64),
+                                          ctypes.POINTER(ctypes.c_uint32)]),
     "oit_device_count": (ctypes.c_int, [ctypes.POINTER(ctypes.c_int)]),
     "oit_set_device": (ctypes.c_int, [ctypes.c_int]),
     "oit_device_alloc": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_uint64]),

@@ -684,6 +684,41 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
   return OIT_OK;
 }
 
+int oit_codec_tables(float* thr257, float* dec256, float* alpha256) {
+  if (!thr257 || !dec256 || !alpha256) return OIT_ERR_INVALID_ARG;
+  oit::CodecTables t;
+  build_codec(&t);
+  std::memcpy(thr257, t.thr, sizeof t.thr);
+  std::memcpy(dec256, t.dec, sizeof t.dec);
+  std::memcpy(alpha256, t.alpha, sizeof t.alpha);
+  return OIT_OK;
+}
+
+int oit_selftest_codec(oit_ctx* c, uint32_t lo_bits, uint32_t hi_bits, uint64_t* mismatches, uint32_t* first_bad) {
+  if (!c || !mismatches || !first_bad || hi_bits < lo_bits) return OIT_ERR_INVALID_ARG;
+  DeviceGuard g(c->device);
+  struct {
+    unsigned long long bad;
+    uint32_t first;
+    uint32_t pad;
+  } h{0ull, 0xFFFFFFFFu, 0u};
+  void* d = nullptr;
+  if (hipMalloc(&d, sizeof h) != hipSuccess) return OIT_ERR_OOM;
+  int rc = OIT_OK;
+  if (hipMemcpy(d, &h, sizeof h, hipMemcpyHostToDevice) != hipSuccess) rc = OIT_ERR_DEVICE;
+  if (!rc && hi_bits > lo_bits) {
+    auto* dp = static_cast<unsigned long long*>(d);
+    if (oit::launch_codec_selftest(c->d_tab, lo_bits, hi_bits, dp, reinterpret_cast<uint32_t*>(dp + 1), nullptr) !=
+        hipSuccess)
+      rc = OIT_ERR_DEVICE;
+  }
+  if (!rc && hipMemcpy(&h, d, sizeof h, hipMemcpyDeviceToHost) != hipSuccess) rc = OIT_ERR_DEVICE;
+  (void)hipFree(d);
+  *mismatches = h.bad;
+  *first_bad = h.first;
+  return rc;
+}
+
 int oit_profile_enable(oit_ctx* c, int enable) {
   if (!c) return OIT_ERR_INVALID_ARG;
   c->profiling = enable != 0;

@@ -70,10 +70,11 @@ __device__ __forceinline__ uint32_t encode_color(float4 p, const CodecTables* __
   return code_rgb(r, tab->thr) | (code_rgb(g, tab->thr) << 8) | (code_rgb(b, tab->thr) << 16) | (qa << 24);
 }
 
-// The same code with the table in LDS and both neighbouring thresholds read
-// together: the approximation is exact to within one code, so one round trip
-// settles it almost always.
-__device__ __forceinline__ uint32_t code_rgb_lds(float c, const float* thr) {
+// Branch-free code with the threshold table in LDS: the curve evaluated with
+// v_log_f32 / v_exp_f32 lands within one code of the exact one, and the two
+// neighbouring thresholds settle it. oit_selftest_codec checks this against
+// code_rgb over every non-negative binary32 on the device.
+__device__ __forceinline__ uint32_t code_rgb_fast(float c, const float* thr) {
   float s;
   if (c < 0.0031308f) {
     s = c * 12.92f;
@@ -81,17 +82,11 @@ __device__ __forceinline__ uint32_t code_rgb_lds(float c, const float* thr) {
     s = 1.055f * __builtin_amdgcn_exp2f(__builtin_amdgcn_logf(c) * (1.0f / 2.4f)) - 0.055f;
   }
   int q = (int)fminf(fmaxf(s * 255.0f + 0.5f, 0.0f), 255.0f);
-  for (;;) {
-    const float lo = thr[q];
-    const float hi = thr[q + 1];
-    if (q > 0 && !(c >= lo)) {
-      --q;
-    } else if (q < 255 && c >= hi) {
-      ++q;
-    } else {
-      return (uint32_t)q;
-    }
-  }
+  const float lo = thr[q];
+  const float hi = thr[q + 1];
+  q -= (q > 0 && c < lo) ? 1 : 0;
+  q += (q < 255 && c >= hi) ? 1 : 0;
+  return (uint32_t)q;
 }
 
 __device__ __forceinline__ uint32_t encode_color_lds(float4 p, const float* thr) {
@@ -104,7 +99,7 @@ __device__ __forceinline__ uint32_t encode_color_lds(float4 p, const float* thr)
   }
   const float ac = fminf(fmaxf(a, 0.0f), 1.0f);
   const uint32_t qa = (uint32_t)floorf(ac * 255.0f + 0.5f);
-  return code_rgb_lds(r, thr) | (code_rgb_lds(g, thr) << 8) | (code_rgb_lds(b, thr) << 16) | (qa << 24);
+  return code_rgb_fast(r, thr) | (code_rgb_fast(g, thr) << 8) | (code_rgb_fast(b, thr) << 16) | (qa << 24);
 }
 
 __device__ __forceinline__ float4 decode_premult_lds(uint32_t c, const float* dec, const float* alpha) {

@@ -86,25 +86,28 @@ struct FrontK {
     col[0] = first ? xc : col[0];
   }
   // The same insert when at most w - 1 slots can be occupied (w is wave
-  // uniform): slots at and beyond w hold the +inf sentinel both before and
-  // after the insert, so their compare/select steps are skipped with scalar
-  // branches. Exact for any w >= (occupied slots) + 1.
+  // uniform): slots at and beyond w hold the +inf sentinel before and after
+  // the insert, so the walk over slots stops there (ascending order carries
+  // the displaced element; the scalar exit makes the cost grow with w, not K).
   __device__ __forceinline__ void insert_bounded(uint64_t x, uint32_t xc, uint64_t& ev, uint32_t& evc, int w) {
     const bool fits = x < sk[K - 1];
     ev = fits ? sk[K - 1] : x;
     evc = fits ? col[K - 1] : xc;
+    bool moved = false;  // an earlier slot took x, so this slot takes its old neighbour
+    uint64_t prev = 0;
+    uint32_t prevc = 0;
 #pragma unroll
-    for (int j = K - 1; j > 0; --j) {
-      if (j < w) {
-        const bool before = x < sk[j - 1];
-        const bool here = x < sk[j];
-        sk[j] = before ? sk[j - 1] : (here ? x : sk[j]);
-        col[j] = before ? col[j - 1] : (here ? xc : col[j]);
-      }
+    for (int j = 0; j < K; ++j) {
+      if (j >= w) break;
+      const uint64_t o = sk[j];
+      const uint32_t oc = col[j];
+      const bool here = x < o;
+      sk[j] = moved ? prev : (here ? x : o);
+      col[j] = moved ? prevc : (here ? xc : oc);
+      moved = moved || here;
+      prev = o;
+      prevc = oc;
     }
-    const bool first = x < sk[0];
-    sk[0] = first ? x : sk[0];
-    col[0] = first ? xc : col[0];
   }
 };
 
@@ -192,8 +195,16 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, u
 // fragment in draw order; here the allocation is the fragment's own index, so
 // no global counter is contended), overflow -> tail (:59-68), node payload
 // (:75-82). The link word is set to kUnset for the link kernel.
+// Codec thresholds staged in LDS for the block.
+__device__ __forceinline__ void stage_thresholds(float* s_thr, const CodecTables* tab) {
+  for (uint32_t k = threadIdx.x; k < 257u; k += kBlock) s_thr[k] = tab->thr[k];
+  __syncthreads();
+}
+
 template <bool kTail>
 __global__ __launch_bounds__(kBlock) void k_ll_store(Common c, FragView f, LLBuf ll, TailBuf tb) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const bool valid = i < f.count;
   const uint32_t lid = f.id_base + i;
@@ -208,7 +219,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_store(Common c, FragView f, LLBuf
       ovf = t < c.T;
     } else {
       const uint32_t key = depth_key(f.depth[i]);
-      ll.cd[n] = ((uint64_t)key << 32) | encode_color(col, c.tab);
+      ll.cd[n] = ((uint64_t)key << 32) | encode_color_lds(col, s_thr);
       ll.next[n] = kUnset;
     }
   }
@@ -280,8 +291,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
 template <bool kTail>
 __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBuf ll, TailBuf tb) {
   __shared__ float s_thr[257];
-  for (uint32_t k = threadIdx.x; k < 257u; k += kBlock) s_thr[k] = c.tab->thr[k];
-  __syncthreads();
+  stage_thresholds(s_thr, c.tab);
   const uint32_t nchunks = (f.count + kBlock - 1) / kBlock;
   for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
     const uint32_t i = chunk * kBlock + threadIdx.x;
@@ -394,6 +404,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
 // A run of same-target lanes takes its slots with one atomicAdd.
 template <bool kTail>
 __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, KBuf kb, TailBuf tb) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
   uint32_t t = valid ? f.target[i] : kEmpty;
@@ -413,7 +425,7 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
     col = f.rgba[i];
     if (slot < c.K) {
       const uint64_t idx = (uint64_t)slot * c.T + t;
-      kb.col[idx] = encode_color(col, c.tab);
+      kb.col[idx] = encode_color_lds(col, s_thr);
       kb.key[idx] = depth_key(f.depth[i]);
       kb.id[idx] = lid;
     } else {
@@ -436,6 +448,8 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
 // recorded for the tail with their packed sRGB8 colour.
 template <int K, bool kTail>
 __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBuf kb, TailBuf tb) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
   uint32_t t = valid ? f.target[i] : kEmpty;
@@ -448,7 +462,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   uint32_t my_key = 0u, my_col = 0u;
   if (valid) {
     const uint32_t key = depth_key(f.depth[i]);
-    const uint32_t col = encode_color(f.rgba[i], c.tab);
+    const uint32_t col = encode_color_lds(f.rgba[i], s_thr);
     tail_col = col;
     pending = key <= ld_relaxed(&kb.depth[t]);  // :66-69 early depth test
     if (!pending) tail = true;
@@ -543,11 +557,13 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
 // any arrival order; evictions are classified afterwards.
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, uint64_t* __restrict__ slots) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= f.count) return;
   const uint32_t t = f.target[i];
   if (t >= c.T) return;
-  uint64_t z = ((uint64_t)depth_key(f.depth[i]) << 32) | encode_color(f.rgba[i], c.tab);
+  uint64_t z = ((uint64_t)depth_key(f.depth[i]) << 32) | encode_color_lds(f.rgba[i], s_thr);
   if (z > slots[(uint64_t)(K - 1) * c.T + t]) return;
   int j = 0;
   if (z > slots[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
@@ -566,6 +582,8 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, u
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f, const uint64_t* __restrict__ slots,
                                                             TailBuf tb) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
   uint32_t t = valid ? f.target[i] : kEmpty;
@@ -574,7 +592,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
   uint32_t flag = 0;
   float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
   if (valid) {
-    const uint32_t col = encode_color(f.rgba[i], c.tab);
+    const uint32_t col = encode_color_lds(f.rgba[i], s_thr);
     const uint64_t z = ((uint64_t)depth_key(f.depth[i]) << 32) | col;
     const uint64_t s = slots[(uint64_t)(K - 1) * c.T + t];
     if (z >= s) {
@@ -703,6 +721,28 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb) {
     const uint32_t t = (uint32_t)(rec >> 32);
     const uint32_t link = (uint32_t)rec;
     list_insert_asc(want, want ? &tb.head[t] : nullptr, tb.next, link, link >> 1, tb.cap);
+  }
+}
+
+// Device self-check of the branch-free encode against the exact table search
+// over a range of binary32 bit patterns.
+__global__ __launch_bounds__(kBlock) void k_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t hi,
+                                                           unsigned long long* mismatches, uint32_t* first_bad) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, tab);
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  unsigned long long bad = 0;
+  uint32_t first = 0xFFFFFFFFu;
+  for (uint64_t b = (uint64_t)lo + (uint64_t)blockIdx.x * kBlock + threadIdx.x; b < hi; b += stride) {
+    const float c = __uint_as_float((uint32_t)b);
+    if (code_rgb_fast(c, s_thr) != code_rgb(c, tab->thr)) {
+      ++bad;
+      if ((uint32_t)b < first) first = (uint32_t)b;
+    }
+  }
+  if (bad) {
+    atomicAdd(mismatches, bad);
+    atomicMin(first_bad, first);
   }
 }
 
@@ -903,6 +943,12 @@ hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& w
 
 hipError_t launch_wboit_resolve(const Common& c, const WBBuf& wb, const ResolveOut& o, hipStream_t s) {
   hipLaunchKernelGGL(k_wboit_resolve, dim3(blocks_for(c.T)), dim3(kBlock), 0, s, c, wb, o);
+  return hipGetLastError();
+}
+
+hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t hi, unsigned long long* mismatches,
+                                 uint32_t* first_bad, hipStream_t s) {
+  hipLaunchKernelGGL(k_codec_selftest, dim3(4096), dim3(kBlock), 0, s, tab, lo, hi, mismatches, first_bad);
   return hipGetLastError();
 }
 

@@ -97,4 +97,7 @@ hipError_t launch_wboit_resolve(const Common& c, const WBBuf& wb, const ResolveO
 
 hipError_t launch_tail_link(const TailBuf& tb, hipStream_t s);
 
+hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t hi, unsigned long long* mismatches,
+                                 uint32_t* first_bad, hipStream_t s);
+
 }  // namespace oit

@@ -209,6 +209,15 @@ class Renderer:
         _abi.check("oit_get_stats", self._lib.oit_get_stats(self._ctx, ctypes.byref(out), _stream_ptr(stream)))
         return out
 
+    def selftest_codec(self, lo_bits: int, hi_bits: int) -> tuple[int, int]:
+        """(mismatches, first differing bit pattern) of the device's branch-free
+        sRGB8 encode against its exact table search over [lo_bits, hi_bits)."""
+        bad = ctypes.c_uint64(0)
+        first = ctypes.c_uint32(0)
+        _abi.check("oit_selftest_codec", self._lib.oit_selftest_codec(self._ctx, lo_bits, hi_bits,
+                                                                      ctypes.byref(bad), ctypes.byref(first)))
+        return int(bad.value), int(first.value)
+
     def profile_enable(self, on: bool = True) -> None:
         _abi.check("oit_profile_enable", self._lib.oit_profile_enable(self._ctx, 1 if on else 0))
 
