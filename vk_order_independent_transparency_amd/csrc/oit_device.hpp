@@ -232,6 +232,39 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
   }
 }
 
+// list_insert_desc for a lane that already tried the head and learned that a
+// larger element is there: it starts at that element's link word `p0` and
+// reads it before its first atomic.
+__device__ __forceinline__ void list_insert_desc_from(bool want, uint32_t* p0, uint32_t* next, uint32_t lo,
+                                                      uint32_t hi, uint32_t cap) {
+  bool active = want && hi < cap;
+  uint32_t* p = p0;
+  while (__ballot(active)) {
+    if (active) {
+      bool try_max = false;
+      const uint32_t v = ld_relaxed(p);
+      if (v != kUnset) {
+        if (v > hi) {
+          if (v < cap) p = &next[v]; else active = false;
+        } else {
+          try_max = true;
+        }
+      }
+      if (try_max) {
+        const uint32_t old = max_relaxed(p, hi);
+        if (old < lo) {
+          st_relaxed(&next[lo], old);
+          active = false;
+        } else if (old < cap) {
+          p = &next[old];
+        } else {
+          active = false;
+        }
+      }
+    }
+  }
+}
+
 // Ascending list (head holds the smallest value, kEmpty terminates).
 // Inserts the single value x at link word p0 when `want`; same trip
 // structure as list_insert_desc.

@@ -87,8 +87,9 @@ struct FrontK {
   }
   // The same insert when at most w - 1 slots can be occupied (w is wave
   // uniform): slots at and beyond w hold the +inf sentinel before and after
-  // the insert, so the walk over slots stops there (ascending order carries
-  // the displaced element; the scalar exit makes the cost grow with w, not K).
+  // the insert. Slots are walked upward carrying the displaced element, in
+  // blocks of four; a block that starts at or beyond w is skipped with one
+  // scalar branch (executing more slots than needed is also exact).
   __device__ __forceinline__ void insert_bounded(uint64_t x, uint32_t xc, uint64_t& ev, uint32_t& evc, int w) {
     const bool fits = x < sk[K - 1];
     ev = fits ? sk[K - 1] : x;
@@ -97,16 +98,20 @@ struct FrontK {
     uint64_t prev = 0;
     uint32_t prevc = 0;
 #pragma unroll
-    for (int j = 0; j < K; ++j) {
-      if (j >= w) break;
-      const uint64_t o = sk[j];
-      const uint32_t oc = col[j];
-      const bool here = x < o;
-      sk[j] = moved ? prev : (here ? x : o);
-      col[j] = moved ? prevc : (here ? xc : oc);
-      moved = moved || here;
-      prev = o;
-      prevc = oc;
+    for (int b = 0; b < K; b += 4) {
+      if (b < w) {
+#pragma unroll
+        for (int j = b; j < b + 4 && j < K; ++j) {
+          const uint64_t o = sk[j];
+          const uint32_t oc = col[j];
+          const bool here = x < o;
+          sk[j] = moved ? prev : (here ? x : o);
+          col[j] = moved ? prevc : (here ? xc : oc);
+          moved = moved || here;
+          prev = o;
+          prevc = oc;
+        }
+      }
     }
   }
 };
@@ -283,38 +288,79 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
 }
 
 // Fused colour pass of the Linked List: store and ordered link in one
-// kernel (oitLinkedList.frag.glsl:51-85). Blocks loop over 256-fragment
-// chunks so the encode table is staged in LDS once per block. The link words
-// of the frame's nodes already hold kUnset (left by the previous frame's
-// resetting resolve, or by the clear), so a walker can never follow a stale
-// link; the run chain n -> n-1 and the inserts are relaxed atomics.
+// kernel (oitLinkedList.frag.glsl:51-85). Each wave takes kU consecutive
+// 64-fragment chunks and issues their loads, then their stores, then their
+// head inserts back to back, so kU memory round trips overlap per wave; only
+// lanes that find a larger element already at the head walk the list. The
+// link words of the frame's nodes already hold kUnset (left by the previous
+// frame's resetting resolve, or by the clear), so a walker never follows a
+// stale link; the run chain n -> n-1 and the inserts are relaxed atomics.
+constexpr int kAppendChunks = 4;
+
 template <bool kTail>
 __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBuf ll, TailBuf tb) {
+  constexpr int kU = kAppendChunks;
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
-  const uint32_t nchunks = (f.count + kBlock - 1) / kBlock;
-  for (uint32_t chunk = blockIdx.x; chunk < nchunks; chunk += gridDim.x) {
-    const uint32_t i = chunk * kBlock + threadIdx.x;
-    const bool valid = i < f.count;
-    const uint32_t lid = f.id_base + i;
-    const uint64_t n64 = (uint64_t)lid + 1u;
-    uint32_t t = valid ? f.target[i] : kEmpty;
-    if (t >= c.T) t = kEmpty;
-    const bool in_frame = t != kEmpty;
-    const bool node = in_frame && n64 < ll.cap;
-    const bool ovf = in_frame && !node;
-    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-    if (in_frame) col = f.rgba[i];
-    const uint32_t n = (uint32_t)n64;
-    if (node) ll.cd[n] = ((uint64_t)depth_key(f.depth[i]) << 32) | encode_color_lds(col, s_thr);
-    const Run r = same_target_run(node ? t : kEmpty);
-    if (node && !r.start) st_relaxed(&ll.next[n], n - 1u);
-    const bool ins = node && r.end;
-    list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n, (uint32_t)ll.cap);
-    const uint64_t om = __ballot(ovf);
-    if (om && lane_id() == (uint32_t)__builtin_ctzll(om))
-      atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
-    if constexpr (kTail) tail_record(ovf, t, lid, 0u, col, tb);
+  const uint32_t lane = lane_id();
+  const uint64_t wave_base = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
+  uint32_t t[kU];
+  bool node[kU], ovf[kU];
+  float4 col[kU];
+  float dep[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t i = wave_base + (uint64_t)u * 64u + lane;
+    t[u] = i < f.count ? f.target[i] : kEmpty;
+    if (t[u] >= c.T) t[u] = kEmpty;
+    const uint64_t n64 = (uint64_t)f.id_base + i + 1u;
+    node[u] = t[u] != kEmpty && n64 < ll.cap;
+    ovf[u] = t[u] != kEmpty && !node[u];
+    col[u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    dep[u] = 0.f;
+    if (t[u] != kEmpty) col[u] = f.rgba[i];
+    if (node[u]) dep[u] = f.depth[i];
+  }
+  uint32_t lo[kU], hi[kU];
+  bool ins[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint32_t n = (uint32_t)(f.id_base + wave_base + (uint64_t)u * 64u + lane + 1u);
+    if (node[u]) ll.cd[n] = ((uint64_t)depth_key(dep[u]) << 32) | encode_color_lds(col[u], s_thr);
+    const Run r = same_target_run(node[u] ? t[u] : kEmpty);
+    if (node[u] && !r.start) st_relaxed(&ll.next[n], n - 1u);
+    ins[u] = node[u] && r.end;
+    hi[u] = n;
+    lo[u] = n - (r.lane - r.start_lane);
+  }
+  // first insert attempt of every chunk: the atomics are in flight together
+  uint32_t old[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    old[u] = 0;
+    if (ins[u]) old[u] = max_relaxed(&ll.head[t[u]], hi[u]);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    if (ins[u] && old[u] < lo[u]) {
+      st_relaxed(&ll.next[lo[u]], old[u]);
+      ins[u] = false;
+    }
+  }
+  // a larger element was already at the head: continue behind it
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const bool walk = ins[u] && old[u] < ll.cap;
+    list_insert_desc_from(walk, walk ? &ll.next[old[u]] : nullptr, ll.next, lo[u], hi[u], (uint32_t)ll.cap);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t om = __ballot(ovf[u]);
+    if (om && lane == (uint32_t)__builtin_ctzll(om)) atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
+    if constexpr (kTail) {
+      const uint32_t lid = (uint32_t)(f.id_base + wave_base + (uint64_t)u * 64u + lane);
+      tail_record(ovf[u], t[u], lid, 0u, col[u], tb);
+    }
   }
 }
 
@@ -888,9 +934,8 @@ hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb
 
 hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s) {
   if (f.count == 0) return hipSuccess;
-  uint32_t blocks = blocks_for(f.count);
-  if (blocks > 16384u) blocks = 16384u;
-  const dim3 g(blocks), b(kBlock);
+  const uint64_t per_block = (uint64_t)kBlock * kAppendChunks;
+  const dim3 g((uint32_t)((f.count + per_block - 1) / per_block)), b(kBlock);
   if (c.tail && tb.head) hipLaunchKernelGGL((k_ll_append<true>), g, b, 0, s, c, f, ll, tb);
   else hipLaunchKernelGGL((k_ll_append<false>), g, b, 0, s, c, f, ll, tb);
   return hipGetLastError();
