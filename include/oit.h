@@ -51,6 +51,7 @@ enum oit_algorithm {
  * variant, only the kernel schedule differs. */
 #define OIT_VARIANT_LL_SPLIT_APPEND  0x1u /* Linked List: separate store and link kernels */
 #define OIT_VARIANT_LL_SINGLE_WALK   0x2u /* Linked List: resolve walks one node per round trip */
+#define OIT_VARIANT_LL_APPEND_8      0x4u /* Linked List: fused append takes 8 chunks per wave (default 4) */
 
 /* Status codes. */
 enum oit_status {

@@ -570,7 +570,9 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
         e = staged(c, kStStore, s, [&] { return oit::launch_ll_store(k, v, ll, tb, s); });
         if (e == hipSuccess) e = staged(c, kStLink, s, [&] { return oit::launch_ll_link(k, v, ll, s); });
       } else {
-        e = staged(c, kStStore, s, [&] { return oit::launch_ll_append(k, v, ll, tb, s); });
+        e = staged(c, kStStore, s, [&] {
+          return oit::launch_ll_append(k, v, ll, tb, (c->cfg.variant & OIT_VARIANT_LL_APPEND_8) ? 8 : 4, s);
+        });
       }
       break;
     }

@@ -87,32 +87,27 @@ struct FrontK {
   }
   // The same insert when at most w - 1 slots can be occupied (w is wave
   // uniform): slots at and beyond w hold the +inf sentinel before and after
-  // the insert. Slots are walked upward carrying the displaced element, in
-  // blocks of four; a block that starts at or beyond w is skipped with one
-  // scalar branch (executing more slots than needed is also exact).
+  // the insert, so whole blocks of four slots at or beyond w are skipped
+  // with one scalar branch each (executing more slots is also exact).
   __device__ __forceinline__ void insert_bounded(uint64_t x, uint32_t xc, uint64_t& ev, uint32_t& evc, int w) {
     const bool fits = x < sk[K - 1];
     ev = fits ? sk[K - 1] : x;
     evc = fits ? col[K - 1] : xc;
-    bool moved = false;  // an earlier slot took x, so this slot takes its old neighbour
-    uint64_t prev = 0;
-    uint32_t prevc = 0;
 #pragma unroll
-    for (int b = 0; b < K; b += 4) {
+    for (int b = ((K - 1) / 4) * 4; b >= 0; b -= 4) {
       if (b < w) {
 #pragma unroll
-        for (int j = b; j < b + 4 && j < K; ++j) {
-          const uint64_t o = sk[j];
-          const uint32_t oc = col[j];
-          const bool here = x < o;
-          sk[j] = moved ? prev : (here ? x : o);
-          col[j] = moved ? prevc : (here ? xc : oc);
-          moved = moved || here;
-          prev = o;
-          prevc = oc;
+        for (int j = (b + 3 < K - 1 ? b + 3 : K - 1); j >= b && j > 0; --j) {
+          const bool before = x < sk[j - 1];
+          const bool here = x < sk[j];
+          sk[j] = before ? sk[j - 1] : (here ? x : sk[j]);
+          col[j] = before ? col[j - 1] : (here ? xc : col[j]);
         }
       }
     }
+    const bool first = x < sk[0];
+    sk[0] = first ? x : sk[0];
+    col[0] = first ? xc : col[0];
   }
 };
 
@@ -295,11 +290,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
 // link words of the frame's nodes already hold kUnset (left by the previous
 // frame's resetting resolve, or by the clear), so a walker never follows a
 // stale link; the run chain n -> n-1 and the inserts are relaxed atomics.
-constexpr int kAppendChunks = 4;
-
-template <bool kTail>
+template <bool kTail, int kU>
 __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBuf ll, TailBuf tb) {
-  constexpr int kU = kAppendChunks;
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
   const uint32_t lane = lane_id();
@@ -932,12 +924,20 @@ hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb
   return dispatch_k<LLResolve>(c.K, c, ll, tb, o, s);
 }
 
-hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s) {
+hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, int chunks,
+                            hipStream_t s) {
   if (f.count == 0) return hipSuccess;
-  const uint64_t per_block = (uint64_t)kBlock * kAppendChunks;
+  const int u = chunks >= 8 ? 8 : 4;
+  const uint64_t per_block = (uint64_t)kBlock * u;
   const dim3 g((uint32_t)((f.count + per_block - 1) / per_block)), b(kBlock);
-  if (c.tail && tb.head) hipLaunchKernelGGL((k_ll_append<true>), g, b, 0, s, c, f, ll, tb);
-  else hipLaunchKernelGGL((k_ll_append<false>), g, b, 0, s, c, f, ll, tb);
+  const bool tail = c.tail && tb.head;
+  if (u == 8) {
+    if (tail) hipLaunchKernelGGL((k_ll_append<true, 8>), g, b, 0, s, c, f, ll, tb);
+    else hipLaunchKernelGGL((k_ll_append<false, 8>), g, b, 0, s, c, f, ll, tb);
+  } else {
+    if (tail) hipLaunchKernelGGL((k_ll_append<true, 4>), g, b, 0, s, c, f, ll, tb);
+    else hipLaunchKernelGGL((k_ll_append<false, 4>), g, b, 0, s, c, f, ll, tb);
+  }
   return hipGetLastError();
 }
 

@@ -74,7 +74,8 @@ hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb
 // Fused colour pass: store + ordered link in one kernel. Requires next[] of
 // the frame's nodes to hold kUnset on entry (left so by a resetting resolve
 // or by the clear).
-hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s);
+hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, int chunks,
+                            hipStream_t s);
 // Composite that also returns heads to 0, walked nodes' links to kUnset and
 // moves the frame counters aside; pair_walk consumes node p-1 in the same
 // round trip as p when the list continues there.
