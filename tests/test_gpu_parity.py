@@ -67,7 +67,7 @@ def test_linkedlist_bitexact(oracle, K, tail):
     assert (per_target_counts(tg, p.targets) > K).any(), "scene must engage the tail"
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 6])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4, 6, 8, 10])
 @pytest.mark.parametrize("K", [4, 16])
 def test_linkedlist_kernel_variants_bitexact(oracle, variant, K):
     """Fused or split append, pair or single walk: identical results."""

@@ -111,6 +111,35 @@ struct FrontK {
   }
 };
 
+// Sorted register array of K sort keys only (depth key << 32 | node index),
+// ascending: the colours are reloaded by node index after the walk, which
+// keeps one third fewer registers live during the latency-bound walk.
+template <int K>
+struct FrontKeys {
+  uint64_t sk[K];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int k = 0; k < K; ++k) sk[k] = kEmpty64;
+  }
+  // insertionSortTail with a wave-uniform bound w on the occupied slots + 1
+  __device__ __forceinline__ uint64_t insert_bounded(uint64_t x, int w) {
+    const uint64_t ev = x < sk[K - 1] ? sk[K - 1] : x;
+#pragma unroll
+    for (int b = ((K - 1) / 4) * 4; b >= 0; b -= 4) {
+      if (b < w) {
+#pragma unroll
+        for (int j = (b + 3 < K - 1 ? b + 3 : K - 1); j >= b && j > 0; --j) {
+          const bool before = x < sk[j - 1];
+          const bool here = x < sk[j];
+          sk[j] = before ? sk[j - 1] : (here ? x : sk[j]);
+        }
+      }
+    }
+    sk[0] = x < sk[0] ? x : sk[0];
+    return ev;
+  }
+};
+
 // Largest value of v over the active lanes, as a wave-uniform (scalar) value.
 __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 #pragma unroll
@@ -430,6 +459,91 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
     if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
     o.out[t] = over(col, d);
     if (kDebug) write_debug<K>(o, t, a, n, cnt, true, false);
+  }
+}
+
+// Key-only form of k_ll_resolve_reset: the walk keeps (depth key, node)
+// pairs and the front-K colours are gathered by node index at the end.
+template <int K, bool kTail, bool kColorTail, bool kDebug, bool kPair>
+__global__ __launch_bounds__(kBlock) void k_ll_resolve_reset_keys(Common c, LLBuf ll, TailBuf tb, ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  for (uint32_t k = threadIdx.x; k < 256u; k += kBlock) {
+    s_dec[k] = c.tab->dec[k];
+    s_alpha[k] = c.tab->alpha[k];
+  }
+  if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
+    ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
+    ll.counters[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+    FrontKeys<K> a;
+    a.init();
+    float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t p = ll.head[t];
+    if (p != 0u) ll.head[t] = 0u;
+    uint32_t cnt = 0;
+    uint32_t trip = 0;
+    while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
+      const int w = (int)__builtin_amdgcn_readfirstlane(kPair ? 2u * trip + 2u : trip + 1u);
+      ++trip;
+      const uint32_t key0 = (uint32_t)(ll.cd[p] >> 32);
+      const uint32_t nx0 = ll.next[p];
+      uint32_t key1 = 0;
+      uint32_t nx1 = 0;
+      const uint32_t q = p - 1u;
+      if (kPair) {
+        key1 = (uint32_t)(ll.cd[q] >> 32);
+        nx1 = ll.next[q];
+      }
+      ll.next[p] = kUnset;
+      {
+        const uint64_t ev = a.insert_bounded(((uint64_t)key0 << 32) | p, w);
+        if (kTail && cnt >= (uint32_t)K)
+          tailc = over(tailc, decode_premult_lds((uint32_t)ll.cd[(uint32_t)ev], s_dec, s_alpha));
+        ++cnt;
+      }
+      if (kPair && nx0 == q && q != 0u) {
+        ll.next[q] = kUnset;
+        const uint64_t ev = a.insert_bounded(((uint64_t)key1 << 32) | q, w);
+        if (kTail && cnt >= (uint32_t)K)
+          tailc = over(tailc, decode_premult_lds((uint32_t)ll.cd[(uint32_t)ev], s_dec, s_alpha));
+        ++cnt;
+        p = nx1;
+      } else {
+        p = nx0;
+      }
+    }
+    const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
+    const uint32_t nmax = wave_max_u32(n);
+    uint32_t colr[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      colr[k] = 0u;
+      if ((uint32_t)k < nmax && (uint32_t)k < n) colr[k] = (uint32_t)ll.cd[(uint32_t)a.sk[k]];
+    }
+    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult_lds(colr[k], s_dec, s_alpha));
+    col = over(col, tailc);
+    float4 d = c.background;
+    if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+    o.out[t] = over(col, d);
+    if (kDebug) {
+      if (o.fk_key) {
+#pragma unroll
+        for (int k = 0; k < K; ++k)
+          o.fk_key[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)(a.sk[k] >> 32) : kEmpty;
+      }
+      if (o.fk_val) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) o.fk_val[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)a.sk[k] - 1u : kEmpty;
+      }
+      if (o.counts) o.counts[t] = cnt;
+    }
   }
 }
 
@@ -873,7 +987,8 @@ struct Loop64Resolve {
 template <int K>
 struct LLResolveReset {
   static hipError_t run(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, bool pair,
-                        hipStream_t s) {
+                        bool keys_only, hipStream_t s) {
+    if (keys_only) return run_keys(c, ll, tb, o, pair, s);
     const bool dbg = o.fk_key || o.fk_val || o.counts;
     const bool ctail = tb.head != nullptr;
     uint32_t blocks = blocks_for(c.T);
@@ -892,6 +1007,28 @@ struct LLResolveReset {
     }
 #undef OIT_LLR_P
 #undef OIT_LLR
+    return hipGetLastError();
+  }
+  static hipError_t run_keys(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, bool pair,
+                             hipStream_t s) {
+    const bool dbg = o.fk_key || o.fk_val || o.counts;
+    const bool ctail = tb.head != nullptr;
+    uint32_t blocks = blocks_for(c.T);
+    if (blocks > 8192u) blocks = 8192u;
+    if (blocks == 0) blocks = 1;
+    const dim3 g(blocks), b(kBlock);
+#define OIT_LLK(TAIL, CT, DBG, PAIR) \
+  hipLaunchKernelGGL((k_ll_resolve_reset_keys<K, TAIL, CT, DBG, PAIR>), g, b, 0, s, c, ll, tb, o)
+#define OIT_LLK_P(TAIL, CT, DBG) \
+  do { if (pair) OIT_LLK(TAIL, CT, DBG, true); else OIT_LLK(TAIL, CT, DBG, false); } while (0)
+    if (c.tail) {
+      if (ctail) { if (dbg) OIT_LLK_P(true, true, true); else OIT_LLK_P(true, true, false); }
+      else { if (dbg) OIT_LLK_P(true, false, true); else OIT_LLK_P(true, false, false); }
+    } else {
+      if (dbg) OIT_LLK_P(false, false, true); else OIT_LLK_P(false, false, false);
+    }
+#undef OIT_LLK_P
+#undef OIT_LLK
     return hipGetLastError();
   }
 };
@@ -942,8 +1079,8 @@ hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll,
 }
 
 hipError_t launch_ll_resolve_reset(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o,
-                                   bool pair_walk, hipStream_t s) {
-  return dispatch_k<LLResolveReset>(c.K, c, ll, tb, o, pair_walk, s);
+                                   bool pair_walk, bool keys_only, hipStream_t s) {
+  return dispatch_k<LLResolveReset>(c.K, c, ll, tb, o, pair_walk, keys_only, s);
 }
 
 hipError_t launch_simple_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s) {

@@ -80,7 +80,7 @@ hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll,
 // moves the frame counters aside; pair_walk consumes node p-1 in the same
 // round trip as p when the list continues there.
 hipError_t launch_ll_resolve_reset(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o,
-                                   bool pair_walk, hipStream_t s);
+                                   bool pair_walk, bool keys_only, hipStream_t s);
 
 hipError_t launch_simple_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s);
 hipError_t launch_spin_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s);
