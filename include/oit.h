@@ -52,7 +52,7 @@ enum oit_algorithm {
 #define OIT_VARIANT_LL_SPLIT_APPEND  0x1u /* Linked List: separate store and link kernels */
 #define OIT_VARIANT_LL_SINGLE_WALK   0x2u /* Linked List: resolve walks one node per round trip */
 #define OIT_VARIANT_LL_APPEND_8      0x4u /* Linked List: fused append takes 8 chunks per wave (default 4) */
-#define OIT_VARIANT_LL_COLOR_IN_REGS 0x8u /* Linked List: resolve carries colours in registers (default: reload by node) */
+#define OIT_VARIANT_LL_KEYS_ONLY     0x8u /* Linked List: resolve keeps sort keys only and reloads colours by node */
 
 /* Status codes. */
 enum oit_status {

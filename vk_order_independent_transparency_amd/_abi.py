@@ -43,7 +43,7 @@ OIT_ERR_NOT_READY = -8
 VARIANT_LL_SPLIT_APPEND = 0x1
 VARIANT_LL_SINGLE_WALK = 0x2
 VARIANT_LL_APPEND_8 = 0x4
-VARIANT_LL_COLOR_IN_REGS = 0x8
+VARIANT_LL_KEYS_ONLY = 0x8
 
 OIT_PROFILE_STAGES = 6
 PROFILE_STAGE_NAMES = ("clear", "store", "link", "classify", "tail_link", "resolve")

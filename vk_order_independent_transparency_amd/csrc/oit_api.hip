@@ -639,7 +639,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     case OIT_LINKEDLIST: {
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
       const bool pair = !(c->cfg.variant & OIT_VARIANT_LL_SINGLE_WALK);
-      const bool keys = !(c->cfg.variant & OIT_VARIANT_LL_COLOR_IN_REGS);
+      const bool keys = (c->cfg.variant & OIT_VARIANT_LL_KEYS_ONLY) != 0;
       e = staged(c, kStResolve, s, [&] { return oit::launch_ll_resolve_reset(k, ll, tbr, o, pair, keys, s); });
       if (e == hipSuccess) {
         c->ll_clean = true;
