@@ -214,7 +214,7 @@ int oit_event_elapsed_ms(float* out, void* start, void* stop);
 typedef struct oit_scene_params {
   uint64_t seed;
   uint32_t width, height;
-  uint32_t samples;           /* 1, or 4 for the standard 4x sample positions */
+  uint32_t samples;           /* 1, 2, 4 or 8 (the standard sample positions) */
   uint32_t spheres;
   float    radius_min, radius_max;
   float    alpha_min, alpha_max;
@@ -228,6 +228,21 @@ int oit_scene_count(const oit_scene_params* p, uint64_t* out_count);
  * fragments (target[], depth[], rgba[4*]). */
 int oit_scene_generate_host(const oit_scene_params* p, uint32_t* target, float* depth,
                             float* rgba, uint64_t capacity, uint64_t* out_count);
+
+/* Device generator of the same stream (bit-identical to
+ * oit_scene_generate_host, same order). Replaces the reference's scene
+ * set-up and rasterisation of the transparent spheres as the producer of
+ * fragments (main.cpp:334-417, object.vert.glsl:32-39,
+ * shaderCommon.glsl:48-56). Create builds the sphere table on the host and
+ * counts the fragments on `device` (synchronously); emit writes the stream
+ * into DEVICE arrays (target 8-byte aligned, depth 8-byte aligned, rgba
+ * 16-byte aligned) on `stream`, and may be called any number of times. */
+typedef struct oit_scene_gen oit_scene_gen;
+int oit_scene_gen_create(oit_scene_gen** out, const oit_scene_params* p, int device);
+int oit_scene_gen_destroy(oit_scene_gen* g);
+int oit_scene_gen_count(const oit_scene_gen* g, uint64_t* out_count);
+int oit_scene_gen_emit(oit_scene_gen* g, uint32_t* target, float* depth, float* rgba,
+                       uint64_t capacity, void* stream);
 
 #ifdef __cplusplus
 }

@@ -137,3 +137,21 @@ def test_library_codec_tables_equal_the_oracle(oracle):
     assert np.array_equal(thr[1:256].view(np.uint32), othr[1:256].view(np.uint32))
     assert np.array_equal(dec.view(np.uint32), odec.view(np.uint32))
     assert np.array_equal(alpha.view(np.uint32), oalpha.view(np.uint32))
+
+
+def test_device_scene_generator_argument_checks():
+    lib = _abi.lib()
+    from vk_order_independent_transparency_amd import device_count, scene
+
+    g = ctypes.c_void_p()
+    assert lib.oit_scene_gen_create(None, None, 0) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_scene_gen_create(ctypes.byref(g), None, 0) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_scene_gen_destroy(None) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_scene_gen_emit(None, None, None, None, 0, None) == _abi.OIT_ERR_INVALID_ARG
+    bad = scene.SceneParams(seed=1, width=8, height=8, spheres=1, radius=(0.1, 0.2), alpha=(0.1, 0.2),
+                            samples=3).to_c()
+    assert lib.oit_scene_gen_create(ctypes.byref(g), ctypes.byref(bad), 0) == _abi.OIT_ERR_INVALID_ARG
+    if device_count() == 0:
+        ok = scene.SMALL_256.to_c()
+        assert lib.oit_scene_gen_create(ctypes.byref(g), ctypes.byref(ok), 0) == _abi.OIT_ERR_DEVICE
+        assert not g.value

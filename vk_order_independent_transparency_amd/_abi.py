@@ -159,6 +159,10 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(OitSceneParams), _P, _P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)],
     ),
+    "oit_scene_gen_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(OitSceneParams), ctypes.c_int]),
+    "oit_scene_gen_destroy": (ctypes.c_int, [_P]),
+    "oit_scene_gen_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),
+    "oit_scene_gen_emit": (ctypes.c_int, [_P, _P, _P, _P, ctypes.c_uint64, _P]),
 }
 
 _lib = None

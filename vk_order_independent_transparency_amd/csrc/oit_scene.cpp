@@ -21,6 +21,7 @@
 #include <cstring>
 
 #include "oit.h"
+#include "oit_scene.hpp"
 
 namespace {
 
@@ -52,28 +53,41 @@ const float (*sample_positions(uint32_t S))[2] {
   }
 }
 
-bool valid(const oit_scene_params* p) {
+}  // namespace
+
+namespace oit {
+
+bool scene_params_valid(const oit_scene_params* p) {
   if (!p || p->width == 0 || p->height == 0) return false;
   if (!sample_positions(p->samples)) return false;
   if ((uint64_t)p->width * p->height * p->samples >= 0xFFFFFFFFull) return false;
-  if (!(p->radius_min >= 0.f) || !(p->radius_max >= p->radius_min)) return false;
+  if (!(p->radius_min >= 0.f) || !(p->radius_max >= p->radius_min) || !(p->radius_max <= 1e6f)) return false;
   if (!(p->alpha_min >= 0.f) || !(p->alpha_max >= p->alpha_min) || p->alpha_max > 1.f) return false;
   if (!(p->cluster_fraction >= 0.f) || p->cluster_fraction > 1.f) return false;
   if (!(p->cluster_extent >= 0.f) || p->cluster_extent > 1.f) return false;
   return true;
 }
 
-// Walks the scene; calls emit(target, depth, rgba) in draw order.
-template <typename Emit>
-void walk(const oit_scene_params* p, Emit&& emit) {
+ScenePositions scene_positions(uint32_t samples) {
+  ScenePositions sp{};
+  sp.samples = samples;
+  const float(*pos)[2] = sample_positions(samples);
+  for (uint32_t s = 0; pos && s < samples; ++s) {
+    sp.xy[s][0] = pos[s][0];
+    sp.xy[s][1] = pos[s][1];
+  }
+  return sp;
+}
+
+void build_spheres(const oit_scene_params* p, std::vector<SphereDesc>& out) {
+  out.clear();
+  out.reserve(p->spheres);
   SplitMix64 rng{p->seed};
   const float ext = p->cluster_extent;
   const float ox = rng.range(0.0f, 1.0f - ext);
   const float oy = rng.range(0.0f, 1.0f - ext);
   const uint32_t ncl = (uint32_t)std::floor((double)p->spheres * (double)p->cluster_fraction);
-  const uint32_t W = p->width, H = p->height, S = p->samples;
-  const float(*pos)[2] = sample_positions(S);
-  const float fW = (float)W, fH = (float)H;
+  const uint32_t W = p->width, H = p->height;
   for (uint32_t i = 0; i < p->spheres; ++i) {
     float cx = rng.uniform();
     float cy = rng.uniform();
@@ -87,28 +101,56 @@ void walk(const oit_scene_params* p, Emit&& emit) {
       cx = ox + cx * ext;
       cy = oy + cy * ext;
     }
-    const float r2 = r * r;
-    const float rgba[4] = {R * a, G * a, B * a, a};
-    // conservative pixel bounds, exact coverage test below
+    SphereDesc d;
+    d.cx = cx;
+    d.cy = cy;
+    d.cz = cz;
+    d.r2 = r * r;
+    d.rgba[0] = R * a;
+    d.rgba[1] = G * a;
+    d.rgba[2] = B * a;
+    d.rgba[3] = a;
+    // conservative pixel bounds, exact coverage test in the walkers
     long x0 = (long)std::floor(((double)cx - r) * W) - 1, x1 = (long)std::ceil(((double)cx + r) * W) + 1;
     long y0 = (long)std::floor(((double)cy - r) * H) - 1, y1 = (long)std::ceil(((double)cy + r) * H) + 1;
     if (x0 < 0) x0 = 0;
     if (y0 < 0) y0 = 0;
     if (x1 > (long)W - 1) x1 = (long)W - 1;
     if (y1 > (long)H - 1) y1 = (long)H - 1;
-    for (long y = y0; y <= y1; ++y) {
-      for (long x = x0; x <= x1; ++x) {
+    d.x0 = (int32_t)x0;
+    d.x1 = (int32_t)x1;
+    d.y0 = (int32_t)y0;
+    d.y1 = (int32_t)y1;
+    out.push_back(d);
+  }
+}
+
+}  // namespace oit
+
+namespace {
+
+// Walks the scene; calls emit(target, depth, rgba) in draw order.
+template <typename Emit>
+void walk(const oit_scene_params* p, Emit&& emit) {
+  std::vector<oit::SphereDesc> spheres;
+  oit::build_spheres(p, spheres);
+  const uint32_t W = p->width, H = p->height, S = p->samples;
+  const float(*pos)[2] = sample_positions(S);
+  const float fW = (float)W, fH = (float)H;
+  for (const oit::SphereDesc& d : spheres) {
+    for (long y = d.y0; y <= d.y1; ++y) {
+      for (long x = d.x0; x <= d.x1; ++x) {
         for (uint32_t s = 0; s < S; ++s) {
           const float px = ((float)x + pos[s][0]) / fW;
           const float py = ((float)y + pos[s][1]) / fH;
-          const float dx = px - cx;
-          const float dy = py - cy;
+          const float dx = px - d.cx;
+          const float dy = py - d.cy;
           const float d2 = dx * dx + dy * dy;
-          if (!(d2 < r2)) continue;
-          const float h = std::sqrt(r2 - d2);
+          if (!(d2 < d.r2)) continue;
+          const float h = std::sqrt(d.r2 - d2);
           const uint32_t t = s * W * H + (uint32_t)y * W + (uint32_t)x;
-          emit(t, cz - h, rgba);
-          emit(t, cz + h, rgba);
+          emit(t, d.cz - h, d.rgba);
+          emit(t, d.cz + h, d.rgba);
         }
       }
     }
@@ -120,7 +162,7 @@ void walk(const oit_scene_params* p, Emit&& emit) {
 extern "C" {
 
 int oit_scene_count(const oit_scene_params* p, uint64_t* out_count) {
-  if (!valid(p) || !out_count) return OIT_ERR_INVALID_ARG;
+  if (!oit::scene_params_valid(p) || !out_count) return OIT_ERR_INVALID_ARG;
   uint64_t n = 0;
   walk(p, [&](uint32_t, float, const float*) { ++n; });
   *out_count = n;
@@ -129,7 +171,7 @@ int oit_scene_count(const oit_scene_params* p, uint64_t* out_count) {
 
 int oit_scene_generate_host(const oit_scene_params* p, uint32_t* target, float* depth, float* rgba, uint64_t capacity,
                             uint64_t* out_count) {
-  if (!valid(p) || !target || !depth || !rgba || !out_count) return OIT_ERR_INVALID_ARG;
+  if (!oit::scene_params_valid(p) || !target || !depth || !rgba || !out_count) return OIT_ERR_INVALID_ARG;
   uint64_t n = 0;
   bool over = false;
   walk(p, [&](uint32_t t, float d, const float* c) {

@@ -1,8 +1,10 @@
 """Synthetic sphere fragment streams (stand-in for the reference's rasterised
 sphere mesh, main.cpp:334-417) and the named workloads.
 
-The generator itself is native (csrc/oit_scene.cpp, exported through the C
-ABI); this module only describes workloads and copies results into numpy.
+The generator itself is native: csrc/oit_scene.cpp on the host and
+csrc/oit_scene_gpu.hip on the device (the same stream, bit for bit), both
+exported through the C ABI; this module describes workloads and moves
+results into numpy or device buffers.
 """
 from __future__ import annotations
 
@@ -87,3 +89,49 @@ def generate(p: SceneParams) -> tuple[np.ndarray, np.ndarray, np.ndarray]:
     )
     assert got.value == n
     return target[:n], depth[:n], rgba[:n]
+
+
+class DeviceScene:
+    """The scene's fragment stream generated on the GPU (oit_scene_gen_*).
+
+    ``fragments`` is known after construction (the device counts once);
+    ``emit`` writes the stream into device buffers on a stream and can be
+    repeated. ``buffers()`` allocates a matching (target, depth, rgba) set."""
+
+    def __init__(self, p: SceneParams, device: int = 0):
+        self.params = p
+        self.handle = ctypes.c_void_p(0)
+        cp = p.to_c()
+        _abi.check("oit_scene_gen_create", _abi.lib().oit_scene_gen_create(ctypes.byref(self.handle),
+                                                                          ctypes.byref(cp), device))
+        n = ctypes.c_uint64(0)
+        _abi.check("oit_scene_gen_count", _abi.lib().oit_scene_gen_count(self.handle, ctypes.byref(n)))
+        self.fragments = int(n.value)
+
+    def buffers(self):
+        from .device import DeviceBuffer
+
+        n = max(self.fragments, 1)
+        return DeviceBuffer((n,), np.uint32), DeviceBuffer((n,), np.float32), DeviceBuffer((n, 4), np.float32)
+
+    def emit(self, target, depth, rgba, stream=None, capacity: int | None = None) -> None:
+        from .device import stream_ptr
+
+        cap = self.fragments if capacity is None else int(capacity)
+        _abi.check("oit_scene_gen_emit", _abi.lib().oit_scene_gen_emit(
+            self.handle, _ptr(target), _ptr(depth), _ptr(rgba), cap, stream_ptr(stream)))
+
+    def close(self) -> None:
+        if self.handle:
+            _abi.lib().oit_scene_gen_destroy(self.handle)
+            self.handle = ctypes.c_void_p(0)
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _ptr(b) -> int:
+    return b if isinstance(b, int) else b.ptr
