@@ -208,6 +208,66 @@ int oit_event_record(void* event, void* stream);
 int oit_event_synchronize(void* event);
 int oit_event_elapsed_ms(float* out, void* start, void* stop);
 
+/* ---- Batched Linked List frames ------------------------------------------
+ * No counterpart in the reference, whose onRender draws one frame per command
+ * buffer (oitRender.cpp:28-154); this is the batched form of the Linked List
+ * pair clearTransparent/drawTransparent<LinkedList> (oitRender.cpp:201-227)
+ * for callers with many frames in flight. Frames of any size, depth
+ * complexity and K share one resident node pool sized from their exact
+ * fragment counts; one append launch covers every frame and one resolve
+ * launch covers each (K, tail_blend) group. Each frame's image equals the
+ * single-frame Linked List result with a pool of at least count + 1 nodes,
+ * bit for bit. */
+typedef struct oit_batch_frame {
+  uint32_t width, height, samples;
+  uint32_t layers;            /* K: 1, 2, 4, 8, 16 or 32 */
+  uint32_t tail_blend;        /* OIT_TAILBLEND */
+  uint32_t reserved;          /* 0 */
+  float    background[4];     /* premultiplied RGBA under the composite */
+  oit_fragments frags;        /* device SoA in draw order, frame-local targets; rgba 16-byte aligned */
+  float*   out;               /* device [width*height*samples][4], 16-byte aligned */
+} oit_batch_frame;
+
+enum {
+  OIT_BATCH_CONCURRENT = 0x1, /* resolve groups run on forked streams (join before return) */
+  OIT_BATCH_GRAPH      = 0x2  /* a run is captured once per frame set and replayed as a hipGraph */
+};
+
+/* Kernel choices of a batch: append chunks per wave (4 or 8) and, per
+ * (K, tail_blend) group, the resolve variant bits (OIT_VARIANT_LL_SINGLE_WALK,
+ * OIT_VARIANT_LL_KEYS_ONLY) and a cap on resolve blocks (0 = one block per
+ * tile). Index [k][tail] with k = log2(K). */
+typedef struct oit_batch_tuning {
+  uint32_t append_chunks;
+  uint32_t resolve_variant[6][2];
+  uint32_t resolve_blocks[6][2];
+} oit_batch_tuning;
+
+typedef struct oit_batch_sizes {
+  uint64_t pool_nodes;        /* nodes the frames need, terminator included */
+  uint64_t head_entries;      /* padded targets */
+  uint64_t bytes;             /* device bytes for those (nodes 12 B, heads 4 B) */
+  uint32_t groups;            /* resolve launches */
+  uint32_t frames;
+} oit_batch_sizes;
+
+typedef struct oit_batch oit_batch;
+int oit_batch_create(oit_batch** out, int device, uint32_t flags);
+int oit_batch_destroy(oit_batch* b);
+/* Pool and head sizes `frames` need (no device work). */
+int oit_batch_query(const oit_batch* b, const oit_batch_frame* frames, uint32_t n, oit_batch_sizes* out);
+/* Grows the resident pool to at least these sizes (never shrinks). */
+int oit_batch_reserve(oit_batch* b, uint64_t pool_nodes, uint64_t head_entries);
+/* Appends and resolves every frame on `stream`. Grows the pool when needed
+ * (the only allocation). Fragment and output buffers must stay valid until
+ * the stream reaches the end of the run. */
+int oit_batch_run(oit_batch* b, const oit_batch_frame* frames, uint32_t n, void* stream);
+/* Times the kernel variants on these frames (synchronous, runs the batch
+ * several times) and keeps the fastest per group for later runs. */
+int oit_batch_tune(oit_batch* b, const oit_batch_frame* frames, uint32_t n, void* stream);
+int oit_batch_get_tuning(const oit_batch* b, oit_batch_tuning* out);
+int oit_batch_set_tuning(oit_batch* b, const oit_batch_tuning* t);
+
 /* ---- Synthetic scene (stands in for the reference's rasterised sphere
  * mesh, main.cpp:334-417): analytic spheres, orthographic, each covered
  * pixel (or sample) centre emits a front and a back fragment. ---------- */

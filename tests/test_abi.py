@@ -37,6 +37,9 @@ STRUCTS = {
     "oit_stats": _abi.OitStats,
     "oit_profile": _abi.OitProfile,
     "oit_scene_params": _abi.OitSceneParams,
+    "oit_batch_frame": _abi.OitBatchFrame,
+    "oit_batch_tuning": _abi.OitBatchTuning,
+    "oit_batch_sizes": _abi.OitBatchSizes,
 }
 
 

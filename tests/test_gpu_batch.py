@@ -1,0 +1,163 @@
+"""Batched Linked List (oit_batch_*) against the single-frame Linked List:
+every frame of a mixed batch (sizes, K, tail flag, sample planes, empty and
+deep clustered frames) must give the single-frame image bit for bit, in
+every launch mode (one stream, forked groups, captured graph), on repeated
+runs (the pool is reset by the resolve), after growth and after tuning."""
+import numpy as np
+import pytest
+
+from vk_order_independent_transparency_amd import (OIT_LINKEDLIST, Batch, BatchFrame, DeviceBuffer, FrameConfig,
+                                                   Renderer, Stream, _abi, scene)
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(*a):
+    print(*a, flush=True)
+
+
+def _scene(seed, w, h, spheres, samples=1, cluster=0.0, radius=(0.03, 0.15)):
+    return scene.SceneParams(seed=seed, width=w, height=h, spheres=spheres, radius=radius, alpha=(0.1, 0.6),
+                             samples=samples, cluster_fraction=cluster, cluster_extent=0.1)
+
+
+MIXED = [  # (scene, K, tail)
+    (_scene(1, 40, 30, 60), 4, True),
+    (_scene(2, 64, 64, 120), 16, True),
+    (_scene(3, 17, 9, 20), 1, True),
+    (_scene(4, 100, 50, 200, cluster=0.3), 32, True),
+    (_scene(5, 33, 33, 0), 8, True),  # no fragments
+    (_scene(6, 48, 40, 90, samples=4), 8, False),
+    (_scene(7, 64, 48, 150), 16, False),
+    (_scene(8, 128, 96, 400, cluster=0.4, radius=(0.02, 0.08)), 2, True),
+    (_scene(9, 70, 70, 250, cluster=0.5), 32, False),
+]
+
+
+class _Frames:
+    """Device streams (from the device generator) and output buffers."""
+
+    def __init__(self, specs, stream):
+        self.specs = specs
+        self.streams = []
+        self.outs = []
+        self.frames = []
+        for p, K, tail in specs:
+            g = scene.DeviceScene(p)
+            t, d, c = g.buffers()
+            g.emit(t, d, c, stream)
+            out = DeviceBuffer((p.targets, 4), "float32")
+            self.streams.append((t, d, c, g.fragments))
+            self.outs.append(out)
+            self.frames.append(BatchFrame(width=p.width, height=p.height, samples=p.samples, layers=K, tail_blend=tail,
+                                          target=t, depth=d, rgba=c, count=g.fragments, out=out))
+            g.close()
+        stream.synchronize()
+
+    def images(self, stream):
+        stream.synchronize()
+        return [o.numpy() for o in self.outs]
+
+
+def _single_frame_images(fr: _Frames, stream):
+    imgs = []
+    with Renderer(0) as r:
+        for (p, K, tail), (t, d, c, F) in zip(fr.specs, fr.streams):
+            r.configure(FrameConfig(algorithm=OIT_LINKEDLIST, layers=K, tail_blend=tail, width=p.width,
+                                    height=p.height, samples=p.samples, ll_pool_nodes=F + 1))
+            out = DeviceBuffer((p.targets, 4), "float32")
+            r.render(t, d, c, out, stream, count=F)
+            stream.synchronize()
+            imgs.append(out.numpy())
+    return imgs
+
+
+def _assert_images(got, want):
+    for i, (g, w) in enumerate(zip(got, want)):
+        if not np.array_equal(g.view(np.uint32), w.view(np.uint32)):
+            bad = np.argwhere(g.view(np.uint32) != w.view(np.uint32))
+            raise AssertionError(f"frame {i}: {len(bad)} words differ, first {bad[:2].tolist()}")
+
+
+@pytest.fixture(scope="module")
+def mixed():
+    st = Stream()
+    fr = _Frames(MIXED, st)
+    want = _single_frame_images(fr, st)
+    return st, fr, want
+
+
+@pytest.mark.parametrize("concurrent,graph", [(False, False), (True, False), (False, True), (True, True)])
+def test_batch_equals_single_frame(mixed, concurrent, graph):
+    st, fr, want = mixed
+    _say("batch", concurrent, graph, [f.count for f in fr.frames])
+    with Batch(0, concurrent=concurrent, graph=graph) as b:
+        sz = b.query(fr.frames)
+        assert sz.frames == len(MIXED) and sz.groups == 9
+        for rep in range(3):  # later runs reuse the pool the resolve left clean
+            for o in fr.outs:
+                o.upload(np.full(o.shape, np.nan, np.float32))
+            b.run(fr.frames, st)
+            _assert_images(fr.images(st), want)
+
+
+def test_batch_matches_oracle_on_one_frame(mixed, oracle):
+    st, fr, want = mixed
+    p, K, tail = MIXED[1]
+    tg, d, c = scene.generate(p)
+    cfg = oracle.make_config(oracle.LINKEDLIST, K, 1 if tail else 0, p.width, p.height, samples=p.samples,
+                             ll_capacity=len(tg) + 1)
+    ref = oracle.run(cfg, tg, d, c)
+    assert np.array_equal(want[1].view(np.uint32), ref["out"].view(np.uint32))
+
+
+def test_batch_grows_and_shrinks_frame_sets(mixed):
+    st, fr, want = mixed
+    with Batch(0) as b:
+        small = [fr.frames[0], fr.frames[2]]
+        b.run(small, st)
+        _assert_images([fr.outs[0].numpy(), fr.outs[2].numpy()], [want[0], want[2]])
+        b.run(fr.frames, st)  # grows the pool and head array
+        _assert_images(fr.images(st), want)
+        b.run(small[::-1], st)
+        _assert_images([fr.outs[0].numpy(), fr.outs[2].numpy()], [want[0], want[2]])
+
+
+def test_batch_tuning_keeps_results(mixed):
+    st, fr, want = mixed
+    with Batch(0, concurrent=True) as b:
+        b.tune(fr.frames, st)
+        t = b.tuning
+        assert t.append_chunks in (4, 8)
+        _say("tuned", t.append_chunks, [list(r) for r in t.resolve_variant], [list(r) for r in t.resolve_blocks])
+        b.run(fr.frames, st)
+        _assert_images(fr.images(st), want)
+        for v in (0, _abi.VARIANT_LL_SINGLE_WALK, _abi.VARIANT_LL_KEYS_ONLY,
+                  _abi.VARIANT_LL_SINGLE_WALK | _abi.VARIANT_LL_KEYS_ONLY):
+            for blocks in (0, 64):
+                t = b.tuning
+                t.append_chunks = 8 if v & _abi.VARIANT_LL_SINGLE_WALK else 4
+                for k in range(6):
+                    for tl in range(2):
+                        t.resolve_variant[k][tl] = v
+                        t.resolve_blocks[k][tl] = blocks
+                b.tuning = t
+                b.run(fr.frames, st)
+                _assert_images(fr.images(st), want)
+        bad = b.tuning
+        bad.append_chunks = 5
+        with pytest.raises(_abi.OitError):
+            b.tuning = bad
+
+
+def test_batch_rejects_bad_frames(mixed):
+    st, fr, _ = mixed
+    import dataclasses
+
+    with Batch(0) as b:
+        for change in ({"layers": 3}, {"width": 0}, {"samples": 3}, {"out": 0}):
+            f = dataclasses.replace(fr.frames[0], **change)
+            with pytest.raises(_abi.OitError) as e:
+                b.run([f], st)
+            assert e.value.status == _abi.OIT_ERR_INVALID_ARG
+        b.run([], st)

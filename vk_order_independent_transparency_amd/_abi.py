@@ -45,6 +45,9 @@ VARIANT_LL_SINGLE_WALK = 0x2
 VARIANT_LL_APPEND_8 = 0x4
 VARIANT_LL_KEYS_ONLY = 0x8
 
+BATCH_CONCURRENT = 0x1
+BATCH_GRAPH = 0x2
+
 OIT_PROFILE_STAGES = 6
 PROFILE_STAGE_NAMES = ("clear", "store", "link", "classify", "tail_link", "resolve")
 
@@ -104,6 +107,38 @@ class OitProfile(ctypes.Structure):
     ]
 
 
+class OitBatchFrame(ctypes.Structure):
+    _fields_ = [
+        ("width", ctypes.c_uint32),
+        ("height", ctypes.c_uint32),
+        ("samples", ctypes.c_uint32),
+        ("layers", ctypes.c_uint32),
+        ("tail_blend", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("background", ctypes.c_float * 4),
+        ("frags", OitFragments),
+        ("out", ctypes.c_void_p),
+    ]
+
+
+class OitBatchTuning(ctypes.Structure):
+    _fields_ = [
+        ("append_chunks", ctypes.c_uint32),
+        ("resolve_variant", (ctypes.c_uint32 * 2) * 6),
+        ("resolve_blocks", (ctypes.c_uint32 * 2) * 6),
+    ]
+
+
+class OitBatchSizes(ctypes.Structure):
+    _fields_ = [
+        ("pool_nodes", ctypes.c_uint64),
+        ("head_entries", ctypes.c_uint64),
+        ("bytes", ctypes.c_uint64),
+        ("groups", ctypes.c_uint32),
+        ("frames", ctypes.c_uint32),
+    ]
+
+
 class OitSceneParams(ctypes.Structure):
     _fields_ = [
         ("seed", ctypes.c_uint64),
@@ -159,6 +194,15 @@ SIGNATURES = {
         ctypes.c_int,
         [ctypes.POINTER(OitSceneParams), _P, _P, _P, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint64)],
     ),
+    "oit_batch_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.c_uint32]),
+    "oit_batch_destroy": (ctypes.c_int, [_P]),
+    "oit_batch_query": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchFrame), ctypes.c_uint32,
+                                       ctypes.POINTER(OitBatchSizes)]),
+    "oit_batch_reserve": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_uint64]),
+    "oit_batch_run": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchFrame), ctypes.c_uint32, _P]),
+    "oit_batch_tune": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchFrame), ctypes.c_uint32, _P]),
+    "oit_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchTuning)]),
+    "oit_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchTuning)]),
     "oit_scene_gen_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(OitSceneParams), ctypes.c_int]),
     "oit_scene_gen_destroy": (ctypes.c_int, [_P]),
     "oit_scene_gen_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),

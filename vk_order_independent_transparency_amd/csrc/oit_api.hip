@@ -20,16 +20,6 @@ using oit::kEmpty;
 
 namespace {
 
-constexpr uint32_t kMaxFrameFragments = 0x7FFFFFFEu;  // ids and tail links fit in 31 bits
-constexpr uint64_t kMaxPoolNodes = 0xFFFFFFFEull;      // node indices stay below the kUnset sentinel
-
-enum Stage { kStClear = 0, kStStore, kStLink, kStClassify, kStTailLink, kStResolve, kStCount };
-
-struct EventPair {
-  hipEvent_t a, b;
-  int stage;
-};
-
 // --- sRGB8 codec tables (host) ---------------------------------------------
 // Encode curve of unPremultLinearToSRGB (shaderCommon.glsl:60-69) and the
 // decode curve of unPremultSRGBToLinear (shaderCommon.glsl:84-94), evaluated
@@ -49,7 +39,11 @@ float from_bits(uint32_t b) {
   return f;
 }
 
-void build_codec(oit::CodecTables* t) {
+}  // namespace
+
+namespace oit {
+
+void build_codec_tables(CodecTables* t) {
   t->thr[0] = -INFINITY;
   t->thr[256] = INFINITY;
   for (int q = 1; q <= 255; ++q) {
@@ -67,6 +61,20 @@ void build_codec(oit::CodecTables* t) {
     t->alpha[u] = (float)u / 255.0f;
   }
 }
+
+}  // namespace oit
+
+namespace {
+
+constexpr uint32_t kMaxFrameFragments = 0x7FFFFFFEu;  // ids and tail links fit in 31 bits
+constexpr uint64_t kMaxPoolNodes = 0xFFFFFFFEull;      // node indices stay below the kUnset sentinel
+
+enum Stage { kStClear = 0, kStStore, kStLink, kStClassify, kStTailLink, kStResolve, kStCount };
+
+struct EventPair {
+  hipEvent_t a, b;
+  int stage;
+};
 
 bool valid_layers(uint32_t k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16 || k == 32; }
 
@@ -318,7 +326,7 @@ int oit_create(oit_ctx** out, int device) {
   c->device = device;
   DeviceGuard g(device);
   oit::CodecTables host;
-  build_codec(&host);
+  oit::build_codec_tables(&host);
   int rc = dev_alloc(&c->d_tab, 1);
   if (!rc) rc = dev_alloc(&c->d_counters, 32);
   if (!rc && hipMemcpy(c->d_tab, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess) rc = OIT_ERR_DEVICE;
@@ -690,7 +698,7 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
 int oit_codec_tables(float* thr257, float* dec256, float* alpha256) {
   if (!thr257 || !dec256 || !alpha256) return OIT_ERR_INVALID_ARG;
   oit::CodecTables t;
-  build_codec(&t);
+  oit::build_codec_tables(&t);
   std::memcpy(thr257, t.thr, sizeof t.thr);
   std::memcpy(dec256, t.dec, sizeof t.dec);
   std::memcpy(alpha256, t.alpha, sizeof t.alpha);

@@ -319,22 +319,35 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
 // link words of the frame's nodes already hold kUnset (left by the previous
 // frame's resetting resolve, or by the clear), so a walker never follows a
 // stale link; the run chain n -> n-1 and the inserts are relaxed atomics.
+// One wave's share of the fused append: fragments first .. first + 64 * kU - 1
+// of a source stream whose fragment i is node node0 + i and targets
+// tbase + target[i] of the pool's head array.
+struct AppendSrc {
+  const uint32_t* target;
+  const float* depth;
+  const float4* rgba;
+  uint64_t count;  // fragments in the source
+  uint64_t first;  // the wave's first fragment
+  uint32_t node0;  // node of fragment 0 (its id + 1)
+  uint32_t T;      // targets of the frame
+  uint32_t tbase;  // head index of the frame's target 0
+  uint32_t lid0;   // frame-local id of fragment 0 (colour-pass tail records)
+};
+
 template <bool kTail, int kU>
-__global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBuf ll, TailBuf tb) {
-  __shared__ float s_thr[257];
-  stage_thresholds(s_thr, c.tab);
+__device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* s_thr, const LLBuf& ll,
+                                               const TailBuf& tb) {
   const uint32_t lane = lane_id();
-  const uint64_t wave_base = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
   uint32_t t[kU];
   bool node[kU], ovf[kU];
   float4 col[kU];
   float dep[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    const uint64_t i = wave_base + (uint64_t)u * 64u + lane;
+    const uint64_t i = f.first + (uint64_t)u * 64u + lane;
     t[u] = i < f.count ? f.target[i] : kEmpty;
-    if (t[u] >= c.T) t[u] = kEmpty;
-    const uint64_t n64 = (uint64_t)f.id_base + i + 1u;
+    if (t[u] >= f.T) t[u] = kEmpty;
+    const uint64_t n64 = (uint64_t)f.node0 + i;
     node[u] = t[u] != kEmpty && n64 < ll.cap;
     ovf[u] = t[u] != kEmpty && !node[u];
     col[u] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -346,7 +359,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBu
   bool ins[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    const uint32_t n = (uint32_t)(f.id_base + wave_base + (uint64_t)u * 64u + lane + 1u);
+    const uint32_t n = (uint32_t)(f.node0 + f.first + (uint64_t)u * 64u + lane);
     if (node[u]) ll.cd[n] = ((uint64_t)depth_key(dep[u]) << 32) | encode_color_lds(col[u], s_thr);
     const Run r = same_target_run(node[u] ? t[u] : kEmpty);
     if (node[u] && !r.start) st_relaxed(&ll.next[n], n - 1u);
@@ -359,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBu
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     old[u] = 0;
-    if (ins[u]) old[u] = max_relaxed(&ll.head[t[u]], hi[u]);
+    if (ins[u]) old[u] = max_relaxed(&ll.head[f.tbase + t[u]], hi[u]);
   }
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
@@ -379,9 +392,146 @@ __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBu
     const uint64_t om = __ballot(ovf[u]);
     if (om && lane == (uint32_t)__builtin_ctzll(om)) atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
     if constexpr (kTail) {
-      const uint32_t lid = (uint32_t)(f.id_base + wave_base + (uint64_t)u * 64u + lane);
+      const uint32_t lid = (uint32_t)(f.lid0 + f.first + (uint64_t)u * 64u + lane);
       tail_record(ovf[u], t[u], lid, 0u, col[u], tb);
     }
+  }
+}
+
+template <bool kTail, int kU>
+__global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBuf ll, TailBuf tb) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, c.tab);
+  AppendSrc a;
+  a.target = f.target;
+  a.depth = f.depth;
+  a.rgba = f.rgba;
+  a.count = f.count;
+  a.first = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
+  a.node0 = f.id_base + 1u;
+  a.T = c.T;
+  a.tbase = 0;
+  a.lid0 = f.id_base;
+  ll_append_wave<kTail, kU>(a, s_thr, ll, tb);
+}
+
+// Walk of target t's list (t indexes the pool's head array): front K in `a`,
+// list length in `cnt`, and the composite of the front K over the evicted
+// tail (before the colour buffer). Resets head[t] and every walked link.
+// Reached by every lane of the wave (wave_max_u32).
+template <int K, bool kTail, bool kPair>
+__device__ __forceinline__ float4 ll_walk_reset(uint32_t t, const LLBuf& ll, const float* s_dec,
+                                                const float* s_alpha, FrontK<K>& a, uint32_t& cnt) {
+  a.init();
+  float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t p = ll.head[t];
+  if (p != 0u) ll.head[t] = 0u;
+  cnt = 0;
+  // trips of this walk so far; every active lane has the same value, so
+  // readfirstlane makes it a scalar bound on the occupied slots (a trip
+  // consumes at most two nodes)
+  uint32_t trip = 0;
+  while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
+    const int w = (int)__builtin_amdgcn_readfirstlane(kPair ? 2u * trip + 2u : trip + 1u);
+    ++trip;
+    const uint64_t cd0 = ll.cd[p];
+    const uint32_t nx0 = ll.next[p];
+    uint64_t cd1 = 0;
+    uint32_t nx1 = 0;
+    const uint32_t q = p - 1u;
+    if (kPair) {
+      cd1 = ll.cd[q];
+      nx1 = ll.next[q];
+    }
+    ll.next[p] = kUnset;
+    {
+      uint64_t ev;
+      uint32_t evc;
+      a.insert_bounded((cd0 & 0xFFFFFFFF00000000ull) | p, (uint32_t)cd0, ev, evc, w);
+      if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
+      ++cnt;
+    }
+    if (kPair && nx0 == q && q != 0u) {
+      ll.next[q] = kUnset;
+      uint64_t ev;
+      uint32_t evc;
+      a.insert_bounded((cd1 & 0xFFFFFFFF00000000ull) | q, (uint32_t)cd1, ev, evc, w);
+      if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
+      ++cnt;
+      p = nx1;
+    } else {
+      p = nx0;
+    }
+  }
+  const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
+  const uint32_t nmax = wave_max_u32(n);
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
+  return over(col, tailc);
+}
+
+// Key-only form of ll_walk_reset: the walk keeps (depth key, node) pairs and
+// the front-K colours are gathered by node index at the end.
+template <int K, bool kTail, bool kPair>
+__device__ __forceinline__ float4 ll_walk_reset_keys(uint32_t t, const LLBuf& ll, const float* s_dec,
+                                                     const float* s_alpha, FrontKeys<K>& a, uint32_t& cnt) {
+  a.init();
+  float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t p = ll.head[t];
+  if (p != 0u) ll.head[t] = 0u;
+  cnt = 0;
+  uint32_t trip = 0;
+  while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
+    const int w = (int)__builtin_amdgcn_readfirstlane(kPair ? 2u * trip + 2u : trip + 1u);
+    ++trip;
+    const uint32_t key0 = (uint32_t)(ll.cd[p] >> 32);
+    const uint32_t nx0 = ll.next[p];
+    uint32_t key1 = 0;
+    uint32_t nx1 = 0;
+    const uint32_t q = p - 1u;
+    if (kPair) {
+      key1 = (uint32_t)(ll.cd[q] >> 32);
+      nx1 = ll.next[q];
+    }
+    ll.next[p] = kUnset;
+    {
+      const uint64_t ev = a.insert_bounded(((uint64_t)key0 << 32) | p, w);
+      if (kTail && cnt >= (uint32_t)K)
+        tailc = over(tailc, decode_premult_lds((uint32_t)ll.cd[(uint32_t)ev], s_dec, s_alpha));
+      ++cnt;
+    }
+    if (kPair && nx0 == q && q != 0u) {
+      ll.next[q] = kUnset;
+      const uint64_t ev = a.insert_bounded(((uint64_t)key1 << 32) | q, w);
+      if (kTail && cnt >= (uint32_t)K)
+        tailc = over(tailc, decode_premult_lds((uint32_t)ll.cd[(uint32_t)ev], s_dec, s_alpha));
+      ++cnt;
+      p = nx1;
+    } else {
+      p = nx0;
+    }
+  }
+  const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
+  const uint32_t nmax = wave_max_u32(n);
+  uint32_t colr[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    colr[k] = 0u;
+    if ((uint32_t)k < nmax && (uint32_t)k < n) colr[k] = (uint32_t)ll.cd[(uint32_t)a.sk[k]];
+  }
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult_lds(colr[k], s_dec, s_alpha));
+  return over(col, tailc);
+}
+
+__device__ __forceinline__ void stage_decode(float* s_dec, float* s_alpha, const CodecTables* tab) {
+  for (uint32_t k = threadIdx.x; k < 256u; k += kBlock) {
+    s_dec[k] = tab->dec[k];
+    s_alpha[k] = tab->alpha[k];
   }
 }
 
@@ -395,10 +545,7 @@ template <int K, bool kTail, bool kColorTail, bool kDebug, bool kPair>
 __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll, TailBuf tb, ResolveOut o) {
   __shared__ float s_dec[256];
   __shared__ float s_alpha[256];
-  for (uint32_t k = threadIdx.x; k < 256u; k += kBlock) {
-    s_dec[k] = c.tab->dec[k];
-    s_alpha[k] = c.tab->alpha[k];
-  }
+  stage_decode(s_dec, s_alpha, c.tab);
   if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
     ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
     ll.counters[threadIdx.x] = 0u;
@@ -407,71 +554,20 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
   const uint32_t stride = gridDim.x * kBlock;
   for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
     FrontK<K> a;
-    a.init();
-    float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t p = ll.head[t];
-    if (p != 0u) ll.head[t] = 0u;
-    uint32_t cnt = 0;
-    // trips of this walk so far; every active lane has the same value, so
-    // readfirstlane makes it a scalar bound on the occupied slots (a trip
-    // consumes at most two nodes)
-    uint32_t trip = 0;
-    while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
-      const int w = (int)__builtin_amdgcn_readfirstlane(kPair ? 2u * trip + 2u : trip + 1u);
-      ++trip;
-      const uint64_t cd0 = ll.cd[p];
-      const uint32_t nx0 = ll.next[p];
-      uint64_t cd1 = 0;
-      uint32_t nx1 = 0;
-      const uint32_t q = p - 1u;
-      if (kPair) {
-        cd1 = ll.cd[q];
-        nx1 = ll.next[q];
-      }
-      ll.next[p] = kUnset;
-      {
-        uint64_t ev;
-        uint32_t evc;
-        a.insert_bounded((cd0 & 0xFFFFFFFF00000000ull) | p, (uint32_t)cd0, ev, evc, w);
-        if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
-        ++cnt;
-      }
-      if (kPair && nx0 == q && q != 0u) {
-        ll.next[q] = kUnset;
-        uint64_t ev;
-        uint32_t evc;
-        a.insert_bounded((cd1 & 0xFFFFFFFF00000000ull) | q, (uint32_t)cd1, ev, evc, w);
-        if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
-        ++cnt;
-        p = nx1;
-      } else {
-        p = nx0;
-      }
-    }
-    const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
-    const uint32_t nmax = wave_max_u32(n);
-    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
-    col = over(col, tailc);
+    uint32_t cnt;
+    const float4 col = ll_walk_reset<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
     float4 d = c.background;
     if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
     o.out[t] = over(col, d);
-    if (kDebug) write_debug<K>(o, t, a, n, cnt, true, false);
+    if (kDebug) write_debug<K>(o, t, a, cnt < (uint32_t)K ? cnt : (uint32_t)K, cnt, true, false);
   }
 }
 
-// Key-only form of k_ll_resolve_reset: the walk keeps (depth key, node)
-// pairs and the front-K colours are gathered by node index at the end.
 template <int K, bool kTail, bool kColorTail, bool kDebug, bool kPair>
 __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset_keys(Common c, LLBuf ll, TailBuf tb, ResolveOut o) {
   __shared__ float s_dec[256];
   __shared__ float s_alpha[256];
-  for (uint32_t k = threadIdx.x; k < 256u; k += kBlock) {
-    s_dec[k] = c.tab->dec[k];
-    s_alpha[k] = c.tab->alpha[k];
-  }
+  stage_decode(s_dec, s_alpha, c.tab);
   if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
     ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
     ll.counters[threadIdx.x] = 0u;
@@ -480,59 +576,13 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset_keys(Common c, LLBu
   const uint32_t stride = gridDim.x * kBlock;
   for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
     FrontKeys<K> a;
-    a.init();
-    float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t p = ll.head[t];
-    if (p != 0u) ll.head[t] = 0u;
-    uint32_t cnt = 0;
-    uint32_t trip = 0;
-    while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
-      const int w = (int)__builtin_amdgcn_readfirstlane(kPair ? 2u * trip + 2u : trip + 1u);
-      ++trip;
-      const uint32_t key0 = (uint32_t)(ll.cd[p] >> 32);
-      const uint32_t nx0 = ll.next[p];
-      uint32_t key1 = 0;
-      uint32_t nx1 = 0;
-      const uint32_t q = p - 1u;
-      if (kPair) {
-        key1 = (uint32_t)(ll.cd[q] >> 32);
-        nx1 = ll.next[q];
-      }
-      ll.next[p] = kUnset;
-      {
-        const uint64_t ev = a.insert_bounded(((uint64_t)key0 << 32) | p, w);
-        if (kTail && cnt >= (uint32_t)K)
-          tailc = over(tailc, decode_premult_lds((uint32_t)ll.cd[(uint32_t)ev], s_dec, s_alpha));
-        ++cnt;
-      }
-      if (kPair && nx0 == q && q != 0u) {
-        ll.next[q] = kUnset;
-        const uint64_t ev = a.insert_bounded(((uint64_t)key1 << 32) | q, w);
-        if (kTail && cnt >= (uint32_t)K)
-          tailc = over(tailc, decode_premult_lds((uint32_t)ll.cd[(uint32_t)ev], s_dec, s_alpha));
-        ++cnt;
-        p = nx1;
-      } else {
-        p = nx0;
-      }
-    }
-    const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
-    const uint32_t nmax = wave_max_u32(n);
-    uint32_t colr[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      colr[k] = 0u;
-      if ((uint32_t)k < nmax && (uint32_t)k < n) colr[k] = (uint32_t)ll.cd[(uint32_t)a.sk[k]];
-    }
-    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult_lds(colr[k], s_dec, s_alpha));
-    col = over(col, tailc);
+    uint32_t cnt;
+    const float4 col = ll_walk_reset_keys<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
     float4 d = c.background;
     if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
     o.out[t] = over(col, d);
     if (kDebug) {
+      const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
       if (o.fk_key) {
 #pragma unroll
         for (int k = 0; k < K; ++k)
@@ -544,6 +594,73 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset_keys(Common c, LLBu
       }
       if (o.counts) o.counts[t] = cnt;
     }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Batched Linked List: many frames in one pool. Frame f's fragments are pool
+// nodes frag_base + 1 .. frag_base + count, its targets are head entries
+// target_base .. target_base + T - 1; both bases are padded so that no append
+// unit (64 * kU fragments) and no resolve tile (kBlock targets) spans two
+// frames. Each frame gets exactly the single-frame result.
+// ---------------------------------------------------------------------------
+
+// Last frame in [lo, hi) whose base is <= key (bases ascend); key is wave
+// uniform, so the probes are scalar loads.
+template <bool kTargets>
+__device__ __forceinline__ uint32_t find_frame(const BatchFrameDev* fr, uint32_t lo, uint32_t hi, uint64_t key) {
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    const uint64_t base = kTargets ? (uint64_t)fr[mid].target_base : (uint64_t)fr[mid].frag_base;
+    if (base <= key) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+template <int kU>
+__global__ __launch_bounds__(kBlock) void k_batch_append(BatchArgs b, LLBuf ll) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, b.tab);
+  const uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+  if (unit >= b.units) return;
+  const uint64_t g = (uint64_t)unit * (64u * kU);
+  const uint32_t f = find_frame<false>(b.frames, b.f0, b.f1, g);
+  const BatchFrameDev fr = b.frames[f];
+  AppendSrc a;
+  a.target = fr.target;
+  a.depth = fr.depth;
+  a.rgba = fr.rgba;
+  a.count = fr.count;
+  a.first = g - fr.frag_base;
+  a.node0 = fr.frag_base + 1u;
+  a.T = fr.T;
+  a.tbase = fr.target_base;
+  a.lid0 = 0;
+  ll_append_wave<false, kU>(a, s_thr, ll, TailBuf{});
+}
+
+template <int K, bool kTail, bool kPair, bool kKeys>
+__global__ __launch_bounds__(kBlock) void k_batch_resolve(BatchArgs b, LLBuf ll) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, b.tab);
+  __syncthreads();
+  for (uint32_t tile = b.tile0 + blockIdx.x; tile < b.tile0 + b.tiles; tile += gridDim.x) {
+    const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, (uint64_t)tile * kBlock);
+    const BatchFrameDev& fr = b.frames[f];
+    const uint32_t t = tile * kBlock + threadIdx.x;
+    uint32_t cnt;
+    float4 col;
+    if constexpr (kKeys) {
+      FrontKeys<K> a;
+      col = ll_walk_reset_keys<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
+    } else {
+      FrontK<K> a;
+      col = ll_walk_reset<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
+    }
+    const uint32_t local = t - fr.target_base;
+    if (local < fr.T) fr.out[local] = over(col, fr.background);
   }
 }
 
@@ -1137,6 +1254,44 @@ hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t h
 hipError_t launch_tail_link(const TailBuf& tb, hipStream_t s) {
   hipLaunchKernelGGL(k_tail_link, dim3(1024), dim3(kBlock), 0, s, tb);
   return hipGetLastError();
+}
+
+namespace {
+
+template <int K>
+struct BatchResolve {
+  static hipError_t run(const BatchArgs& a, const LLBuf& ll, bool tail, bool pair, bool keys, uint32_t grid_cap,
+                        hipStream_t s) {
+    if (a.tiles == 0) return hipSuccess;
+    uint32_t blocks = a.tiles;
+    if (grid_cap && blocks > grid_cap) blocks = grid_cap;
+    const dim3 g(blocks), b(kBlock);
+#define OIT_BR(TAIL, PAIR, KEYS) hipLaunchKernelGGL((k_batch_resolve<K, TAIL, PAIR, KEYS>), g, b, 0, s, a, ll)
+#define OIT_BR_K(TAIL, PAIR) do { if (keys) OIT_BR(TAIL, PAIR, true); else OIT_BR(TAIL, PAIR, false); } while (0)
+#define OIT_BR_P(TAIL) do { if (pair) OIT_BR_K(TAIL, true); else OIT_BR_K(TAIL, false); } while (0)
+    if (tail) OIT_BR_P(true);
+    else OIT_BR_P(false);
+#undef OIT_BR_P
+#undef OIT_BR_K
+#undef OIT_BR
+    return hipGetLastError();
+  }
+};
+
+}  // namespace
+
+hipError_t launch_batch_append(const BatchArgs& a, const LLBuf& ll, uint32_t chunks, hipStream_t s) {
+  if (a.units == 0) return hipSuccess;
+  const dim3 g((a.units + kBlock / 64 - 1) / (kBlock / 64)), b(kBlock);
+  if (chunks == 8) hipLaunchKernelGGL((k_batch_append<8>), g, b, 0, s, a, ll);
+  else if (chunks == 4) hipLaunchKernelGGL((k_batch_append<4>), g, b, 0, s, a, ll);
+  else return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, bool pair, bool keys,
+                                uint32_t grid_cap, hipStream_t s) {
+  return dispatch_k<BatchResolve>(K, a, ll, tail, pair, keys, grid_cap, s);
 }
 
 }  // namespace oit

@@ -66,7 +66,34 @@ struct Common {
   const CodecTables* tab;
 };
 
+// One frame of a batch as the batched Linked List kernels see it.
+struct BatchFrameDev {
+  const uint32_t* target;  // frame-local targets
+  const float* depth;
+  const float4* rgba;
+  float4* out;             // [T] composite over the background
+  float4 background;
+  uint32_t count;          // fragments
+  uint32_t T;              // targets
+  uint32_t target_base;    // head index of target 0 (multiple of 256)
+  uint32_t frag_base;      // fragment 0 is pool node frag_base + 1 (multiple of 512)
+};
+
+struct BatchArgs {
+  const BatchFrameDev* frames;  // device table, bases ascending
+  uint32_t f0, f1;              // frames this launch covers
+  uint32_t units;               // append: waves of 64 * chunks fragments from frames[f0].frag_base
+  uint32_t tile0, tiles;        // resolve: tiles of 256 targets
+  const CodecTables* tab;
+};
+
+// sRGB8 codec tables built on the host (exact curve, bisection in double).
+void build_codec_tables(CodecTables* t);
+
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+hipError_t launch_batch_append(const BatchArgs& a, const LLBuf& ll, uint32_t chunks, hipStream_t s);
+hipError_t launch_batch_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, bool pair, bool keys,
+                                uint32_t grid_cap, hipStream_t s);
 
 hipError_t launch_ll_store(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s);
 hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, hipStream_t s);

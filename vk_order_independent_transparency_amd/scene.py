@@ -135,3 +135,53 @@ class DeviceScene:
 
 def _ptr(b) -> int:
     return b if isinstance(b, int) else b.ptr
+
+
+class _SplitMix64:
+    """The same splitmix64 the native generator uses, for workload draws."""
+
+    def __init__(self, seed: int):
+        self.s = seed & 0xFFFFFFFFFFFFFFFF
+
+    def next(self) -> int:
+        self.s = (self.s + 0x9E3779B97F4A7C15) & 0xFFFFFFFFFFFFFFFF
+        z = self.s
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & 0xFFFFFFFFFFFFFFFF
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & 0xFFFFFFFFFFFFFFFF
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next() >> 11) * 2.0 ** -53
+
+    def choice(self, options):
+        return options[self.next() % len(options)]
+
+    def log_uniform_int(self, lo: int, hi: int) -> int:
+        import math
+
+        return int(round(math.exp(math.log(lo) + self.uniform() * (math.log(hi) - math.log(lo)))))
+
+
+@dataclasses.dataclass(frozen=True)
+class BatchFrame:
+    scene: SceneParams
+    layers: int
+
+
+def batch_workload(seed: int = 1, frames: int = 64) -> list[BatchFrame]:
+    """Batched workload: `frames` frames, square resolution uniform from
+    {256, 512, 1024, 2048}, sphere count log-uniform in [100, 5000], 20 % of
+    the spheres packed into a 0.1-wide square (local depth complexity up to
+    a few hundred layers), K uniform from {4, 8, 16, 32}; radius [0.01, 0.05]
+    and alpha [0.1, 0.5] as in the 1920x1080 workload. Frame i's scene seed
+    is drawn from `seed`, so the list is a pure function of (seed, frames)."""
+    rng = _SplitMix64(seed ^ 0xB47C4)
+    out = []
+    for _ in range(frames):
+        res = rng.choice((256, 512, 1024, 2048))
+        spheres = rng.log_uniform_int(100, 5000)
+        K = rng.choice((4, 8, 16, 32))
+        fseed = rng.next()
+        out.append(BatchFrame(SceneParams(seed=fseed, width=res, height=res, spheres=spheres, radius=(0.01, 0.05),
+                                          alpha=(0.1, 0.5), cluster_fraction=0.2, cluster_extent=0.1), K))
+    return out
