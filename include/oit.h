@@ -104,6 +104,8 @@ typedef struct oit_sizes {
   uint64_t ll_capacity;       /* LL nodes including the terminator */
   uint32_t bytes_per_element; /* per stored fragment */
   uint32_t targets;           /* width * height * samples */
+  uint32_t reserved;
+  uint64_t resident_bytes;    /* device bytes the context holds (reconfigurations reuse them) */
 } oit_sizes;
 
 /* Counters of the last frame (synchronises `stream`). */
@@ -127,6 +129,10 @@ int oit_destroy(oit_ctx* ctx);
 int oit_configure(oit_ctx* ctx, const oit_config* cfg);
 /* Reports the allocation sizes of the current configuration. */
 int oit_query_sizes(const oit_ctx* ctx, oit_sizes* out);
+/* Frees every frame buffer (after the context's last stream-ordered call has
+ * completed) and leaves the context unconfigured. No counterpart in the
+ * reference; for hosts that share a device-memory budget between contexts. */
+int oit_trim(oit_ctx* ctx);
 /* Starts a frame: clears counters, heads, locks and k-buffers to the
  * reference's clear values. Replaces clearTransparent<Alg> (oitRender.cpp:
  * 156-174, 201-216, 303-311, 337-356) and the WBOIT clear values
@@ -208,6 +214,42 @@ int oit_event_record(void* event, void* stream);
 int oit_event_synchronize(void* event);
 int oit_event_elapsed_ms(float* out, void* start, void* stop);
 
+/* ---- Synthetic scene (stands in for the reference's rasterised sphere
+ * mesh, main.cpp:334-417): analytic spheres, orthographic, each covered
+ * pixel (or sample) centre emits a front and a back fragment. ---------- */
+typedef struct oit_scene_params {
+  uint64_t seed;
+  uint32_t width, height;
+  uint32_t samples;           /* 1, 2, 4 or 8 (the standard sample positions) */
+  uint32_t spheres;
+  float    radius_min, radius_max;
+  float    alpha_min, alpha_max;
+  float    cluster_fraction;  /* fraction of spheres packed in a cluster_extent-wide square */
+  float    cluster_extent;
+} oit_scene_params;
+
+/* Number of fragments the scene emits (host computation). */
+int oit_scene_count(const oit_scene_params* p, uint64_t* out_count);
+/* Writes the fragment stream in draw order into HOST arrays of `capacity`
+ * fragments (target[], depth[], rgba[4*]). */
+int oit_scene_generate_host(const oit_scene_params* p, uint32_t* target, float* depth,
+                            float* rgba, uint64_t capacity, uint64_t* out_count);
+
+/* Device generator of the same stream (bit-identical to
+ * oit_scene_generate_host, same order). Replaces the reference's scene
+ * set-up and rasterisation of the transparent spheres as the producer of
+ * fragments (main.cpp:334-417, object.vert.glsl:32-39,
+ * shaderCommon.glsl:48-56). Create builds the sphere table on the host and
+ * counts the fragments on `device` (synchronously); emit writes the stream
+ * into DEVICE arrays (target 8-byte aligned, depth 8-byte aligned, rgba
+ * 16-byte aligned) on `stream`, and may be called any number of times. */
+typedef struct oit_scene_gen oit_scene_gen;
+int oit_scene_gen_create(oit_scene_gen** out, const oit_scene_params* p, int device);
+int oit_scene_gen_destroy(oit_scene_gen* g);
+int oit_scene_gen_count(const oit_scene_gen* g, uint64_t* out_count);
+int oit_scene_gen_emit(oit_scene_gen* g, uint32_t* target, float* depth, float* rgba,
+                       uint64_t capacity, void* stream);
+
 /* ---- Batched Linked List frames ------------------------------------------
  * No counterpart in the reference, whose onRender draws one frame per command
  * buffer (oitRender.cpp:28-154); this is the batched form of the Linked List
@@ -268,41 +310,71 @@ int oit_batch_tune(oit_batch* b, const oit_batch_frame* frames, uint32_t n, void
 int oit_batch_get_tuning(const oit_batch* b, oit_batch_tuning* out);
 int oit_batch_set_tuning(oit_batch* b, const oit_batch_tuning* t);
 
-/* ---- Synthetic scene (stands in for the reference's rasterised sphere
- * mesh, main.cpp:334-417): analytic spheres, orthographic, each covered
- * pixel (or sample) centre emits a front and a back fragment. ---------- */
-typedef struct oit_scene_params {
-  uint64_t seed;
-  uint32_t width, height;
-  uint32_t samples;           /* 1, 2, 4 or 8 (the standard sample positions) */
-  uint32_t spheres;
-  float    radius_min, radius_max;
-  float    alpha_min, alpha_max;
-  float    cluster_fraction;  /* fraction of spheres packed in a cluster_extent-wide square */
-  float    cluster_extent;
-} oit_scene_params;
+/* ---- Request loop ---------------------------------------------------------
+ * No counterpart in the reference (one interactive frame loop,
+ * Sample::onRender, oitRender.cpp:28-154). Requests (a seeded scene, an
+ * algorithm, K) are taken from any thread; a dispatcher thread runs up to
+ * `slots` of them at once on their own streams (continuous batching: a
+ * request that arrives while others run starts as soon as a slot and memory
+ * are free, and its kernels overlap theirs). Each request's fragments are
+ * generated and counted on the device; its A-buffer is sized from that exact
+ * count (Linked List pool = fragments + 1 nodes) and granted against
+ * `budget_bytes` in arrival order; contexts keep their buffers between
+ * requests and idle ones are trimmed when the budget needs room; a request
+ * larger than the whole budget completes with OIT_ERR_OVER_BUDGET. Each
+ * result equals the single-frame result of the same request. */
+typedef struct oit_server_config {
+  uint32_t slots;             /* requests in flight, 1..16 */
+  uint32_t reserved;          /* 0 */
+  uint64_t budget_bytes;      /* device bytes for A-buffers, pools, aux and tail arrays */
+} oit_server_config;
 
-/* Number of fragments the scene emits (host computation). */
-int oit_scene_count(const oit_scene_params* p, uint64_t* out_count);
-/* Writes the fragment stream in draw order into HOST arrays of `capacity`
- * fragments (target[], depth[], rgba[4*]). */
-int oit_scene_generate_host(const oit_scene_params* p, uint32_t* target, float* depth,
-                            float* rgba, uint64_t capacity, uint64_t* out_count);
+typedef struct oit_request {
+  oit_scene_params scene;
+  uint32_t algorithm;         /* OIT_SIMPLE, OIT_LINKEDLIST, OIT_LOOP64, OIT_SPINLOCK, OIT_WEIGHTED */
+  uint32_t layers;            /* K */
+  uint32_t tail_blend;
+  uint32_t reserved;          /* 0 */
+  float*   out;               /* device [W*H*S][4] (16-byte aligned), or NULL for the slot's own buffer */
+  uint64_t tag;               /* returned in the completion */
+} oit_request;
 
-/* Device generator of the same stream (bit-identical to
- * oit_scene_generate_host, same order). Replaces the reference's scene
- * set-up and rasterisation of the transparent spheres as the producer of
- * fragments (main.cpp:334-417, object.vert.glsl:32-39,
- * shaderCommon.glsl:48-56). Create builds the sphere table on the host and
- * counts the fragments on `device` (synchronously); emit writes the stream
- * into DEVICE arrays (target 8-byte aligned, depth 8-byte aligned, rgba
- * 16-byte aligned) on `stream`, and may be called any number of times. */
-typedef struct oit_scene_gen oit_scene_gen;
-int oit_scene_gen_create(oit_scene_gen** out, const oit_scene_params* p, int device);
-int oit_scene_gen_destroy(oit_scene_gen* g);
-int oit_scene_gen_count(const oit_scene_gen* g, uint64_t* out_count);
-int oit_scene_gen_emit(oit_scene_gen* g, uint32_t* target, float* depth, float* rgba,
-                       uint64_t capacity, void* stream);
+typedef struct oit_completion {
+  uint64_t tag;
+  int32_t  status;            /* OIT_OK or the first error */
+  uint32_t slot;
+  uint64_t fragments;         /* exact count of the request's scene */
+  uint64_t abuffer_bytes;     /* bytes granted from the budget */
+  const float* out;           /* the image (a slot's own buffer stays valid until the slot's next request) */
+  double   t_submit_us;       /* steady clock, microseconds since the server was created */
+  double   t_start_us;        /* memory granted, kernels enqueued */
+  double   t_done_us;         /* completion seen by the dispatcher */
+} oit_completion;
+
+typedef struct oit_server_stats {
+  uint64_t submitted, completed, failed;
+  uint64_t held_bytes;        /* device bytes the contexts hold now */
+  uint64_t peak_held_bytes;
+  uint32_t peak_in_flight;
+  uint32_t slots;
+} oit_server_stats;
+
+typedef struct oit_server oit_server;
+int oit_server_create(oit_server** out, int device, const oit_server_config* cfg);
+/* Finishes every submitted request, then frees everything. */
+int oit_server_destroy(oit_server* s);
+/* Queues a request (stamps its arrival time); returns at once. */
+int oit_server_submit(oit_server* s, const oit_request* r);
+/* Moves up to `max` completions into out[], *n of them. */
+int oit_server_poll(oit_server* s, oit_completion* out, uint32_t max, uint32_t* n);
+/* Waits until every submitted request has completed (OIT_ERR_NOT_READY on timeout). */
+int oit_server_drain(oit_server* s, uint32_t timeout_ms);
+/* Load driver: submits reqs[i] at arrival_us[i] (non-decreasing, relative to
+ * the call) from the calling thread, waits for all, and writes out[i] for
+ * reqs[i]. */
+int oit_server_replay(oit_server* s, const oit_request* reqs, const double* arrival_us, uint32_t n,
+                      uint32_t timeout_ms, oit_completion* out);
+int oit_server_get_stats(oit_server* s, oit_server_stats* out);
 
 #ifdef __cplusplus
 }

@@ -6,6 +6,7 @@ import os
 import re
 import subprocess
 
+import numpy as np
 import pytest
 
 from vk_order_independent_transparency_amd import _abi
@@ -40,6 +41,10 @@ STRUCTS = {
     "oit_batch_frame": _abi.OitBatchFrame,
     "oit_batch_tuning": _abi.OitBatchTuning,
     "oit_batch_sizes": _abi.OitBatchSizes,
+    "oit_server_config": _abi.OitServerConfig,
+    "oit_request": _abi.OitRequest,
+    "oit_completion": _abi.OitCompletion,
+    "oit_server_stats": _abi.OitServerStats,
 }
 
 
@@ -158,3 +163,33 @@ def test_device_scene_generator_argument_checks():
         ok = scene.SMALL_256.to_c()
         assert lib.oit_scene_gen_create(ctypes.byref(g), ctypes.byref(ok), 0) == _abi.OIT_ERR_DEVICE
         assert not g.value
+
+
+def test_server_argument_checks():
+    lib = _abi.lib()
+    from vk_order_independent_transparency_amd import device_count
+
+    s = ctypes.c_void_p()
+    cfg = _abi.OitServerConfig(slots=0, reserved=0, budget_bytes=1 << 30)
+    assert lib.oit_server_create(ctypes.byref(s), 0, ctypes.byref(cfg)) == _abi.OIT_ERR_INVALID_ARG
+    cfg.slots = 17
+    assert lib.oit_server_create(ctypes.byref(s), 0, ctypes.byref(cfg)) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_server_submit(None, None) == _abi.OIT_ERR_INVALID_ARG
+    assert lib.oit_server_destroy(None) == _abi.OIT_ERR_INVALID_ARG
+    if device_count() == 0:
+        cfg.slots = 2
+        assert lib.oit_server_create(ctypes.byref(s), 0, ctypes.byref(cfg)) == _abi.OIT_ERR_DEVICE
+
+
+def test_served_workload_is_deterministic():
+    from vk_order_independent_transparency_amd import serve
+
+    a, ta = serve.served_workload(7, 50, 200.0)
+    b, tb = serve.served_workload(7, 50, 200.0)
+    assert a == b and (ta == tb).all()
+    assert (np.diff(ta) > 0).all()
+    gaps = np.diff(np.concatenate([[0.0], ta]))
+    assert 2000 < gaps.mean() < 8000  # mean gap 1/rate = 5000 us
+    assert {r.algorithm for r in a} <= {1, 3, 6} and {r.layers for r in a} <= {8, 16, 32}
+    assert all((r.scene.width, r.scene.height) in serve.RESOLUTIONS for r in a)
+    assert all(200 <= r.scene.spheres <= 8000 for r in a)

@@ -23,3 +23,4 @@ from .renderer import FrameConfig, Renderer, State  # noqa: F401
 from . import scene  # noqa: F401
 from .device import DeviceBuffer, Event, Stream, device_count  # noqa: F401
 from .batch import Batch, BatchFrame  # noqa: F401
+from .serve import Request, Server  # noqa: F401

@@ -88,6 +88,8 @@ class OitSizes(ctypes.Structure):
         ("ll_capacity", ctypes.c_uint64),
         ("bytes_per_element", ctypes.c_uint32),
         ("targets", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("resident_bytes", ctypes.c_uint64),
     ]
 
 
@@ -155,6 +157,48 @@ class OitSceneParams(ctypes.Structure):
     ]
 
 
+class OitServerConfig(ctypes.Structure):
+    _fields_ = [("slots", ctypes.c_uint32), ("reserved", ctypes.c_uint32), ("budget_bytes", ctypes.c_uint64)]
+
+
+class OitRequest(ctypes.Structure):
+    _fields_ = [
+        ("scene", OitSceneParams),
+        ("algorithm", ctypes.c_uint32),
+        ("layers", ctypes.c_uint32),
+        ("tail_blend", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
+        ("out", ctypes.c_void_p),
+        ("tag", ctypes.c_uint64),
+    ]
+
+
+class OitCompletion(ctypes.Structure):
+    _fields_ = [
+        ("tag", ctypes.c_uint64),
+        ("status", ctypes.c_int32),
+        ("slot", ctypes.c_uint32),
+        ("fragments", ctypes.c_uint64),
+        ("abuffer_bytes", ctypes.c_uint64),
+        ("out", ctypes.c_void_p),
+        ("t_submit_us", ctypes.c_double),
+        ("t_start_us", ctypes.c_double),
+        ("t_done_us", ctypes.c_double),
+    ]
+
+
+class OitServerStats(ctypes.Structure):
+    _fields_ = [
+        ("submitted", ctypes.c_uint64),
+        ("completed", ctypes.c_uint64),
+        ("failed", ctypes.c_uint64),
+        ("held_bytes", ctypes.c_uint64),
+        ("peak_held_bytes", ctypes.c_uint64),
+        ("peak_in_flight", ctypes.c_uint32),
+        ("slots", ctypes.c_uint32),
+    ]
+
+
 # name -> (restype, argtypes); every symbol include/oit.h declares
 _P = ctypes.c_void_p
 SIGNATURES = {
@@ -162,6 +206,7 @@ SIGNATURES = {
     "oit_destroy": (ctypes.c_int, [_P]),
     "oit_configure": (ctypes.c_int, [_P, ctypes.POINTER(OitConfig)]),
     "oit_query_sizes": (ctypes.c_int, [_P, ctypes.POINTER(OitSizes)]),
+    "oit_trim": (ctypes.c_int, [_P]),
     "oit_clear": (ctypes.c_int, [_P, _P]),
     "oit_append": (ctypes.c_int, [_P, ctypes.POINTER(OitFragments), _P]),
     "oit_resolve": (ctypes.c_int, [_P, _P, _P]),
@@ -203,6 +248,15 @@ SIGNATURES = {
     "oit_batch_tune": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchFrame), ctypes.c_uint32, _P]),
     "oit_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchTuning)]),
     "oit_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchTuning)]),
+    "oit_server_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(OitServerConfig)]),
+    "oit_server_destroy": (ctypes.c_int, [_P]),
+    "oit_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(OitRequest)]),
+    "oit_server_poll": (ctypes.c_int, [_P, ctypes.POINTER(OitCompletion), ctypes.c_uint32,
+                                       ctypes.POINTER(ctypes.c_uint32)]),
+    "oit_server_drain": (ctypes.c_int, [_P, ctypes.c_uint32]),
+    "oit_server_replay": (ctypes.c_int, [_P, ctypes.POINTER(OitRequest), ctypes.POINTER(ctypes.c_double),
+                                         ctypes.c_uint32, ctypes.c_uint32, ctypes.POINTER(OitCompletion)]),
+    "oit_server_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(OitServerStats)]),
     "oit_scene_gen_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.POINTER(OitSceneParams), ctypes.c_int]),
     "oit_scene_gen_destroy": (ctypes.c_int, [_P]),
     "oit_scene_gen_count": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_uint64)]),

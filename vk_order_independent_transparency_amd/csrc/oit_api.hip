@@ -127,6 +127,15 @@ struct oit_ctx {
   std::vector<oit::FragView> segments;
   // sizes
   uint64_t bytes_abuf = 0, bytes_aux = 0;
+  // allocated elements of each frame buffer: a reconfiguration reuses a
+  // buffer while the new configuration fits, so a context that serves many
+  // frame sizes stops allocating once it has seen the largest
+  uint64_t cap_ll_head = 0, ll_alloc = 0;
+  uint64_t cap_kb_count = 0, cap_kb_lock = 0, cap_kb_depth = 0, cap_kb_elems = 0;
+  uint64_t cap_l64 = 0, cap_wb = 0, cap_tl_head = 0;
+  // the last stream-ordered call's completion: buffers are freed only after it
+  hipEvent_t work_ev = nullptr;
+  bool work_pending = false;
   // profiling
   bool profiling = false;
   std::vector<EventPair> events;
@@ -167,7 +176,48 @@ void dev_free(T** p) {
   *p = nullptr;
 }
 
+// Waits for the context's last stream-ordered work (before buffers it may
+// use are freed).
+void wait_idle(oit_ctx* c) {
+  if (c->work_pending) {
+    (void)hipEventSynchronize(c->work_ev);
+    c->work_pending = false;
+  }
+}
+
+void mark_work(oit_ctx* c, hipStream_t s) {
+  if (c->work_ev && hipEventRecord(c->work_ev, s) == hipSuccess) c->work_pending = true;
+}
+
+// Makes *p hold at least `need` elements, reusing the allocation when it
+// fits; *grown reports a new allocation (its contents are undefined).
+template <typename T>
+int reserve(oit_ctx* c, T** p, uint64_t* cap, uint64_t need, bool* grown = nullptr) {
+  if (need == 0 || (*p && need <= *cap)) return OIT_OK;
+  wait_idle(c);
+  dev_free(p);
+  *cap = 0;
+  const int rc = dev_alloc(p, need);
+  if (rc) return rc;
+  *cap = need;
+  if (grown) *grown = true;
+  return OIT_OK;
+}
+
+template <typename T>
+void release(oit_ctx* c, T** p, uint64_t* cap) {
+  if (*p) {
+    wait_idle(c);
+    dev_free(p);
+  }
+  *cap = 0;
+}
+
 void free_frame_buffers(oit_ctx* c) {
+  wait_idle(c);
+  c->cap_ll_head = c->ll_alloc = 0;
+  c->cap_kb_count = c->cap_kb_lock = c->cap_kb_depth = c->cap_kb_elems = 0;
+  c->cap_l64 = c->cap_wb = c->cap_tl_head = 0;
   dev_free(&c->ll_head);
   dev_free(&c->ll_cd);
   dev_free(&c->ll_next);
@@ -250,8 +300,8 @@ bool uses_tail_lists(const oit_ctx* c) { return c->cfg.tail_blend && c->cfg.algo
 // frame's records (stream-ordered copies; the old buffers are freed after a
 // stream synchronisation).
 int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
-  if (!c->tl_head) {
-    int rc = dev_alloc(&c->tl_head, c->T);
+  if (!c->tl_head || c->cap_tl_head < c->T) {
+    int rc = reserve(c, &c->tl_head, &c->cap_tl_head, c->T);
     if (rc) return rc;
     c->tl_head_valid = false;
   }
@@ -331,6 +381,7 @@ int oit_create(oit_ctx** out, int device) {
   if (!rc) rc = dev_alloc(&c->d_counters, 32);
   if (!rc && hipMemcpy(c->d_tab, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess) rc = OIT_ERR_DEVICE;
   if (!rc && hipMemset(c->d_counters, 0, 32 * sizeof(uint32_t)) != hipSuccess) rc = OIT_ERR_DEVICE;
+  if (!rc && hipEventCreateWithFlags(&c->work_ev, hipEventDisableTiming) != hipSuccess) rc = OIT_ERR_DEVICE;
   if (rc) {
     dev_free(&c->d_tab);
     dev_free(&c->d_counters);
@@ -348,6 +399,7 @@ int oit_destroy(oit_ctx* c) {
   free_frame_buffers(c);
   dev_free(&c->d_tab);
   dev_free(&c->d_counters);
+  if (c->work_ev) (void)hipEventDestroy(c->work_ev);
   for (auto& p : c->events) {
     (void)hipEventDestroy(p.a);
     (void)hipEventDestroy(p.b);
@@ -371,78 +423,126 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   if (T >= 0xFFFFFFFFull) return OIT_ERR_INVALID_ARG;
   for (int k = 0; k < 4; ++k)
     if (!std::isfinite(cfg->background[k])) return OIT_ERR_INVALID_ARG;
+  uint64_t ll_cap = 0;
+  if (alg == OIT_LINKEDLIST) {  // oit.cpp:119-126
+    ll_cap = cfg->ll_pool_nodes;
+    if (ll_cap == 0) ll_cap = (uint64_t)(cfg->ll_nodes_per_pixel ? cfg->ll_nodes_per_pixel : 10) * T;
+    if (ll_cap < 1) ll_cap = 1;
+    if (ll_cap > kMaxPoolNodes) return OIT_ERR_CAPACITY;
+  }
   DeviceGuard g(c->device);
-  (void)hipDeviceSynchronize();
-  free_frame_buffers(c);
+  c->configured = false;
   c->cfg = *cfg;
   c->cfg.samples = S;
   if (alg == OIT_WEIGHTED && !valid_layers(c->cfg.layers)) c->cfg.layers = 1;
   c->T = (uint32_t)T;
   const uint32_t K = c->cfg.layers;
+  // buffers of the other algorithms go; this algorithm's are reused if they fit
+  if (alg != OIT_LINKEDLIST) {
+    release(c, &c->ll_head, &c->cap_ll_head);
+    uint64_t dummy = c->ll_alloc;
+    release(c, &c->ll_cd, &dummy);
+    release(c, &c->ll_next, &c->ll_alloc);
+    c->ll_cap = 0;
+  }
+  if (alg != OIT_SIMPLE && alg != OIT_SPINLOCK) {
+    release(c, &c->kb_count, &c->cap_kb_count);
+    uint64_t d1 = c->cap_kb_elems, d2 = c->cap_kb_elems;
+    release(c, &c->kb_col, &d1);
+    release(c, &c->kb_key, &d2);
+    release(c, &c->kb_id, &c->cap_kb_elems);
+  }
+  if (alg != OIT_SPINLOCK) {
+    release(c, &c->kb_lock, &c->cap_kb_lock);
+    release(c, &c->kb_depth, &c->cap_kb_depth);
+  }
+  if (alg != OIT_LOOP64) release(c, &c->l64, &c->cap_l64);
+  if (alg != OIT_WEIGHTED) {
+    uint64_t d = c->cap_wb;
+    release(c, &c->wb_accum, &d);
+    release(c, &c->wb_reveal, &c->cap_wb);
+  }
   int rc = OIT_OK;
+  bool ll_grown = false;
   switch (alg) {
-    case OIT_LINKEDLIST: {  // oit.cpp:119-126
-      uint64_t cap = cfg->ll_pool_nodes;
-      if (cap == 0) cap = (uint64_t)(cfg->ll_nodes_per_pixel ? cfg->ll_nodes_per_pixel : 10) * T;
-      if (cap < 1) cap = 1;
-      if (cap > kMaxPoolNodes) return OIT_ERR_CAPACITY;
-      c->ll_cap = cap;
-      rc = dev_alloc(&c->ll_head, T);
-      if (!rc) rc = dev_alloc(&c->ll_cd, cap);
-      if (!rc) rc = dev_alloc(&c->ll_next, cap);
-      c->bytes_abuf = cap * 12u;
+    case OIT_LINKEDLIST: {
+      rc = reserve(c, &c->ll_head, &c->cap_ll_head, T, &ll_grown);
+      if (!rc && ll_cap > c->ll_alloc) {  // node payloads and links share one capacity
+        uint64_t d = c->ll_alloc;
+        rc = reserve(c, &c->ll_cd, &d, ll_cap, &ll_grown);
+        if (!rc) rc = reserve(c, &c->ll_next, &c->ll_alloc, ll_cap, &ll_grown);
+      }
+      c->ll_cap = ll_cap;
+      c->bytes_abuf = ll_cap * 12u;
       c->bytes_aux = T * 4u;
       break;
     }
     case OIT_SIMPLE:
     case OIT_SPINLOCK: {  // oit.cpp:102-118
-      rc = dev_alloc(&c->kb_count, T);
-      if (!rc) rc = dev_alloc(&c->kb_col, T * K);
-      if (!rc) rc = dev_alloc(&c->kb_key, T * K);
-      if (!rc) rc = dev_alloc(&c->kb_id, T * K);
-      if (!rc && alg == OIT_SPINLOCK) rc = dev_alloc(&c->kb_lock, T);
-      if (!rc && alg == OIT_SPINLOCK) rc = dev_alloc(&c->kb_depth, T);
+      rc = reserve(c, &c->kb_count, &c->cap_kb_count, T);
+      if (!rc && T * K > c->cap_kb_elems) {
+        uint64_t d1 = c->cap_kb_elems, d2 = c->cap_kb_elems;
+        rc = reserve(c, &c->kb_col, &d1, T * K);
+        if (!rc) rc = reserve(c, &c->kb_key, &d2, T * K);
+        if (!rc) rc = reserve(c, &c->kb_id, &c->cap_kb_elems, T * K);
+      }
+      if (!rc && alg == OIT_SPINLOCK) rc = reserve(c, &c->kb_lock, &c->cap_kb_lock, T);
+      if (!rc && alg == OIT_SPINLOCK) rc = reserve(c, &c->kb_depth, &c->cap_kb_depth, T);
       c->bytes_abuf = T * K * 12u;
       c->bytes_aux = T * (alg == OIT_SPINLOCK ? 12u : 4u);
       break;
     }
     case OIT_LOOP64: {  // oit.cpp:134-140
-      rc = dev_alloc(&c->l64, T * K);
+      rc = reserve(c, &c->l64, &c->cap_l64, T * K);
       c->bytes_abuf = T * K * 8u;
+      c->bytes_aux = 0;
       break;
     }
     case OIT_WEIGHTED: {  // oit.h:234-235 (binary32 here)
-      rc = dev_alloc(&c->wb_accum, T);
-      if (!rc) rc = dev_alloc(&c->wb_reveal, T);
+      if (T > c->cap_wb) {
+        uint64_t d = c->cap_wb;
+        rc = reserve(c, &c->wb_accum, &d, T);
+        if (!rc) rc = reserve(c, &c->wb_reveal, &c->cap_wb, T);
+      }
       c->bytes_abuf = T * 20u;
+      c->bytes_aux = 0;
       break;
     }
   }
   if (!rc && uses_tail_lists(c)) {
-    rc = dev_alloc(&c->tl_head, T);
-    c->tl_head_valid = false;
-    if (!rc && cfg->max_fragments) {
+    rc = reserve(c, &c->tl_head, &c->cap_tl_head, T);
+    if (!rc && cfg->max_fragments && cfg->max_fragments > c->tl_cap) {
       const uint64_t cap = cfg->max_fragments < kMaxFrameFragments ? cfg->max_fragments : kMaxFrameFragments;
-      rc = dev_alloc(&c->tl_next, cap);
-      if (!rc) rc = dev_alloc(&c->tl_rgba, cap);
-      if (!rc) rc = dev_alloc(&c->tl_rec, cap);
+      uint64_t d1 = c->tl_cap, d2 = c->tl_cap, d3 = c->tl_cap;
+      rc = reserve(c, &c->tl_next, &d1, cap);
+      if (!rc) rc = reserve(c, &c->tl_rgba, &d2, cap);
+      if (!rc) rc = reserve(c, &c->tl_rec, &d3, cap);
       if (!rc) c->tl_cap = (uint32_t)cap;
     }
-  }
-  if (!rc && alg == OIT_LINKEDLIST) {  // pool starts clean: heads 0, links kUnset
-    hipError_t e = oit::launch_fill_u32(c->ll_head, T, 0u, nullptr);
-    if (e == hipSuccess) e = oit::launch_fill_u32(c->ll_next, c->ll_cap, oit::kUnset, nullptr);
-    if (e == hipSuccess) e = oit::launch_fill_u32(c->d_counters, 32, 0u, nullptr);
-    if (e == hipSuccess) e = hipDeviceSynchronize();
-    if (e != hipSuccess) rc = status_of(e);
+  } else if (!rc) {
+    release(c, &c->tl_head, &c->cap_tl_head);
+    uint64_t d1 = c->tl_cap, d2 = c->tl_cap, d3 = c->tl_cap;
+    release(c, &c->tl_next, &d1);
+    release(c, &c->tl_rgba, &d2);
+    release(c, &c->tl_rec, &d3);
+    c->tl_cap = 0;
   }
   if (rc) {
     free_frame_buffers(c);
     return rc;
   }
-  c->ll_clean = alg == OIT_LINKEDLIST;
-  c->ll_dirty_upto = 0;
-  c->counters_clean = alg == OIT_LINKEDLIST;
+  c->tl_head_valid = false;
+  if (alg == OIT_LINKEDLIST && ll_grown) {
+    // a new allocation is filled clean (every head 0, every link kUnset) by
+    // the next clear, on its stream
+    c->ll_clean = false;
+    c->ll_dirty_upto = c->ll_alloc;
+  }
+  if (alg != OIT_LINKEDLIST) {
+    c->ll_clean = false;
+    c->ll_dirty_upto = 0;
+  }
+  c->counters_clean = false;
   c->stats_moved = false;
   c->configured = true;
   c->frame_open = false;
@@ -450,6 +550,17 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   c->frame_frags = 0;
   c->appends = 0;
   c->segments.clear();
+  return OIT_OK;
+}
+
+int oit_trim(oit_ctx* c) {
+  if (!c) return OIT_ERR_INVALID_ARG;
+  DeviceGuard g(c->device);
+  free_frame_buffers(c);
+  c->ll_clean = false;
+  c->ll_dirty_upto = 0;
+  c->frame_open = false;
+  c->resolved = false;
   return OIT_OK;
 }
 
@@ -470,6 +581,9 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
     default: out->bytes_per_element = 20; break;
   }
   out->targets = c->T;
+  out->resident_bytes = c->cap_ll_head * 4u + c->ll_alloc * 12u + c->cap_kb_count * 4u + c->cap_kb_lock * 4u +
+                        c->cap_kb_depth * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_wb * 20u +
+                        c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u);
   return OIT_OK;
 }
 
@@ -502,9 +616,10 @@ int oit_clear(oit_ctx* c, void* stream) {
     if (need_counters) r = oit::launch_fill_u32(c->d_counters, 16, 0u, s);
     switch (c->cfg.algorithm) {
       case OIT_LINKEDLIST:  // head and counter to 0, oitRender.cpp:206-212; links of a dirty range to kUnset
-        if (need_ll_fill && r == hipSuccess) r = oit::launch_fill_u32(c->ll_head, T, 0u, s);
+        // every allocated head: a larger earlier frame may have left some
+        if (need_ll_fill && r == hipSuccess) r = oit::launch_fill_u32(c->ll_head, c->cap_ll_head, 0u, s);
         if (need_ll_fill && r == hipSuccess)
-          r = oit::launch_fill_u32(c->ll_next, c->ll_dirty_upto ? c->ll_dirty_upto : c->ll_cap, oit::kUnset, s);
+          r = oit::launch_fill_u32(c->ll_next, c->ll_dirty_upto ? c->ll_dirty_upto : c->ll_alloc, oit::kUnset, s);
         break;
       case OIT_SIMPLE:  // count to 0, oitRender.cpp:161-169
         if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_count, T, 0u, s);
@@ -537,6 +652,7 @@ int oit_clear(oit_ctx* c, void* stream) {
   c->segments.clear();
   c->frame_open = e == hipSuccess;
   c->resolved = false;
+  mark_work(c, s);
   return status_of(e);
 }
 
@@ -607,6 +723,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       break;
     }
   }
+  mark_work(c, s);
   if (e != hipSuccess) return status_of(e);
   c->frame_frags = end;
   c->appends++;
@@ -672,6 +789,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
       break;
     }
   }
+  mark_work(c, s);
   return status_of(e);
 }
 

@@ -21,7 +21,7 @@
 
 #include "oit.h"
 #include "oit_device.hpp"
-#include "oit_scene.hpp"
+#include "oit_scene_gen.hpp"
 
 namespace oit {
 namespace {
@@ -31,16 +31,8 @@ constexpr int kScanThreads = 256;
 constexpr int kScanItems = 4;
 constexpr int kScanTile = kScanThreads * kScanItems;
 constexpr uint32_t kMaxSpans = 0x7FFFFFFFu;
+constexpr uint32_t kMaxSpheres = 0x7FFFFFFFu;
 
-struct SpanArgs {
-  const SphereDesc* spheres;
-  const uint32_t* span_sphere;  // span -> sphere index
-  const uint32_t* sphere_span;  // sphere -> index of its first span (row y0)
-  uint32_t spans;
-  uint32_t width, height;
-  float fw, fh;
-  ScenePositions pos;
-};
 
 // One wave per span. kEmit=false writes the span's fragment count; kEmit=true
 // writes its fragments at span_base[span].
@@ -53,7 +45,14 @@ __global__ __launch_bounds__(64 * kWavesPerBlock) void k_span_walk(SpanArgs a, u
   const uint32_t span = __builtin_amdgcn_readfirstlane(blockIdx.x * kWavesPerBlock + threadIdx.x / 64);
   if (span >= a.spans) return;
   const uint32_t lane = lane_id();
-  const uint32_t sid = __builtin_amdgcn_readfirstlane(a.span_sphere[span]);
+  // the sphere of this span: the last one whose first span is <= span
+  uint32_t lo = 0, hi = a.nspheres;
+  while (hi - lo > 1u) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (a.sphere_span[mid] <= span) lo = mid;
+    else hi = mid;
+  }
+  const uint32_t sid = __builtin_amdgcn_readfirstlane(lo);
   const SphereDesc d = a.spheres[sid];
   const uint32_t y = (uint32_t)d.y0 + (span - a.sphere_span[sid]);
   const float fy = (float)y;
@@ -202,85 +201,147 @@ int st(hipError_t e) {
   return OIT_ERR_DEVICE;
 }
 
-}  // namespace
-}  // namespace oit
-
-struct oit_scene_gen {
-  int device = 0;
-  uint32_t samples = 1;
-  oit::SpanArgs args{};
-  oit::SphereDesc* d_spheres = nullptr;
-  uint32_t* d_span_sphere = nullptr;
-  uint32_t* d_sphere_span = nullptr;
-  uint32_t* d_span_frags = nullptr;
-  uint64_t* d_span_base = nullptr;  // spans + 1 entries; the last is the total
-  uint64_t* d_tiles = nullptr;
-  uint64_t fragments = 0;
-};
-
-namespace {
-
-void release(oit_scene_gen* g) {
-  (void)hipFree(g->d_spheres);
-  (void)hipFree(g->d_span_sphere);
-  (void)hipFree(g->d_sphere_span);
-  (void)hipFree(g->d_span_frags);
-  (void)hipFree(g->d_span_base);
-  (void)hipFree(g->d_tiles);
-  delete g;
+template <typename T>
+int grow_device(T** p, uint64_t* cap, uint64_t need) {
+  if (need <= *cap && *p) return OIT_OK;
+  if (*p) {  // earlier scenes' launches may still read it
+    (void)hipDeviceSynchronize();
+    (void)hipFree(*p);
+  }
+  *p = nullptr;
+  *cap = 0;
+  const int rc = st(hipMalloc((void**)p, need * sizeof(T)));
+  if (!rc) *cap = need;
+  return rc;
 }
 
-int build(oit_scene_gen* g, const oit_scene_params* p) {
-  using namespace oit;
-  std::vector<SphereDesc> spheres;
-  build_spheres(p, spheres);
-  std::vector<uint32_t> sphere_span(spheres.size()), span_sphere;
-  uint64_t spans = 0;
-  for (size_t i = 0; i < spheres.size(); ++i) {
-    sphere_span[i] = (uint32_t)spans;
-    const int64_t rows = (int64_t)spheres[i].y1 - spheres[i].y0 + 1;
-    if (rows <= 0) continue;
-    if (spans + (uint64_t)rows > kMaxSpans) return OIT_ERR_INVALID_ARG;
-    span_sphere.insert(span_sphere.end(), (size_t)rows, (uint32_t)i);
-    spans += (uint64_t)rows;
+template <typename T>
+int grow_host(T** p, uint64_t* cap, uint64_t need) {
+  if (need <= *cap && *p) return OIT_OK;
+  if (*p) {
+    (void)hipDeviceSynchronize();
+    (void)hipHostFree(*p);
   }
-  g->samples = p->samples;
-  SpanArgs& a = g->args;
+  *p = nullptr;
+  *cap = 0;
+  const int rc = st(hipHostMalloc((void**)p, need * sizeof(T), hipHostMallocDefault));
+  if (!rc) *cap = need;
+  return rc;
+}
+
+}  // namespace
+
+SceneGen::SceneGen(int device) : device_(device) {}
+
+SceneGen::~SceneGen() {
+  (void)hipFree(d_spheres_);
+  (void)hipFree(d_sphere_span_);
+  (void)hipFree(d_span_frags_);
+  (void)hipFree(d_span_base_);
+  (void)hipFree(d_tiles_);
+  if (h_spheres_) (void)hipHostFree(h_spheres_);
+  if (h_sphere_span_) (void)hipHostFree(h_sphere_span_);
+  if (h_total_) (void)hipHostFree(h_total_);
+  if (ev_staged_) (void)hipEventDestroy(ev_staged_);
+  if (ev_counted_) (void)hipEventDestroy(ev_counted_);
+}
+
+int SceneGen::begin(const oit_scene_params* p, hipStream_t s) {
+  if (!scene_params_valid(p)) return OIT_ERR_INVALID_ARG;
+  int rc;
+  if (!ev_staged_ && (rc = st(hipEventCreateWithFlags(&ev_staged_, hipEventDisableTiming)))) return rc;
+  if (!ev_counted_ && (rc = st(hipEventCreateWithFlags(&ev_counted_, hipEventDisableTiming)))) return rc;
+  if (staged_pending_) {  // the previous scene's copies have read the staging buffers
+    if ((rc = st(hipEventSynchronize(ev_staged_)))) return rc;
+    staged_pending_ = false;
+  }
+  spheres_.clear();
+  build_spheres(p, spheres_);
+  if (spheres_.size() > kMaxSpheres) return OIT_ERR_INVALID_ARG;
+  const uint64_t ns = spheres_.size();
+  if ((rc = grow_host(&h_spheres_, &cap_h_spheres_, ns ? ns : 1))) return rc;
+  if ((rc = grow_host(&h_sphere_span_, &cap_h_sphere_span_, ns ? ns : 1))) return rc;
+  if ((rc = grow_host(&h_total_, &cap_h_total_, 1))) return rc;
+  uint64_t spans = 0;
+  for (uint64_t i = 0; i < ns; ++i) {
+    h_sphere_span_[i] = (uint32_t)spans;
+    h_spheres_[i] = spheres_[i];
+    const int64_t rows = (int64_t)spheres_[i].y1 - spheres_[i].y0 + 1;
+    if (rows > 0) spans += (uint64_t)rows;
+    if (spans > kMaxSpans) return OIT_ERR_INVALID_ARG;
+  }
+  samples_ = p->samples;
+  SpanArgs& a = args_;
+  a.nspheres = (uint32_t)ns;
   a.spans = (uint32_t)spans;
   a.width = p->width;
   a.height = p->height;
   a.fw = (float)p->width;
   a.fh = (float)p->height;
   a.pos = scene_positions(p->samples);
+  fragments_ = 0;
+  counting_ = false;
   if (spans == 0) return OIT_OK;
-  const uint32_t tiles = (uint32_t)((spans + kScanTile - 1) / kScanTile);
-  int rc;
-  if ((rc = st(hipMalloc((void**)&g->d_spheres, spheres.size() * sizeof(SphereDesc))))) return rc;
-  if ((rc = st(hipMalloc((void**)&g->d_span_sphere, spans * sizeof(uint32_t))))) return rc;
-  if ((rc = st(hipMalloc((void**)&g->d_sphere_span, spheres.size() * sizeof(uint32_t))))) return rc;
-  if ((rc = st(hipMalloc((void**)&g->d_span_frags, spans * sizeof(uint32_t))))) return rc;
-  if ((rc = st(hipMalloc((void**)&g->d_span_base, (spans + 1) * sizeof(uint64_t))))) return rc;
-  if ((rc = st(hipMalloc((void**)&g->d_tiles, tiles * sizeof(uint64_t))))) return rc;
-  if ((rc = st(hipMemcpy(g->d_spheres, spheres.data(), spheres.size() * sizeof(SphereDesc), hipMemcpyHostToDevice))))
+  const uint64_t tiles = (spans + kScanTile - 1) / kScanTile;
+  if ((rc = grow_device(&d_spheres_, &cap_spheres_, ns))) return rc;
+  if ((rc = grow_device(&d_sphere_span_, &cap_sphere_span_, ns))) return rc;
+  if ((rc = grow_device(&d_span_frags_, &cap_spans_, spans))) return rc;
+  if ((rc = grow_device(&d_span_base_, &cap_span_base_, spans + 1))) return rc;
+  if ((rc = grow_device(&d_tiles_, &cap_tiles_, tiles))) return rc;
+  if ((rc = st(hipMemcpyAsync(d_spheres_, h_spheres_, ns * sizeof(SphereDesc), hipMemcpyHostToDevice, s)))) return rc;
+  if ((rc = st(hipMemcpyAsync(d_sphere_span_, h_sphere_span_, ns * sizeof(uint32_t), hipMemcpyHostToDevice, s))))
     return rc;
-  if ((rc = st(hipMemcpy(g->d_span_sphere, span_sphere.data(), spans * sizeof(uint32_t), hipMemcpyHostToDevice))))
-    return rc;
-  if ((rc = st(hipMemcpy(g->d_sphere_span, sphere_span.data(), sphere_span.size() * sizeof(uint32_t),
-                         hipMemcpyHostToDevice))))
-    return rc;
-  a.spheres = g->d_spheres;
-  a.span_sphere = g->d_span_sphere;
-  a.sphere_span = g->d_sphere_span;
-  hipStream_t s = nullptr;
-  if ((rc = st(launch_walk<false>(a, g->samples, g->d_span_frags, nullptr, nullptr, nullptr, nullptr, s)))) return rc;
-  k_scan_tiles<<<tiles, kScanThreads, 0, s>>>(g->d_span_frags, a.spans, g->d_tiles);
-  k_scan_tile_sums<<<1, kScanThreads, 0, s>>>(g->d_tiles, tiles, g->d_span_base + spans);
-  k_scan_apply<<<tiles, kScanThreads, 0, s>>>(g->d_span_frags, a.spans, g->d_tiles, g->d_span_base);
+  if ((rc = st(hipEventRecord(ev_staged_, s)))) return rc;
+  staged_pending_ = true;
+  a.spheres = d_spheres_;
+  a.sphere_span = d_sphere_span_;
+  if ((rc = st(launch_walk<false>(a, samples_, d_span_frags_, nullptr, nullptr, nullptr, nullptr, s)))) return rc;
+  k_scan_tiles<<<(uint32_t)tiles, kScanThreads, 0, s>>>(d_span_frags_, a.spans, d_tiles_);
+  k_scan_tile_sums<<<1, kScanThreads, 0, s>>>(d_tiles_, (uint32_t)tiles, d_span_base_ + spans);
+  k_scan_apply<<<(uint32_t)tiles, kScanThreads, 0, s>>>(d_span_frags_, a.spans, d_tiles_, d_span_base_);
   if ((rc = st(hipGetLastError()))) return rc;
-  return st(hipMemcpy(&g->fragments, g->d_span_base + spans, sizeof(uint64_t), hipMemcpyDeviceToHost));
+  if ((rc = st(hipMemcpyAsync(h_total_, d_span_base_ + spans, sizeof(uint64_t), hipMemcpyDeviceToHost, s))))
+    return rc;
+  if ((rc = st(hipEventRecord(ev_counted_, s)))) return rc;
+  counting_ = true;
+  return OIT_OK;
 }
 
-}  // namespace
+bool SceneGen::count_ready() {
+  if (!counting_) return true;
+  const hipError_t e = hipEventQuery(ev_counted_);
+  if (e == hipErrorNotReady) return false;
+  (void)hipGetLastError();
+  fragments_ = *h_total_;
+  counting_ = false;
+  return true;
+}
+
+int SceneGen::wait_count() {
+  if (!counting_) return OIT_OK;
+  const int rc = st(hipEventSynchronize(ev_counted_));
+  if (rc) return rc;
+  fragments_ = *h_total_;
+  counting_ = false;
+  return OIT_OK;
+}
+
+int SceneGen::emit(uint32_t* target, float* depth, float* rgba, uint64_t capacity, hipStream_t s) const {
+  if (counting_) return OIT_ERR_STATE;
+  if (fragments_ == 0) return OIT_OK;
+  if (!target || !depth || !rgba) return OIT_ERR_INVALID_ARG;
+  if (((uintptr_t)target & 7) || ((uintptr_t)depth & 7) || ((uintptr_t)rgba & 15)) return OIT_ERR_INVALID_ARG;
+  if (capacity < fragments_) return OIT_ERR_CAPACITY;
+  return st(launch_walk<true>(args_, samples_, nullptr, d_span_base_, target, depth, rgba, s));
+}
+
+}  // namespace oit
+
+struct oit_scene_gen {
+  explicit oit_scene_gen(int dev) : device(dev), gen(dev) {}
+  int device;
+  oit::SceneGen gen;
+};
 
 extern "C" {
 
@@ -294,12 +355,12 @@ int oit_scene_gen_create(oit_scene_gen** out, const oit_scene_params* p, int dev
     return OIT_ERR_DEVICE;
   }
   if (hipSetDevice(device) != hipSuccess) return OIT_ERR_DEVICE;
-  oit_scene_gen* g = new (std::nothrow) oit_scene_gen();
+  oit_scene_gen* g = new (std::nothrow) oit_scene_gen(device);
   if (!g) return OIT_ERR_OOM;
-  g->device = device;
-  const int rc = build(g, p);
+  int rc = g->gen.begin(p, nullptr);
+  if (!rc) rc = g->gen.wait_count();
   if (rc) {
-    release(g);
+    delete g;
     return rc;
   }
   *out = g;
@@ -309,26 +370,22 @@ int oit_scene_gen_create(oit_scene_gen** out, const oit_scene_params* p, int dev
 int oit_scene_gen_destroy(oit_scene_gen* g) {
   if (!g) return OIT_ERR_INVALID_ARG;
   if (hipSetDevice(g->device) != hipSuccess) return OIT_ERR_DEVICE;
-  release(g);
+  (void)hipDeviceSynchronize();
+  delete g;
   return OIT_OK;
 }
 
 int oit_scene_gen_count(const oit_scene_gen* g, uint64_t* out_count) {
   if (!g || !out_count) return OIT_ERR_INVALID_ARG;
-  *out_count = g->fragments;
+  *out_count = g->gen.fragments();
   return OIT_OK;
 }
 
 int oit_scene_gen_emit(oit_scene_gen* g, uint32_t* target, float* depth, float* rgba, uint64_t capacity,
                        void* stream) {
   if (!g) return OIT_ERR_INVALID_ARG;
-  if (g->fragments == 0) return OIT_OK;
-  if (!target || !depth || !rgba) return OIT_ERR_INVALID_ARG;
-  if (((uintptr_t)target & 7) || ((uintptr_t)depth & 7) || ((uintptr_t)rgba & 15)) return OIT_ERR_INVALID_ARG;
-  if (capacity < g->fragments) return OIT_ERR_CAPACITY;
   if (hipSetDevice(g->device) != hipSuccess) return OIT_ERR_DEVICE;
-  return oit::st(oit::launch_walk<true>(g->args, g->samples, nullptr, g->d_span_base, target, depth, rgba,
-                                        static_cast<hipStream_t>(stream)));
+  return g->gen.emit(target, depth, rgba, capacity, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -1,0 +1,511 @@
+// oit_server.cpp - request loop over the OIT path (oit_server_*).
+//
+// Requests (a seeded scene, an algorithm, K) arrive at any time from any
+// thread. A dispatcher thread owns `slots` HIP streams; each slot keeps a
+// scene generator and one context per algorithm whose device buffers stay
+// resident and are reused by later requests. Per request a slot:
+//   1. stages the scene and counts its fragments on the device (async),
+//   2. sizes the A-buffer from that exact count (Linked List pool = F + 1
+//      nodes, Loop64/k-buffer tail arrays = F) and asks the HBM budget for
+//      it: granted in arrival order, idle contexts are trimmed under
+//      pressure, a request larger than the whole budget is refused,
+//   3. generates the fragments into the slot's stream buffers and runs
+//      clear, append and resolve, then records its completion event.
+// Slots run concurrently (continuous batching: a request that arrives while
+// others run is admitted as soon as a slot and memory are free, and its
+// kernels overlap theirs on the device). Latency is arrival to the
+// dispatcher seeing the completion event, on the host steady clock.
+//
+// No counterpart in the reference (a single interactive frame loop,
+// Sample::onRender, oitRender.cpp:28-154); the per-request work is the
+// reference's clear/draw pair through the same C ABI as a single frame.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <thread>
+#include <vector>
+
+#include "oit.h"
+#include "oit_scene_gen.hpp"
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+constexpr int kAlgs = 7;
+constexpr uint32_t kMaxSlots = 16;
+
+int st(hipError_t e) {
+  if (e == hipSuccess) return OIT_OK;
+  (void)hipGetLastError();
+  if (e == hipErrorOutOfMemory) return OIT_ERR_OOM;
+  if (e == hipErrorInvalidValue) return OIT_ERR_INVALID_ARG;
+  return OIT_ERR_DEVICE;
+}
+
+bool valid_layers(uint32_t k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16 || k == 32; }
+
+bool supported(uint32_t alg) {
+  return alg == OIT_SIMPLE || alg == OIT_LINKEDLIST || alg == OIT_LOOP64 || alg == OIT_SPINLOCK ||
+         alg == OIT_WEIGHTED;
+}
+
+// Device bytes a context holds for one request (the sizes oit_configure
+// allocates; tail arrays 28 B per fragment, tail heads 4 B per target).
+uint64_t request_bytes(const oit_request& r, uint64_t F) {
+  const uint64_t T = (uint64_t)r.scene.width * r.scene.height * r.scene.samples;
+  const uint64_t K = r.algorithm == OIT_WEIGHTED ? 1 : r.layers;
+  const bool tail = r.tail_blend != 0;
+  switch (r.algorithm) {
+    case OIT_LINKEDLIST: return 4 * T + 12 * (F + 1) + (tail ? 4 * T : 0);
+    case OIT_LOOP64: return 8 * K * T + (tail ? 4 * T + 28 * F : 0);
+    case OIT_SIMPLE: return 4 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
+    case OIT_SPINLOCK: return 12 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
+    default: return 20 * T;
+  }
+}
+
+double micros(Clock::time_point t, Clock::time_point t0) {
+  return std::chrono::duration<double, std::micro>(t - t0).count();
+}
+
+template <typename T>
+int grow(T** p, uint64_t* cap, uint64_t need) {
+  if (need <= *cap && *p) return OIT_OK;
+  if (*p) {  // earlier requests' launches may still read it
+    (void)hipDeviceSynchronize();
+    (void)hipFree(*p);
+  }
+  *p = nullptr;
+  *cap = 0;
+  const int rc = st(hipMalloc((void**)p, (need ? need : 1) * sizeof(T)));
+  if (!rc) *cap = need;
+  return rc;
+}
+
+struct Pending {
+  oit_request req;
+  Clock::time_point t_submit;
+};
+
+enum class SlotState { kIdle, kCounting, kSizing, kRunning };
+
+struct Slot {
+  explicit Slot(int device) : gen(device) {}
+  uint32_t index = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t done = nullptr;
+  oit::SceneGen gen;
+  oit_ctx* ctx[kAlgs] = {};
+  uint64_t resident[kAlgs] = {};
+  uint32_t* target = nullptr;
+  float* depth = nullptr;
+  float4* rgba = nullptr;
+  uint64_t cap_target = 0, cap_depth = 0, cap_rgba = 0;
+  float4* out_own = nullptr;
+  uint64_t cap_out = 0;
+  SlotState state = SlotState::kIdle;
+  Pending job{};
+  uint64_t seq = 0;  // admission order (memory is granted in this order)
+  uint64_t fragments = 0;
+  uint64_t budgeted = 0;
+  const float* out = nullptr;
+  Clock::time_point t_start;
+};
+
+}  // namespace
+
+struct oit_server {
+  int device = 0;
+  oit_server_config cfg{};
+  Clock::time_point t0;
+  std::vector<std::unique_ptr<Slot>> slots;
+  // shared with submitters
+  std::mutex mu;
+  std::condition_variable cv;       // work arrived / stop
+  std::condition_variable idle_cv;  // everything finished
+  std::deque<Pending> queue;
+  std::vector<oit_completion> done;
+  bool stop = false;
+  uint64_t submitted = 0, completed = 0, failed = 0;
+  uint64_t held = 0, peak_held = 0;
+  uint32_t in_flight = 0, peak_in_flight = 0;
+  bool busy = false;  // the dispatcher has work in a slot
+  // dispatcher only
+  uint64_t next_seq = 0;
+  std::thread thread;
+};
+
+namespace {
+
+uint64_t held_bytes(const oit_server* s) {
+  uint64_t h = 0;
+  for (const auto& sl : s->slots)
+    for (int a = 0; a < kAlgs; ++a) h += sl->resident[a];
+  return h;
+}
+
+void publish_held(oit_server* s) {
+  const uint64_t h = held_bytes(s);
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->held = h;
+  s->peak_held = std::max(s->peak_held, h);
+}
+
+void finish(oit_server* s, Slot& sl, int status) {
+  oit_completion c;
+  std::memset(&c, 0, sizeof c);
+  c.tag = sl.job.req.tag;
+  c.status = status;
+  c.slot = sl.index;
+  c.fragments = sl.fragments;
+  c.abuffer_bytes = sl.budgeted;
+  c.out = status == OIT_OK ? sl.out : nullptr;
+  const auto now = Clock::now();
+  c.t_submit_us = micros(sl.job.t_submit, s->t0);
+  c.t_start_us = micros(sl.t_start, s->t0);
+  c.t_done_us = micros(now, s->t0);
+  sl.state = SlotState::kIdle;
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->done.push_back(c);
+  s->completed++;
+  if (status != OIT_OK) s->failed++;
+  s->in_flight--;
+}
+
+// Frees idle contexts (largest first) until `need` more bytes fit the budget.
+bool make_room(oit_server* s, Slot& self, int self_alg, uint64_t need) {
+  uint64_t held = held_bytes(s);
+  if (held + need <= s->cfg.budget_bytes) return true;
+  struct Cand {
+    Slot* sl;
+    int alg;
+    uint64_t bytes;
+  };
+  std::vector<Cand> c;
+  for (auto& sp : s->slots) {
+    Slot& sl = *sp;
+    for (int a = 0; a < kAlgs; ++a) {
+      if (!sl.ctx[a] || !sl.resident[a]) continue;
+      if (&sl == &self && a == self_alg) continue;
+      if (sl.state == SlotState::kRunning && a == (int)sl.job.req.algorithm) continue;
+      c.push_back({&sl, a, sl.resident[a]});
+    }
+  }
+  std::sort(c.begin(), c.end(), [](const Cand& x, const Cand& y) { return x.bytes > y.bytes; });
+  for (const Cand& x : c) {
+    if (held + need <= s->cfg.budget_bytes) break;
+    if (oit_trim(x.sl->ctx[x.alg]) != OIT_OK) continue;
+    held -= x.sl->resident[x.alg];
+    x.sl->resident[x.alg] = 0;
+  }
+  return held + need <= s->cfg.budget_bytes;
+}
+
+// Enqueues generation and the OIT passes of the slot's request.
+int launch(oit_server* s, Slot& sl) {
+  const oit_request& r = sl.job.req;
+  const uint32_t alg = r.algorithm;
+  const uint64_t F = sl.fragments;
+  const uint64_t T = (uint64_t)r.scene.width * r.scene.height * r.scene.samples;
+  int rc;
+  if ((rc = grow(&sl.target, &sl.cap_target, F))) return rc;
+  if ((rc = grow(&sl.depth, &sl.cap_depth, F))) return rc;
+  if ((rc = grow(&sl.rgba, &sl.cap_rgba, F))) return rc;
+  float* out = r.out;
+  if (!out) {
+    if ((rc = grow(&sl.out_own, &sl.cap_out, T))) return rc;
+    out = reinterpret_cast<float*>(sl.out_own);
+  }
+  sl.out = out;
+  if (!sl.ctx[alg] && (rc = oit_create(&sl.ctx[alg], s->device))) return rc;
+  oit_config c;
+  std::memset(&c, 0, sizeof c);
+  c.algorithm = alg;
+  c.layers = alg == OIT_WEIGHTED ? 1 : r.layers;
+  c.tail_blend = r.tail_blend;
+  c.width = r.scene.width;
+  c.height = r.scene.height;
+  c.samples = r.scene.samples;
+  c.ll_pool_nodes = alg == OIT_LINKEDLIST ? F + 1 : 0;
+  c.max_fragments = (r.tail_blend && alg != OIT_LINKEDLIST && alg != OIT_WEIGHTED) ? (F ? F : 1) : 0;
+  c.background[0] = c.background[1] = c.background[2] = 0.5f;
+  c.background[3] = 1.0f;
+  c.wboit_view_near = 2.0f;
+  c.wboit_view_span = 8.0f;
+  if ((rc = oit_configure(sl.ctx[alg], &c))) return rc;
+  oit_sizes z;
+  if (oit_query_sizes(sl.ctx[alg], &z) == OIT_OK) sl.resident[alg] = z.resident_bytes;
+  publish_held(s);
+  if ((rc = sl.gen.emit(sl.target, sl.depth, reinterpret_cast<float*>(sl.rgba), sl.cap_target, sl.stream))) return rc;
+  oit_fragments f{sl.target, sl.depth, reinterpret_cast<const float*>(sl.rgba), F};
+  if ((rc = oit_clear(sl.ctx[alg], sl.stream))) return rc;
+  if ((rc = oit_append(sl.ctx[alg], &f, sl.stream))) return rc;
+  if ((rc = oit_resolve(sl.ctx[alg], out, sl.stream))) return rc;
+  return st(hipEventRecord(sl.done, sl.stream));
+}
+
+// One pass over the slots and the queue; true if anything moved.
+bool step(oit_server* s) {
+  bool moved = false;
+  // completions and counts
+  for (auto& sp : s->slots) {
+    Slot& sl = *sp;
+    if (sl.state == SlotState::kRunning) {
+      const hipError_t e = hipEventQuery(sl.done);
+      if (e == hipSuccess) {
+        finish(s, sl, OIT_OK);
+        moved = true;
+      } else if (e != hipErrorNotReady) {
+        (void)hipGetLastError();
+        finish(s, sl, OIT_ERR_DEVICE);
+        moved = true;
+      }
+    } else if (sl.state == SlotState::kCounting && sl.gen.count_ready()) {
+      sl.fragments = sl.gen.fragments();
+      sl.budgeted = request_bytes(sl.job.req, sl.fragments);
+      sl.state = SlotState::kSizing;
+      moved = true;
+    }
+  }
+  // memory grants, oldest admission first; a later request waits behind an
+  // earlier one that does not fit yet
+  std::vector<Slot*> sizing;
+  for (auto& sp : s->slots)
+    if (sp->state == SlotState::kSizing) sizing.push_back(sp.get());
+  std::sort(sizing.begin(), sizing.end(), [](const Slot* a, const Slot* b) { return a->seq < b->seq; });
+  for (Slot* slp : sizing) {
+    Slot& sl = *slp;
+    const int alg = (int)sl.job.req.algorithm;
+    if (sl.budgeted > s->cfg.budget_bytes) {
+      finish(s, sl, OIT_ERR_OVER_BUDGET);
+      moved = true;
+      continue;
+    }
+    const uint64_t extra = sl.budgeted > sl.resident[alg] ? sl.budgeted - sl.resident[alg] : 0;
+    if (!make_room(s, sl, alg, extra)) break;
+    sl.t_start = Clock::now();
+    sl.state = SlotState::kRunning;
+    const int rc = launch(s, sl);
+    if (rc) finish(s, sl, rc);
+    moved = true;
+  }
+  // admissions
+  for (auto& sp : s->slots) {
+    Slot& sl = *sp;
+    if (sl.state != SlotState::kIdle) continue;
+    Pending p;
+    {
+      std::lock_guard<std::mutex> lk(s->mu);
+      if (s->queue.empty()) break;
+      p = s->queue.front();
+      s->queue.pop_front();
+      s->in_flight++;
+      s->peak_in_flight = std::max(s->peak_in_flight, s->in_flight);
+    }
+    sl.job = p;
+    sl.seq = s->next_seq++;
+    sl.fragments = 0;
+    sl.budgeted = 0;
+    sl.out = nullptr;
+    sl.t_start = Clock::now();
+    sl.state = SlotState::kCounting;
+    const int rc = sl.gen.begin(&p.req.scene, sl.stream);
+    if (rc) finish(s, sl, rc);
+    moved = true;
+  }
+  return moved;
+}
+
+void dispatcher(oit_server* s) {
+  (void)hipSetDevice(s->device);
+  for (;;) {
+    const bool moved = step(s);
+    bool active = false;
+    for (auto& sp : s->slots) active |= sp->state != SlotState::kIdle;
+    std::unique_lock<std::mutex> lk(s->mu);
+    s->busy = active || !s->queue.empty();
+    if (!s->busy) {
+      s->idle_cv.notify_all();
+      if (s->stop) return;
+      s->cv.wait_for(lk, std::chrono::milliseconds(2));
+      continue;
+    }
+    lk.unlock();
+    if (!moved) std::this_thread::sleep_for(std::chrono::microseconds(20));
+  }
+}
+
+void release(oit_server* s) {
+  for (auto& sp : s->slots) {
+    Slot& sl = *sp;
+    for (int a = 0; a < kAlgs; ++a)
+      if (sl.ctx[a]) (void)oit_destroy(sl.ctx[a]);
+    (void)hipFree(sl.target);
+    (void)hipFree(sl.depth);
+    (void)hipFree(sl.rgba);
+    (void)hipFree(sl.out_own);
+    if (sl.done) (void)hipEventDestroy(sl.done);
+    if (sl.stream) (void)hipStreamDestroy(sl.stream);
+  }
+  s->slots.clear();
+}
+
+int check_request(const oit_request* r) {
+  if (!r) return OIT_ERR_INVALID_ARG;
+  if (r->algorithm == OIT_LOOP || r->algorithm == OIT_INTERLOCK) return OIT_ERR_UNSUPPORTED;
+  if (!supported(r->algorithm)) return OIT_ERR_INVALID_ARG;
+  if (r->algorithm != OIT_WEIGHTED && !valid_layers(r->layers)) return OIT_ERR_INVALID_ARG;
+  if (r->tail_blend > 1u || r->reserved != 0u) return OIT_ERR_INVALID_ARG;
+  if (!oit::scene_params_valid(&r->scene)) return OIT_ERR_INVALID_ARG;
+  if (r->out && ((uintptr_t)r->out & 15u)) return OIT_ERR_INVALID_ARG;
+  return OIT_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int oit_server_create(oit_server** out, int device, const oit_server_config* cfg) {
+  if (!out || !cfg) return OIT_ERR_INVALID_ARG;
+  *out = nullptr;
+  if (cfg->slots == 0 || cfg->slots > kMaxSlots || cfg->budget_bytes == 0) return OIT_ERR_INVALID_ARG;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
+    (void)hipGetLastError();
+    return OIT_ERR_DEVICE;
+  }
+  if (hipSetDevice(device) != hipSuccess) return OIT_ERR_DEVICE;
+  oit_server* s = new (std::nothrow) oit_server();
+  if (!s) return OIT_ERR_OOM;
+  s->device = device;
+  s->cfg = *cfg;
+  s->t0 = Clock::now();
+  int rc = OIT_OK;
+  for (uint32_t i = 0; i < cfg->slots && !rc; ++i) {
+    auto sl = std::make_unique<Slot>(device);
+    sl->index = i;
+    rc = st(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
+    if (!rc) rc = st(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+    s->slots.push_back(std::move(sl));
+  }
+  if (rc) {
+    release(s);
+    delete s;
+    return rc;
+  }
+  s->thread = std::thread(dispatcher, s);
+  *out = s;
+  return OIT_OK;
+}
+
+int oit_server_destroy(oit_server* s) {
+  if (!s) return OIT_ERR_INVALID_ARG;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->stop = true;
+  }
+  s->cv.notify_all();
+  if (s->thread.joinable()) s->thread.join();
+  (void)hipSetDevice(s->device);
+  (void)hipDeviceSynchronize();
+  release(s);
+  delete s;
+  return OIT_OK;
+}
+
+int oit_server_submit(oit_server* s, const oit_request* r) {
+  if (!s) return OIT_ERR_INVALID_ARG;
+  const int rc = check_request(r);
+  if (rc) return rc;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return OIT_ERR_STATE;
+    s->queue.push_back(Pending{*r, Clock::now()});
+    s->submitted++;
+    s->busy = true;
+  }
+  s->cv.notify_all();
+  return OIT_OK;
+}
+
+int oit_server_poll(oit_server* s, oit_completion* out, uint32_t max, uint32_t* n) {
+  if (!s || !n || (max && !out)) return OIT_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  const uint32_t k = (uint32_t)std::min<size_t>(max, s->done.size());
+  for (uint32_t i = 0; i < k; ++i) out[i] = s->done[i];
+  s->done.erase(s->done.begin(), s->done.begin() + k);
+  *n = k;
+  return OIT_OK;
+}
+
+int oit_server_drain(oit_server* s, uint32_t timeout_ms) {
+  if (!s) return OIT_ERR_INVALID_ARG;
+  std::unique_lock<std::mutex> lk(s->mu);
+  const bool ok = s->idle_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !s->busy; });
+  return ok ? OIT_OK : OIT_ERR_NOT_READY;
+}
+
+int oit_server_replay(oit_server* s, const oit_request* reqs, const double* arrival_us, uint32_t n,
+                      uint32_t timeout_ms, oit_completion* out) {
+  if (!s || (n && (!reqs || !arrival_us || !out))) return OIT_ERR_INVALID_ARG;
+  for (uint32_t i = 0; i < n; ++i) {
+    const int rc = check_request(&reqs[i]);
+    if (rc) return rc;
+    if (i && arrival_us[i] < arrival_us[i - 1]) return OIT_ERR_INVALID_ARG;
+  }
+  const auto start = Clock::now();
+  for (uint32_t i = 0; i < n; ++i) {
+    const auto due = start + std::chrono::duration_cast<Clock::duration>(
+                                 std::chrono::duration<double, std::micro>(arrival_us[i]));
+    for (;;) {
+      const auto now = Clock::now();
+      if (now >= due) break;
+      if (due - now > std::chrono::microseconds(300))
+        std::this_thread::sleep_for(due - now - std::chrono::microseconds(200));
+    }
+    oit_request r = reqs[i];
+    r.tag = i;
+    const int rc = oit_server_submit(s, &r);
+    if (rc) return rc;
+  }
+  int rc = oit_server_drain(s, timeout_ms);
+  if (rc) return rc;
+  std::vector<bool> seen(n, false);
+  std::lock_guard<std::mutex> lk(s->mu);
+  for (auto it = s->done.begin(); it != s->done.end();) {
+    if (it->tag < n && !seen[it->tag]) {
+      out[it->tag] = *it;
+      out[it->tag].tag = reqs[it->tag].tag;
+      seen[it->tag] = true;
+      it = s->done.erase(it);
+    } else {
+      ++it;
+    }
+  }
+  for (uint32_t i = 0; i < n; ++i)
+    if (!seen[i]) return OIT_ERR_STATE;
+  return OIT_OK;
+}
+
+int oit_server_get_stats(oit_server* s, oit_server_stats* out) {
+  if (!s || !out) return OIT_ERR_INVALID_ARG;
+  std::lock_guard<std::mutex> lk(s->mu);
+  std::memset(out, 0, sizeof *out);
+  out->submitted = s->submitted;
+  out->completed = s->completed;
+  out->failed = s->failed;
+  out->held_bytes = s->held;
+  out->peak_held_bytes = s->peak_held;
+  out->peak_in_flight = s->peak_in_flight;
+  out->slots = s->cfg.slots;
+  return OIT_OK;
+}
+
+}  // extern "C"

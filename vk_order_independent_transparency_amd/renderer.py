@@ -174,6 +174,11 @@ class Renderer:
         _abi.check("oit_query_sizes", self._lib.oit_query_sizes(self._ctx, ctypes.byref(out)))
         return out
 
+    def trim(self) -> None:
+        """Frees the context's frame buffers (oit_trim); configure() again before use."""
+        _abi.check("oit_trim", self._lib.oit_trim(self._ctx))
+        self.config = None
+
     def clear(self, stream=None) -> None:
         _abi.check("oit_clear", self._lib.oit_clear(self._ctx, _stream_ptr(stream)))
 
