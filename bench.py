@@ -54,6 +54,125 @@ def cpu_baseline(tg, d, c, K, cap, budget_s):
     }
 
 
+def bench_batch(seed, frames_n):
+    """The 64-frame batched workload: every frame one at a time (clear, append,
+    resolve per frame, one context each) against the batched Linked List
+    (one append launch, one resolve launch per layer group on forked streams,
+    replayed from a graph, variants tuned). Device time from HIP events; the
+    batched images are checked bit for bit against the one-at-a-time ones."""
+    import numpy as np
+
+    from vk_order_independent_transparency_amd import (Batch, BatchFrame, DeviceBuffer, Event, FrameConfig,
+                                                       OIT_LINKEDLIST, Renderer, Stream, scene)
+
+    work = scene.batch_workload(seed, frames_n)
+    st = Stream()
+    frames, total = [], 0
+    for fr in work:
+        g = scene.DeviceScene(fr.scene)
+        t, d, c = g.buffers()
+        g.emit(t, d, c, st)
+        out = DeviceBuffer((fr.scene.targets, 4), "float32")
+        frames.append(BatchFrame(width=fr.scene.width, height=fr.scene.height, layers=fr.layers, target=t, depth=d,
+                                 rgba=c, count=g.fragments, out=out))
+        total += g.fragments
+        g.close()
+    st.synchronize()
+    log(f"batch workload: {len(frames)} frames, {total} fragments")
+    rends, seq_out = [], []
+    for f in frames:
+        r = Renderer(0)
+        r.configure(FrameConfig(algorithm=OIT_LINKEDLIST, layers=f.layers, tail_blend=True, width=f.width,
+                                height=f.height, ll_pool_nodes=f.count + 1))
+        rends.append(r)
+        seq_out.append(DeviceBuffer((f.width * f.height, 4), "float32"))
+    ev = [(Event(), Event()) for _ in frames]
+
+    def seq_pass():
+        for i, (r, f) in enumerate(zip(rends, frames)):
+            ev[i][0].record(st)
+            r.render(f.target, f.depth, f.rgba, seq_out[i], st, count=f.count)
+            ev[i][1].record(st)
+
+    seq_pass()
+    st.synchronize()
+    seq = []
+    for _ in range(3):
+        seq_pass()
+        st.synchronize()
+        seq.append(sum(a.elapsed_ms(b) for a, b in ev))
+    seq_ms = min(seq)
+    want = [o.numpy() for o in seq_out]
+    for r in rends:
+        r.close()
+    del seq_out
+    b = Batch(0, concurrent=True, graph=True)
+    b.run(frames, st)
+    st.synchronize()
+    b.tune(frames, st)
+    b.run(frames, st)
+    st.synchronize()
+    e0, e1 = Event(), Event()
+    runs = []
+    for _ in range(5):
+        e0.record(st)
+        b.run(frames, st)
+        e1.record(st)
+        e1.synchronize()
+        runs.append(e0.elapsed_ms(e1))
+    bat_ms = min(runs)
+    exact = all(np.array_equal(f.out.numpy().view(np.uint32), w.view(np.uint32)) for f, w in zip(frames, want))
+    t = b.tuning
+    b.close()
+    log(f"batch: sequential {seq_ms:.3f} ms, batched {bat_ms:.3f} ms, {seq_ms / bat_ms:.2f}x, bit-exact {exact}")
+    return {"workload": f"batch_{len(frames)}_frames_seed{seed}", "frames": len(frames), "fragments": total,
+            "sequential_ms": round(seq_ms, 4), "batched_ms": round(bat_ms, 4),
+            "speedup": round(seq_ms / bat_ms, 3), "G_fragments_per_s": round(total / bat_ms / 1e6, 3),
+            "bit_exact_vs_sequential": bool(exact), "append_chunks": int(t.append_chunks)}
+
+
+def bench_serve(seed, n, rates, slots):
+    """Served workload at each arrival rate: p50/p99 request latency (arrival
+    to result on the device) and G fragments/s over the busy span."""
+    from vk_order_independent_transparency_amd import serve
+
+    out = []
+    srv = serve.Server(0, slots=slots)
+    # warm pass: every request of the trace once, all at t = 0, so the
+    # contexts and stream buffers reach their resident sizes
+    warm, _ = serve.served_workload(seed, n, rates[0])
+    srv.replay(warm, [0.0] * len(warm))
+    for rate in rates:
+        reqs, arr = serve.served_workload(seed, n, rate)
+        done = srv.replay(reqs, arr)
+        stt = srv.stats()
+        ok = [c for c in done if c.status == 0]
+        frags = sum(c.fragments for c in ok)
+        span_us = max(c.t_done_us for c in ok) - min(c.t_submit_us for c in ok)
+        lat = serve.latency_summary(done)
+        offered_s = float(arr[-1]) / 1e6
+        row = {"rate_per_s": rate, "requests": n, "completed": len(ok), "failed": len(done) - len(ok),
+               "offered_s": round(offered_s, 4), "span_s": round(span_us / 1e6, 4),
+               "saturated": bool(span_us / 1e6 > 1.1 * offered_s + 0.05),
+               "p50_ms": round(lat["p50_ms"], 3), "p99_ms": round(lat["p99_ms"], 3),
+               "G_fragments_per_s": round(frags / span_us / 1e3, 3),
+               "peak_held_GB": round(stt.peak_held_bytes / 2 ** 30, 2), "peak_in_flight": int(stt.peak_in_flight)}
+        log(json.dumps(row))
+        out.append(row)
+    srv.close()
+    return out
+
+
+def _under_load(served):
+    """p50/p99 at the highest arrival rate the GPU keeps up with (else the lowest rate)."""
+    if not served:
+        return None
+    kept = [r for r in served if not r["saturated"]]
+    row = kept[-1] if kept else served[0]
+    return {"p50": row["p50_ms"], "p99": row["p99_ms"],
+            "under_load": f"served workload at {row['rate_per_s']:g} requests/s"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -63,6 +182,12 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="OIT_VARIANT_* bits (kernel schedule; results identical)")
+    ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched workload")
+    ap.add_argument("--batch-frames", type=int, default=64)
+    ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
+    ap.add_argument("--serve-requests", type=int, default=150)
+    ap.add_argument("--serve-rates", type=str, default="100,300,1000", help="arrival rates, requests/s")
+    ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MB",
                     help="also run one device-to-device copy of MB megabytes (a known-traffic kernel for counter scaling)")
     args = ap.parse_args()
@@ -143,6 +268,12 @@ def main():
     st = r.stats(stream)
     r.close()
 
+    batch = None if args.no_batch else bench_batch(1, args.batch_frames)
+    served = None
+    if not args.no_serve:
+        served = bench_serve(1, args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
+                             args.serve_slots)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(tg, d, c, K, cap, args.cpu_budget)
@@ -185,8 +316,11 @@ def main():
         },
         "frame_roofline": {"bytes": rl.ll_frame_bytes_reference(F, P), "GBps": round(frame_gbs, 1),
                            "frac": round(frame_frac, 4)},
-        "latency_ms": {"p50": round(per_frame[len(per_frame) // 2], 4),
-                       "p99": round(per_frame[min(len(per_frame) - 1, int(0.99 * len(per_frame)))], 4)},
+        "frame_latency_ms": {"p50": round(per_frame[len(per_frame) // 2], 4),
+                             "p99": round(per_frame[min(len(per_frame) - 1, int(0.99 * len(per_frame)))], 4)},
+        "latency_ms": _under_load(served),
+        "batch": batch,
+        "serve": served,
         "stages": stages,
         "ll_overflow": int(st.ll_overflow),
         "cpu_baseline": cpu,
