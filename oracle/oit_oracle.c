@@ -222,10 +222,37 @@ static int ev_cmp(const void* a, const void* b) {
 }
 
 /* Blends the tail events over the colour buffer in ascending position:
- * the ROP applies PREMULTIPLIED in primitive order (main.cpp:548-558). */
-static void fold_tail(tail_list* L, float* colorbuf) {
+ * the ROP applies PREMULTIPLIED in primitive order (main.cpp:548-558).
+ * Sequential order: one ROP blend per event, oldest first. Canonical order
+ * (fragment ids): the same composite (over is associative), associated
+ * newest first: acc = e_n over e_(n-1) over ... over e_1, then acc over the
+ * background. The reference blends into an 8-bit target, so the float
+ * association is not fixed by it; this one lets the kernels fold a
+ * newest-first list in one walk. */
+static int fold_tail(tail_list* L, float* colorbuf, uint64_t T, int canonical) {
   qsort(L->ev, L->n, sizeof(tail_ev), ev_cmp); /* positions are unique per target in canonical order */
-  for (uint64_t i = 0; i < L->n; ++i) rop_blend(&colorbuf[4 * (uint64_t)L->ev[i].target], L->ev[i].c);
+  if (!canonical) {
+    for (uint64_t i = 0; i < L->n; ++i) rop_blend(&colorbuf[4 * (uint64_t)L->ev[i].target], L->ev[i].c);
+    return 0;
+  }
+  float* acc = (float*)calloc(T * 4, sizeof(float));
+  unsigned char* seen = (unsigned char*)calloc(T, 1);
+  if (!acc || !seen) { free(acc); free(seen); return -2; }
+  for (uint64_t i = L->n; i-- > 0;) {
+    const uint64_t t = L->ev[i].target;
+    over_into(&acc[4 * t], L->ev[i].c);
+    seen[t] = 1;
+  }
+  for (uint64_t t = 0; t < T; ++t) {
+    if (!seen[t]) continue;
+    float o[4];
+    memcpy(o, &acc[4 * t], sizeof o);
+    over_into(o, &colorbuf[4 * t]);
+    memcpy(&colorbuf[4 * t], o, sizeof o);
+  }
+  free(acc);
+  free(seen);
+  return 0;
 }
 
 /* ------------------------------------------------------------------ */
@@ -289,7 +316,7 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
       node[n].key = orc_depth_key(depth[i]);
       node[n].id = (uint32_t)i;
     }
-    fold_tail(&tails, colorbuf);
+    if ((rc = fold_tail(&tails, colorbuf, T, canonical))) goto ll_free;
     /* composite, :106-172 */
     elem* a = (elem*)malloc(K * sizeof(elem));
     if (!a) { rc = -2; goto ll_free; }
@@ -380,7 +407,7 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
         res->tail_fragments++;
       }
     }
-    fold_tail(&tails, colorbuf);
+    if ((rc = fold_tail(&tails, colorbuf, T, canonical))) goto kb_free;
     {
       elem* a = (elem*)malloc(K * sizeof(elem));
       if (!a) { rc = -2; goto kb_free; }
@@ -437,7 +464,7 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
         res->tail_fragments++;
       }
     }
-    fold_tail(&tails, colorbuf);
+    if ((rc = fold_tail(&tails, colorbuf, T, canonical))) { free(slot); free(sid); goto done; }
     elem* a = (elem*)malloc(K * sizeof(elem));
     if (!a) { free(slot); free(sid); rc = -2; goto done; }
     for (uint64_t t = 0; t < T; ++t) {
@@ -612,14 +639,18 @@ int orc_replay_kbuffer(const orc_config* cfg, uint64_t F, const uint32_t* target
   if (!colorbuf || !a) { free(colorbuf); free(a); return -2; }
   for (uint64_t t = 0; t < T; ++t) memcpy(&colorbuf[4 * t], cfg->background, 4 * sizeof(float));
   if (cfg->tail_blend && tail_flag) {
+    tail_list tails = {0, 0, 0};
     for (uint64_t i = 0; i < F; ++i) {
       if (!tail_flag[i]) continue;
-      if (target[i] >= T) { free(colorbuf); free(a); return -1; }
+      if (target[i] >= T) { free(tails.ev); free(colorbuf); free(a); return -1; }
       float c4[4];
       if (tail_quantized) orc_decode_premult(orc_encode(&rgba[4 * i]), c4);
       else memcpy(c4, &rgba[4 * i], sizeof c4);
-      rop_blend(&colorbuf[4 * (uint64_t)target[i]], c4); /* ascending id = primitive order */
+      if (tail_push(&tails, (uint32_t)i, target[i], c4)) { free(tails.ev); free(colorbuf); free(a); return -2; }
     }
+    const int frc = fold_tail(&tails, colorbuf, T, 1); /* id order, canonical association */
+    free(tails.ev);
+    if (frc) { free(colorbuf); free(a); return frc; }
   }
   for (uint64_t t = 0; t < T; ++t) {
     int n = (int)(count[t] < K ? count[t] : K);

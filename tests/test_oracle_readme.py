@@ -48,7 +48,8 @@ def test_simple_readme_example(oracle):
     assert list(r["frontk_val"][0]) == [1, 0]
     assert list(r["tail_flag"]) == [0, 0, 1, 1]
     assert r["count"][0] == 4
-    colorbuf = over(premult(C[3]), over(premult(C[1]), BG))  # c1 first, c3 on top (primitive order)
+    # c1 first, c3 on top (primitive order); the canonical fold associates newest first
+    colorbuf = over(over(premult(C[3]), premult(C[1])), BG)
     comp = over(q(oracle, C[2]), q(oracle, C[4]))
     assert np.array_equal(r["colorbuf"][0], colorbuf)
     assert np.array_equal(r["composite"][0], comp)
@@ -96,11 +97,12 @@ def test_loop64_readme_example(oracle):
     assert list(r["frontk_val"][0]) == [oracle.encode(premult(C[1])), oracle.encode(premult(C[2]))]
     assert list(r["tail_flag"]) == [1, 0, 0, 1]
     # canonical tail: evicted fragments at their own draw position, packed colour
-    assert np.array_equal(r["colorbuf"][0], over(q(oracle, C[3]), over(q(oracle, C[4]), BG)))
+    assert np.array_equal(r["colorbuf"][0], over(over(q(oracle, C[3]), q(oracle, C[4])), BG))
     # sequential reading of the shader: c4 is evicted by c1 (position 2), c3 rejects itself
-    # (position 3): the same order here
+    # (position 3): the same order, one ROP blend per event
     rs = oracle.run(oracle.make_config(oracle.LOOP64, 2, 1, 1, 1, tail_order=oracle.TAIL_SEQUENTIAL), tg, d, c)
-    assert np.array_equal(rs["colorbuf"][0], r["colorbuf"][0])
+    assert np.array_equal(rs["colorbuf"][0], over(q(oracle, C[3]), over(q(oracle, C[4]), BG)))
+    assert np.allclose(rs["colorbuf"][0], r["colorbuf"][0], rtol=0, atol=2 ** -22)
 
 
 def test_spinlock_readme_trace(oracle):
@@ -117,7 +119,7 @@ def test_spinlock_readme_trace(oracle):
     rs = oracle.run(oracle.make_config(oracle.SPINLOCK, 2, 1, 1, 1, tail_order=oracle.TAIL_SEQUENTIAL), tg, d, c)
     assert np.array_equal(rs["colorbuf"][0], over(q(oracle, C[3]), over(premult(C[4]), BG)))
     # canonical: both at their own ids (c3 id 0, c4 id 2), packed colours
-    assert np.array_equal(r["colorbuf"][0], over(q(oracle, C[4]), over(q(oracle, C[3]), BG)))
+    assert np.array_equal(r["colorbuf"][0], over(over(q(oracle, C[4]), q(oracle, C[3])), BG))
 
 
 @pytest.mark.parametrize("alg", [0, 1, 3, 4])

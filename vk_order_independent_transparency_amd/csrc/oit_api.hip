@@ -306,7 +306,7 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
     c->tl_head_valid = false;
   }
   if (!c->tl_head_valid) {
-    hipError_t e = oit::launch_fill_u32(c->tl_head, c->T, kEmpty, s);
+    hipError_t e = oit::launch_fill_u32(c->tl_head, c->T, 0u, s);  // empty tail lists
     if (e != hipSuccess) return status_of(e);
     c->tl_head_valid = true;
   }
@@ -334,6 +334,9 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
     (void)hipMemcpyAsync(nrec, c->tl_rec, (uint64_t)c->tl_cap * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
     (void)hipStreamSynchronize(s);
   }
+  // records this frame has not written read as "not in the tail"
+  const hipError_t fe = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(nrec + c->tl_cap),
+                                             2 * (cap - c->tl_cap), kEmpty, s);
   dev_free(&c->tl_next);
   dev_free(&c->tl_rgba);
   dev_free(&c->tl_rec);
@@ -341,7 +344,7 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
   c->tl_rgba = nr;
   c->tl_rec = nrec;
   c->tl_cap = (uint32_t)cap;
-  return OIT_OK;
+  return fe == hipSuccess ? OIT_OK : status_of(fe);
 }
 
 }  // namespace
@@ -637,7 +640,7 @@ int oit_clear(oit_ctx* c, void* stream) {
         if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_reveal), T, 0x3F800000u, s);
         break;
     }
-    if (r == hipSuccess && need_tail) r = oit::launch_fill_u32(c->tl_head, T, kEmpty, s);
+    if (r == hipSuccess && need_tail) r = oit::launch_fill_u32(c->tl_head, T, 0u, s);  // empty tail lists
     return r;
   });
   if (e == hipSuccess && ll) {
@@ -707,7 +710,11 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
     }
     case OIT_SPINLOCK: {
       oit::KBuf kb{c->kb_count, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
-      e = staged(c, kStStore, s, [&] { return oit::launch_spin_store(k, v, kb, tb, s); });
+      // a fragment's record is written by whichever fragment pushes it to the
+      // tail, so this append's records start empty
+      if (tb.head && tb.rec)
+        e = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(tb.rec + c->frame_frags), 2 * f->count, kEmpty, s);
+      if (e == hipSuccess) e = staged(c, kStStore, s, [&] { return oit::launch_spin_store(k, v, kb, tb, s); });
       break;
     }
     case OIT_LOOP64: {
@@ -754,7 +761,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     }
   }
   if (link_tails && tb.head && tb.next) {
-    e = staged(c, kStTailLink, s, [&] { return oit::launch_tail_link(tb, s); });
+    e = staged(c, kStTailLink, s, [&] { return oit::launch_tail_link(tb, (uint32_t)c->frame_frags, s); });
     if (e != hipSuccess) return status_of(e);
   }
   c->resolved = true;

@@ -170,13 +170,13 @@ __device__ __forceinline__ uint32_t min_relaxed(uint32_t* p, uint32_t v) {
 // Id-ordered lock-free lists
 //
 // A list is a chain of link words: the head word per target, then next[x]
-// for each element x. Element values are ordered (descending for the
-// Linked List, whose walk order is newest first as in the reference's
-// head-exchange lists, oitLinkedList.frag.glsl:72; ascending for the
-// colour-pass tail lists, which blend oldest first). Insertion of a block
-// of consecutive values [lo, hi] (a run of same-target lanes):
-//   old = atomic_max/min(link, hi/lo)  -- publishes the block
-//   if old is beyond the block: next[last of block] = old  (one store)
+// for each element x. Element values are ordered descending: newest first,
+// the walk order of the reference's head-exchange lists
+// (oitLinkedList.frag.glsl:72); the colour-pass tail lists use the same
+// order and fold newest first. Insertion of a block of values [lo, hi] (a
+// run of same-target lanes):
+//   old = atomic_max(link, hi)  -- publishes the block
+//   if old < lo: next[lo] = old  (one store)
 //   else: walk to the successor of `old` and retry there.
 // next[] words of elements that are about to be inserted hold kUnset, written
 // by an EARLIER kernel; a walker that reaches a published element whose link
@@ -189,15 +189,21 @@ __device__ __forceinline__ uint32_t min_relaxed(uint32_t* p, uint32_t v) {
 // behind its own wait.
 // --------------------------------------------------------------------------
 
-// Descending list (head holds the largest value, 0 terminates).
-// Inserts block [lo, hi] (hi >= lo, consecutive values already chained
-// hi -> hi-1 -> ... -> lo by the caller) at link word `p0` when `want`.
+// Descending list (head holds the largest value, 0 terminates). A value v
+// keeps its link word at next[(v >> kShift) - kBias]: the Linked List's
+// values are node indices (kShift 0, kBias 0); the colour-pass tail lists'
+// values are ((id + 1) << 1) | boundary flag (kShift 1, kBias 1). `cap`
+// bounds the values whose link words exist.
+// Inserts block [lo, hi] (hi >= lo; its values already chained hi -> ... ->
+// lo by the caller; no value of the list outside the block lies between lo
+// and hi) at link word `p0` when `want`.
 // Called by every lane of the wave (lanes without a block pass want=false).
 // The loop condition is a wave-wide ballot, so the compiler cannot turn a
 // lane's wait for a kUnset link into an inner loop that the owner of that
 // link (possibly a lane of the same wave) would never leave: every trip is
 // one step for every active lane, and the owner's link store happens in the
 // trip of its successful atomic.
+template <int kShift = 0, uint32_t kBias = 0>
 __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32_t* next, uint32_t lo,
                                                  uint32_t hi, uint32_t cap) {
   bool active = want && hi < cap;
@@ -211,7 +217,7 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
         const uint32_t v = ld_relaxed(p);
         if (v != kUnset) {
           if (v > hi) {
-            if (v < cap) p = &next[v]; else active = false;  // a link outside the pool: stop
+            if (v < cap) p = &next[(v >> kShift) - kBias]; else active = false;  // a link outside the pool: stop
           } else {
             try_max = true;
           }
@@ -220,10 +226,10 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
       if (try_max) {
         const uint32_t old = max_relaxed(p, hi);
         if (old < lo) {
-          st_relaxed(&next[lo], old);
+          st_relaxed(&next[(lo >> kShift) - kBias], old);
           active = false;
         } else if (old < cap) {
-          p = &next[old];  // a larger element got there first: continue behind it
+          p = &next[(old >> kShift) - kBias];  // a larger element got there first: continue behind it
         } else {
           active = false;
         }
@@ -235,6 +241,7 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
 // list_insert_desc for a lane that already tried the head and learned that a
 // larger element is there: it starts at that element's link word `p0` and
 // reads it before its first atomic.
+template <int kShift = 0, uint32_t kBias = 0>
 __device__ __forceinline__ void list_insert_desc_from(bool want, uint32_t* p0, uint32_t* next, uint32_t lo,
                                                       uint32_t hi, uint32_t cap) {
   bool active = want && hi < cap;
@@ -245,7 +252,7 @@ __device__ __forceinline__ void list_insert_desc_from(bool want, uint32_t* p0, u
       const uint32_t v = ld_relaxed(p);
       if (v != kUnset) {
         if (v > hi) {
-          if (v < cap) p = &next[v]; else active = false;
+          if (v < cap) p = &next[(v >> kShift) - kBias]; else active = false;
         } else {
           try_max = true;
         }
@@ -253,47 +260,10 @@ __device__ __forceinline__ void list_insert_desc_from(bool want, uint32_t* p0, u
       if (try_max) {
         const uint32_t old = max_relaxed(p, hi);
         if (old < lo) {
-          st_relaxed(&next[lo], old);
+          st_relaxed(&next[(lo >> kShift) - kBias], old);
           active = false;
         } else if (old < cap) {
-          p = &next[old];
-        } else {
-          active = false;
-        }
-      }
-    }
-  }
-}
-
-// Ascending list (head holds the smallest value, kEmpty terminates).
-// Inserts the single value x at link word p0 when `want`; same trip
-// structure as list_insert_desc.
-__device__ __forceinline__ void list_insert_asc(bool want, uint32_t* p0, uint32_t* next, uint32_t x,
-                                                uint32_t next_index, uint32_t cap) {
-  bool active = want && next_index < cap;
-  bool first = true;
-  uint32_t* p = p0;
-  while (__ballot(active)) {
-    if (active) {
-      bool try_min = first;
-      first = false;
-      if (!try_min) {
-        const uint32_t v = ld_relaxed(p);
-        if (v != kUnset) {
-          if (v < x) {
-            if ((v >> 1) < cap) p = &next[v >> 1]; else active = false;  // a link outside the arrays: stop
-          } else {
-            try_min = true;
-          }
-        }
-      }
-      if (try_min) {
-        const uint32_t old = min_relaxed(p, x);
-        if (old > x) {
-          st_relaxed(&next[next_index], old);
-          active = false;
-        } else if ((old >> 1) < cap) {
-          p = &next[old >> 1];
+          p = &next[(old >> kShift) - kBias];
         } else {
           active = false;
         }

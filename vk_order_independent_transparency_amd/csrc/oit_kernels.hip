@@ -43,17 +43,40 @@ __device__ __forceinline__ Run same_target_run(uint32_t t) {
   return r;
 }
 
-// Records a colour-pass tail event for frame-local fragment `lid` (phase 1 of
-// the tail lists): payload colour, kUnset link, and a compact record whose
-// slot comes from one atomicAdd per wave. Callable from divergent code.
-__device__ __forceinline__ void tail_record(bool want, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
-                                            const TailBuf& tb) {
-  const uint32_t m = wave_alloc(tb.counter, want);
-  if (want && m < tb.cap && lid < tb.cap) {
+// Colour-pass tail list value of frame-local fragment lid: ((lid + 1) << 1)
+// | boundary flag, so that 0 can terminate the (descending) lists; its link
+// word is next[lid].
+__device__ __forceinline__ uint32_t tail_value(uint32_t lid, uint32_t flag) { return ((lid + 1u) << 1) | flag; }
+__device__ __forceinline__ uint32_t tail_index(uint32_t v) { return (v >> 1) - 1u; }
+// values below this have link words in arrays of `cap` fragments
+__device__ __forceinline__ uint32_t tail_value_cap(uint32_t cap) { return (cap + 1u) << 1; }
+
+// Colour-pass tail lists, phase 1: the record of frame-local fragment `lid`
+// is rec[lid] = (target << 32) | tail value, or kEmpty64 when the fragment
+// is not in the tail; a tail fragment also stores its premultiplied colour
+// and a kUnset link word. Records are indexed by fragment id, so no counter
+// is shared between waves.
+// tail_record_own: the lane's own fragment, tail or not (writes the record
+// either way).
+__device__ __forceinline__ void tail_record_own(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
+                                                const TailBuf& tb) {
+  if (lid >= tb.cap) return;
+  tb.rec[lid] = tail ? ((uint64_t)t << 32) | tail_value(lid, flag) : kEmpty64;
+  if (tail) {
     tb.rgba[lid] = payload;
     tb.next[lid] = kUnset;
-    tb.rec[m] = ((uint64_t)t << 32) | ((lid << 1) | flag);
   }
+}
+
+// tail_record_evict: fragment `lid` (the lane's own or one it pushed out of
+// the A-buffer) goes to the tail; records of the range were filled with
+// kEmpty64 before the kernel.
+__device__ __forceinline__ void tail_record_evict(bool tail, uint32_t t, uint32_t lid, uint32_t flag,
+                                                  float4 payload, const TailBuf& tb) {
+  if (!tail || lid >= tb.cap) return;
+  tb.rec[lid] = ((uint64_t)t << 32) | tail_value(lid, flag);
+  tb.rgba[lid] = payload;
+  tb.next[lid] = kUnset;
 }
 
 // Sorted register array of K (sort key, colour) pairs, ascending; inserting x
@@ -161,24 +184,33 @@ __device__ __forceinline__ float4 blend_front(const FrontK<K>& a, uint32_t n, co
   return c;
 }
 
-// Colour buffer after the colour pass: the tail fragments of target t blended
-// over the background in ascending id (the ROP's primitive order,
-// main.cpp:548-558). `skip` boundary-flagged entries are in the A-buffer and
-// not blended (Loop64 identical keys, see k_loop64_classify).
+// Colour buffer after the colour pass: the tail fragments of target t over
+// the background in id order (the ROP's primitive order, main.cpp:548-558),
+// folded newest first along the descending list: acc = t_n over ... over
+// t_1, then acc over the background. `skip` boundary-flagged entries are in
+// the A-buffer and not blended (Loop64 identical keys, see
+// k_loop64_classify): the ones with the lowest ids, i.e. the last flagged
+// entries of the walk.
 __device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float4 bg, uint32_t skip) {
-  float4 d = bg;
-  uint32_t link = tb.head[t];
-  while (link != kEmpty && (link >> 1) < tb.cap) {  // a link outside the arrays ends the walk
-    const uint32_t id = link >> 1;
-    const uint32_t nx = tb.next[id];
-    if ((link & 1u) && skip > 0u) {
-      --skip;
-    } else {
-      d = over(tb.rgba[id], d);
-    }
-    link = nx;
+  const uint32_t vcap = tail_value_cap(tb.cap);
+  uint32_t keep = 0xFFFFFFFFu;  // flagged entries to blend
+  if (skip > 0u) {
+    uint32_t flagged = 0;
+    for (uint32_t v = tb.head[t]; v != 0u && v < vcap; v = tb.next[tail_index(v)]) flagged += v & 1u;
+    keep = flagged > skip ? flagged - skip : 0u;
   }
-  return d;
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t v = tb.head[t];
+  while (v != 0u && v < vcap) {  // a link outside the arrays ends the walk
+    const uint32_t id = tail_index(v);
+    const uint32_t nx = tb.next[id];
+    if (!(v & 1u) || keep > 0u) {
+      acc = over(acc, tb.rgba[id]);
+      if (v & 1u) --keep;
+    }
+    v = nx;
+  }
+  return over(acc, bg);
 }
 
 template <int K>
@@ -254,7 +286,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_store(Common c, FragView f, LLBuf
   }
   const uint64_t om = __ballot(ovf);
   if (om && lane_id() == (uint32_t)__builtin_ctzll(om)) atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
-  if constexpr (kTail) tail_record(ovf, t, lid, 0u, col, tb);
+  if constexpr (kTail) tail_record_own(ovf, t, lid, 0u, col, tb);
 }
 
 // Link half: the head exchange of oitLinkedList.frag.glsl:72, done as an
@@ -393,7 +425,7 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
     if (om && lane == (uint32_t)__builtin_ctzll(om)) atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
     if constexpr (kTail) {
       const uint32_t lid = (uint32_t)(f.lid0 + f.first + (uint64_t)u * 64u + lane);
-      tail_record(ovf[u], t[u], lid, 0u, col[u], tb);
+      tail_record_own(ovf[u], t[u], lid, 0u, col[u], tb);
     }
   }
 }
@@ -701,7 +733,7 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
       tail = true;
     }
   }
-  if constexpr (kTail) tail_record(tail, t, lid, 0u, col, tb);
+  if constexpr (kTail) tail_record_own(tail, t, lid, 0u, col, tb);
 }
 
 // ---------------------------------------------------------------------------
@@ -784,7 +816,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   if constexpr (kTail) {
     float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
     if (tail) payload = decode_premult(tail_col, c.tab);
-    tail_record(tail, valid ? t : kEmpty, tail_id, 0u, payload, tb);
+    tail_record_evict(tail, valid ? t : kEmpty, tail_id, 0u, payload, tb);
   }
 }
 
@@ -870,7 +902,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
       payload = decode_premult(col, c.tab);
     }
   }
-  tail_record(tail, t, f.id_base + i, flag, payload, tb);
+  tail_record_own(tail, t, f.id_base + i, flag, payload, tb);
 }
 
 // Composite (oitLoop64.frag.glsl:162-183): blend slots front to back until
@@ -973,23 +1005,40 @@ __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, Re
 
 // ---------------------------------------------------------------------------
 // Tail lists, phase 2: link the frame's records into per-target lists in
-// ascending id (the blend order of the colour-pass tail).
+// descending id (newest first, the order the fold associates in). Records
+// arrive roughly in id order, so most inserts succeed at the head.
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb) {
-  uint32_t nrec = *tb.counter;
-  if (nrec > tb.cap) nrec = tb.cap;
-  const uint32_t stride = gridDim.x * kBlock;
-  // every lane runs the same number of trips so the inserts stay wave-wide
-  const uint32_t base0 = blockIdx.x * kBlock + threadIdx.x;
-  for (uint32_t m0 = blockIdx.x * kBlock; m0 < nrec; m0 += stride) {
-    const uint32_t m = m0 + (base0 - blockIdx.x * kBlock);
-    const bool want = m < nrec;
-    uint64_t rec = 0;
-    if (want) rec = tb.rec[m];
-    const uint32_t t = (uint32_t)(rec >> 32);
-    const uint32_t link = (uint32_t)rec;
-    list_insert_asc(want, want ? &tb.head[t] : nullptr, tb.next, link, link >> 1, tb.cap);
+__global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t frags) {
+  __shared__ uint32_t s_count[kBlock / 64];
+  const uint32_t n = frags < tb.cap ? frags : tb.cap;
+  const uint32_t vcap = tail_value_cap(tb.cap);
+  uint32_t mine = 0;
+  // every lane runs the same number of trips: runs and inserts are wave-wide
+  for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
+    const uint32_t lid = base + threadIdx.x;
+    const uint64_t rec = lid < n ? tb.rec[lid] : kEmpty64;
+    const bool tail = rec != kEmpty64;
+    const uint32_t t = tail ? (uint32_t)(rec >> 32) : kEmpty;
+    const uint32_t v = (uint32_t)rec;
+    mine += tail ? 1u : 0u;
+    // consecutive ids of one target: chained newest to oldest here, then
+    // published with one insert (no other value of the list lies between)
+    const Run r = same_target_run(t);
+    const uint32_t v_prev = __shfl_up(v, 1u);
+    const uint32_t lo = __shfl(v, (int)r.start_lane);
+    if (tail && !r.start) st_relaxed(&tb.next[lid], v_prev);
+    const bool ins = tail && r.end;
+    list_insert_desc<1, 1>(ins, ins ? &tb.head[t] : nullptr, tb.next, lo, v, vcap);
+  }
+  // tail fragments of the frame: one atomic per block
+  for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o);
+  if (lane_id() == 0) s_count[threadIdx.x >> 6] = mine;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t total = 0;
+    for (uint32_t w = 0; w < kBlock / 64; ++w) total += s_count[w];
+    if (total) atomicAdd(tb.counter, total);
   }
 }
 
@@ -1251,8 +1300,11 @@ hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t h
   return hipGetLastError();
 }
 
-hipError_t launch_tail_link(const TailBuf& tb, hipStream_t s) {
-  hipLaunchKernelGGL(k_tail_link, dim3(1024), dim3(kBlock), 0, s, tb);
+hipError_t launch_tail_link(const TailBuf& tb, uint32_t frags, hipStream_t s) {
+  uint32_t blocks = blocks_for(frags < tb.cap ? frags : tb.cap);
+  if (blocks > 4096u) blocks = 4096u;
+  if (blocks == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tail_link, dim3(blocks), dim3(kBlock), 0, s, tb, frags);
   return hipGetLastError();
 }
 

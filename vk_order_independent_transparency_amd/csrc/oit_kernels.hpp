@@ -17,13 +17,13 @@ struct FragView {
   uint32_t id_base;  // frame-local id of the first fragment
 };
 
-// Colour-pass tail lists (per target, ascending fragment id).
+// Colour-pass tail lists (per target, descending fragment id: newest first).
 struct TailBuf {
-  uint32_t* head;     // [T] kEmpty = empty
+  uint32_t* head;     // [T] newest tail value, 0 = empty
   uint32_t* next;     // [Fcap] link words, indexed by frame-local id
   float4* rgba;       // [Fcap] premultiplied colour the tail blend uses
-  uint64_t* rec;      // [Fcap] (target << 32) | (id << 1 | boundary)
-  uint32_t* counter;  // number of records
+  uint64_t* rec;      // [Fcap] (target << 32) | ((id + 1) << 1 | boundary), kEmpty64 = not in the tail
+  uint32_t* counter;  // tail fragments of the frame (counted by the link kernel)
   uint32_t cap;       // Fcap
 };
 
@@ -123,7 +123,7 @@ hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& w
                               hipStream_t s);
 hipError_t launch_wboit_resolve(const Common& c, const WBBuf& wb, const ResolveOut& o, hipStream_t s);
 
-hipError_t launch_tail_link(const TailBuf& tb, hipStream_t s);
+hipError_t launch_tail_link(const TailBuf& tb, uint32_t frags, hipStream_t s);
 
 hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t hi, unsigned long long* mismatches,
                                  uint32_t* first_bad, hipStream_t s);
