@@ -635,9 +635,9 @@ int oit_clear(oit_ctx* c, void* stream) {
       case OIT_LOOP64:  // every byte to 0xFF, oitRender.cpp:307
         if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->l64), 2 * T * K, kEmpty, s);
         break;
-      case OIT_WEIGHTED:  // accum 0, reveal 1, oitRender.cpp:394-397
+      case OIT_WEIGHTED:  // accum 0, reveal 1 (log2 0), oitRender.cpp:394-397
         if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_accum), 4 * T, 0u, s);
-        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_reveal), T, 0x3F800000u, s);
+        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_reveal), T, 0u, s);
         break;
     }
     if (r == hipSuccess && need_tail) r = oit::launch_fill_u32(c->tl_head, T, 0u, s);  // empty tail lists

@@ -958,33 +958,53 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint6
 // reveal *= (1 - a) (ZERO, ONE_MINUS_SRC_COLOR) as a compare-and-swap loop.
 __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WBBuf wb, float vnear, float vspan) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= f.count) return;
-  const uint32_t t = f.target[i];
-  if (t >= c.T) return;
-  const float4 col = f.rgba[i];
-  const float view = vnear + f.depth[i] * vspan;   // stands in for -viewZ (object.vert.glsl:35)
-  const float depthZ = view * 10.0f;               // :65
-  const float x = depthZ / 200.0f;
-  const float x2 = x * x;
-  const float x4 = x2 * x2;                        // pow(depthZ / 200, 4.0)
-  float dw = 0.03f / (1e-5f + x4);                 // :67
-  dw = fminf(fmaxf(dw, 1e-2f), 3e3f);
-  float aw = fminf(1.0f, fmaxf(fmaxf(col.x, col.y), fmaxf(col.z, col.w)) * 40.0f + 0.01f);  // :69
-  aw = aw * aw;                                    // :70
-  const float w = aw * dw;                         // :72
-  float* acc = reinterpret_cast<float*>(&wb.accum[t]);
-  unsafeAtomicAdd(acc + 0, col.x * w);
-  unsafeAtomicAdd(acc + 1, col.y * w);
-  unsafeAtomicAdd(acc + 2, col.z * w);
-  unsafeAtomicAdd(acc + 3, col.w * w);
-  const float om = 1.0f - col.w;
-  uint32_t* rp = reinterpret_cast<uint32_t*>(&wb.reveal[t]);
-  uint32_t old = ld_relaxed(rp);
-  for (;;) {
-    const uint32_t nv = __float_as_uint(__uint_as_float(old) * om);
-    const uint32_t prev = atomicCAS(rp, old, nv);
-    if (prev == old) break;
-    old = prev;
+  uint32_t t = kEmpty;
+  float4 wc = make_float4(0.f, 0.f, 0.f, 0.f);
+  float lr = 0.f;
+  if (i < f.count) {
+    t = f.target[i];
+    if (t >= c.T) t = kEmpty;
+  }
+  if (t != kEmpty) {
+    const float4 col = f.rgba[i];
+    const float view = vnear + f.depth[i] * vspan;   // stands in for -viewZ (object.vert.glsl:35)
+    const float depthZ = view * 10.0f;               // :65
+    const float x = depthZ / 200.0f;
+    const float x2 = x * x;
+    const float x4 = x2 * x2;                        // pow(depthZ / 200, 4.0)
+    float dw = 0.03f / (1e-5f + x4);                 // :67
+    dw = fminf(fmaxf(dw, 1e-2f), 3e3f);
+    float aw = fminf(1.0f, fmaxf(fmaxf(col.x, col.y), fmaxf(col.z, col.w)) * 40.0f + 0.01f);  // :69
+    aw = aw * aw;                                    // :70
+    const float w = aw * dw;                         // :72
+    wc = make_float4(col.x * w, col.y * w, col.z * w, col.w * w);
+    // revealage is a product of (1 - alpha) (the blend's ZERO, ONE_MINUS_SRC_ALPHA
+    // factors, main.cpp:559-577); it is accumulated as a sum of log2 terms so
+    // that every update is an add without a returned value
+    lr = log2f(fmaxf(1.0f - col.w, 0.0f));
+  }
+  // consecutive lanes of one target (a sphere's front and back) are summed
+  // in the wave first: one set of atomics per run
+  const Run r = same_target_run(t);
+#pragma unroll
+  for (uint32_t o = 1; o < 64; o <<= 1) {
+    const float ux = __shfl_up(wc.x, o), uy = __shfl_up(wc.y, o), uz = __shfl_up(wc.z, o), uw = __shfl_up(wc.w, o);
+    const float ul = __shfl_up(lr, o);
+    if (r.lane >= r.start_lane + o) {
+      wc.x += ux;
+      wc.y += uy;
+      wc.z += uz;
+      wc.w += uw;
+      lr += ul;
+    }
+  }
+  if (t != kEmpty && r.end) {
+    float* acc = reinterpret_cast<float*>(&wb.accum[t]);
+    unsafeAtomicAdd(acc + 0, wc.x);
+    unsafeAtomicAdd(acc + 1, wc.y);
+    unsafeAtomicAdd(acc + 2, wc.z);
+    unsafeAtomicAdd(acc + 3, wc.w);
+    unsafeAtomicAdd(&wb.reveal[t], lr);
   }
 }
 
@@ -994,7 +1014,7 @@ __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, Re
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
   const float4 A = wb.accum[t];
-  const float r = wb.reveal[t];
+  const float r = exp2f(wb.reveal[t]);  // product of (1 - alpha), kept as a log2 sum
   const float den = fmaxf(A.w, 1e-5f);
   const float4 s = make_float4(A.x / den, A.y / den, A.z / den, r);
   const float om = 1.0f - r;

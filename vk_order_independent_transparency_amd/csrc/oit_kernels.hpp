@@ -48,7 +48,7 @@ struct KBuf {          // Simple / Spinlock
 
 struct WBBuf {
   float4* accum;      // [T]
-  float* reveal;      // [T]
+  float* reveal;      // [T] log2 of the revealage (sum of log2(1 - alpha)); 0 = fully revealed
 };
 
 struct ResolveOut {
