@@ -23,6 +23,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 
+# HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
+# FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy: FETCH x 2, WRITE x 1) on
+# this workload (profiles/linkedlist_k16_1920x1080_1000spheres.md).
+MEASURED_TRAFFIC = {("linkedlist_k16_1920x1080_1000spheres", "store", 0): 557058150}
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -52,6 +58,47 @@ def cpu_baseline(tg, d, c, K, cap, budget_s):
         "sample": f"{len(times)} full frames of the same workload (1920x1080, 1000 spheres, LL K={K}), "
                   f"median {med * 1e3:.1f} ms/frame, oracle/liboit_oracle.so single thread",
     }
+
+
+def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream):
+    """Every algorithm on the same 1920x1080 stream (Linked List pool 1.5F,
+    tail blending on): device time per frame, G fragments/s and the
+    fraction of the HBM roofline by the algorithm's compulsory bytes."""
+    from vk_order_independent_transparency_amd import DeviceBuffer, Event, FrameConfig, Renderer
+    from vk_order_independent_transparency_amd import roofline as rl
+
+    P = p.targets
+    out = DeviceBuffer((P, 4), "float32")
+    rows = []
+    for alg, name, K in ((1, "linkedlist", 8), (1, "linkedlist", 16), (3, "loop64", 8), (3, "loop64", 16),
+                         (4, "spinlock", 8), (4, "spinlock", 16), (0, "simple", 8), (0, "simple", 16),
+                         (6, "weighted", 1)):
+        with Renderer(0) as r:
+            r.configure(FrameConfig(algorithm=alg, layers=K, tail_blend=alg != 6, width=p.width, height=p.height,
+                                    ll_pool_nodes=F + F // 2 if alg == 1 else 0, max_fragments=F))
+            for _ in range(2):
+                r.render(tg_d, d_d, c_d, out, stream)
+            e0, e1 = Event(), Event()
+            stream.synchronize()
+            e0.record(stream)
+            for _ in range(steps):
+                r.render(tg_d, d_d, c_d, out, stream)
+            e1.record(stream)
+            e1.synchronize()
+            ms = e0.elapsed_ms(e1) / steps
+        if alg == 1:
+            nbytes = rl.ll_frame_bytes_reference(F, P)
+        elif alg == 3:
+            nbytes = sum(rl.loop64_stage_bytes(F, P, K).values())
+        elif alg in (0, 4):
+            nbytes = sum(rl.kbuffer_stage_bytes(F, P, K, alg == 4).values())
+        else:
+            nbytes = sum(rl.wboit_stage_bytes(F, P).values())
+        gbs, fr = rl.frac(nbytes, ms / 1e3)
+        rows.append({"algorithm": name, "layers": K, "ms_per_frame": round(ms, 4),
+                     "G_fragments_per_s": round(F / ms / 1e6, 3), "bytes": nbytes, "frac": round(fr, 4)})
+        log(json.dumps(rows[-1]))
+    return rows
 
 
 def bench_batch(seed, frames_n):
@@ -182,6 +229,7 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="OIT_VARIANT_* bits (kernel schedule; results identical)")
+    ap.add_argument("--no-algorithms", action="store_true", help="skip the other algorithms on the same stream")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched workload")
     ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
@@ -267,6 +315,7 @@ def main():
 
     st = r.stats(stream)
     r.close()
+    algorithms = None if args.no_algorithms else bench_algorithms(tg_d, d_d, c_d, F, p, 10, stream)
 
     batch = None if args.no_batch else bench_batch(1, args.batch_frames)
     served = None
@@ -312,13 +361,14 @@ def main():
             "peak": rl.HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": round(stages[dom]["frac"], 4),
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get((f"linkedlist_k{K}_1920x1080_1000spheres", dom, args.variant)),
         },
         "frame_roofline": {"bytes": rl.ll_frame_bytes_reference(F, P), "GBps": round(frame_gbs, 1),
                            "frac": round(frame_frac, 4)},
         "frame_latency_ms": {"p50": round(per_frame[len(per_frame) // 2], 4),
                              "p99": round(per_frame[min(len(per_frame) - 1, int(0.99 * len(per_frame)))], 4)},
         "latency_ms": _under_load(served),
+        "algorithms": algorithms,
         "batch": batch,
         "serve": served,
         "stages": stages,
