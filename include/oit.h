@@ -85,6 +85,10 @@ typedef struct oit_config {
   float    background[4];      /* premultiplied RGBA the colour pass starts from (opaque gray 0.5 by default) */
   float    wboit_view_near;    /* WBOIT: view distance = near + depth * span ... */
   float    wboit_view_span;    /* ... stands in for -viewZ (object.vert.glsl:35) */
+  uint32_t coverage_samples;   /* OIT_COVERAGE_SHADING (oit.h:82): MSAA samples (2, 4, 8) of per-pixel
+                                  fragments with sample masks; 0 or 1 = off. Needs samples == 1; Simple,
+                                  Linked List and Spinlock (Loop/Loop64 have none, README.md:33-34) */
+  uint32_t reserved;           /* 0 */
 } oit_config;
 
 /* A batch of fragments in draw order (structure of arrays, device memory). */
@@ -93,6 +97,8 @@ typedef struct oit_fragments {
   const float*    depth;  /* gl_FragCoord.z (any finite float; order-preserving key) */
   const float*    rgba;   /* premultiplied linear RGBA, 4 floats per fragment */
   uint64_t        count;
+  const uint32_t* coverage; /* coverage shading: the fragment's sample mask (gl_SampleMaskIn[0],
+                               oitColorDepthDefines.glsl:41-47); NULL otherwise */
 } oit_fragments;
 
 /* Memory footprint, like the reference GUI's "Object Sizes" (oitGui.cpp:327-336). */
@@ -161,6 +167,16 @@ int oit_resolve_debug(oit_ctx* ctx, float* out_rgba, uint32_t* frontk_key,
                       uint32_t* frontk_val, uint32_t* counts, void* stream);
 /* Copies the frame counters to the host (synchronises `stream`). */
 int oit_get_stats(oit_ctx* ctx, oit_stats* out, void* stream);
+
+/* Display image of a resolved frame: the MSAA resolve (average of `samples`
+ * sample planes, main.cpp:700-714) and the supersample downsample (average
+ * of each supersample x supersample block, :715-731) of `in_rgba`
+ * ([samples][height*supersample][width*supersample] premultiplied float4),
+ * then sRGB8 colour (exact curve, round to nearest; the B8G8R8A8_SRGB
+ * target, oit.cpp:58) and linear 8-bit alpha: out_rgba8 is
+ * [height][width] RGBA bytes (device memory). */
+int oit_finalize_srgb8(oit_ctx* ctx, const float* in_rgba, uint32_t width, uint32_t height, uint32_t samples,
+                       uint32_t supersample, uint8_t* out_rgba8, void* stream);
 /* Human-readable name of a status code. */
 const char* oit_status_string(int status);
 

@@ -157,9 +157,28 @@ static void blend_packed(float color[4], uint32_t packed) {
 /* Element helpers: (key, id) total order                              */
 /* ------------------------------------------------------------------ */
 
-typedef struct elem { uint32_t key, id, col; } elem;
+typedef struct elem { uint32_t key, id, col, mask; } elem;
 
 static int elem_lt(elem a, elem b) { return a.key < b.key || (a.key == b.key && a.id < b.id); }
+
+/* Front-to-back blend of the n sorted elements (oitLinkedList.frag.glsl:
+ * 162-166); with coverage shading, per sample over the elements whose mask
+ * holds it, summed in sample order and divided by the sample count
+ * (:147-160, oitSimple.frag.glsl:142-159). */
+static void blend_front(float c[4], const elem* a, int n, uint32_t cov) {
+  if (cov <= 1) {
+    for (int i = 0; i < n; ++i) blend_packed(c, a[i].col);
+    return;
+  }
+  float sum[4] = {0, 0, 0, 0};
+  for (uint32_t s = 0; s < cov; ++s) {
+    float sc[4] = {0, 0, 0, 0};
+    for (int i = 0; i < n; ++i)
+      if ((a[i].mask >> s) & 1u) blend_packed(sc, a[i].col);
+    for (int k = 0; k < 4; ++k) sum[k] = sum[k] + sc[k];
+  }
+  for (int k = 0; k < 4; ++k) c[k] = sum[k] / (float)cov;
+}
 
 /* bubbleSort, oitCompositeDefines.glsl:51-68 (swap on >=). */
 static void bubble_sort(elem* a, int n) {
@@ -199,12 +218,17 @@ static elem insertion_sort_tail(elem* a, int K, elem x) {
 typedef struct tail_ev {
   uint32_t pos;     /* blend position: id of the fragment the colour belongs to (canonical) or of the evictor (sequential) */
   uint32_t target;
+  uint32_t mask;    /* samples the ROP writes (coverage shading); all ones otherwise */
   float    c[4];    /* premultiplied colour blended by the ROP */
 } tail_ev;
 
 typedef struct tail_list { tail_ev* ev; uint64_t n, cap; } tail_list;
 
+static int tail_push_m(tail_list* L, uint32_t pos, uint32_t target, uint32_t mask, const float c[4]);
 static int tail_push(tail_list* L, uint32_t pos, uint32_t target, const float c[4]) {
+  return tail_push_m(L, pos, target, 0xFFFFFFFFu, c);
+}
+static int tail_push_m(tail_list* L, uint32_t pos, uint32_t target, uint32_t mask, const float c[4]) {
   if (L->n == L->cap) {
     uint64_t nc = L->cap ? L->cap * 2 : 1024;
     tail_ev* p = (tail_ev*)realloc(L->ev, nc * sizeof(tail_ev));
@@ -212,7 +236,7 @@ static int tail_push(tail_list* L, uint32_t pos, uint32_t target, const float c[
     L->ev = p; L->cap = nc;
   }
   tail_ev* e = &L->ev[L->n++];
-  e->pos = pos; e->target = target; memcpy(e->c, c, sizeof e->c);
+  e->pos = pos; e->target = target; e->mask = mask; memcpy(e->c, c, sizeof e->c);
   return 0;
 }
 
@@ -229,8 +253,52 @@ static int ev_cmp(const void* a, const void* b) {
  * background. The reference blends into an 8-bit target, so the float
  * association is not fixed by it; this one lets the kernels fold a
  * newest-first list in one walk. */
+static int fold_tail_cov(tail_list* L, float* colorbuf, uint64_t T, int canonical, uint32_t cov);
 static int fold_tail(tail_list* L, float* colorbuf, uint64_t T, int canonical) {
+  return fold_tail_cov(L, colorbuf, T, canonical, 0);
+}
+/* Coverage shading: the ROP blends each event into the samples of its mask;
+ * per sample the same fold, then the colour buffer of a target with tail
+ * events is the sum of its sample values in sample order divided by the
+ * sample count (the MSAA resolve). */
+static int fold_tail_cov(tail_list* L, float* colorbuf, uint64_t T, int canonical, uint32_t cov) {
   qsort(L->ev, L->n, sizeof(tail_ev), ev_cmp); /* positions are unique per target in canonical order */
+  if (cov > 1) {
+    float* acc = (float*)calloc(T * cov * 4, sizeof(float));
+    unsigned char* seen = (unsigned char*)calloc(T, 1);
+    if (!acc || !seen) { free(acc); free(seen); return -2; }
+    if (canonical) {
+      for (uint64_t i = L->n; i-- > 0;) {
+        const uint64_t t = L->ev[i].target;
+        for (uint32_t s = 0; s < cov; ++s)
+          if ((L->ev[i].mask >> s) & 1u) over_into(&acc[(t * cov + s) * 4], L->ev[i].c);
+        seen[t] = 1;
+      }
+      for (uint64_t t = 0; t < T; ++t) {
+        if (!seen[t]) continue;
+        for (uint32_t s = 0; s < cov; ++s) over_into(&acc[(t * cov + s) * 4], &colorbuf[4 * t]);
+      }
+    } else {
+      for (uint64_t t = 0; t < T; ++t)
+        for (uint32_t s = 0; s < cov; ++s) memcpy(&acc[(t * cov + s) * 4], &colorbuf[4 * t], 4 * sizeof(float));
+      for (uint64_t i = 0; i < L->n; ++i) {
+        const uint64_t t = L->ev[i].target;
+        for (uint32_t s = 0; s < cov; ++s)
+          if ((L->ev[i].mask >> s) & 1u) rop_blend(&acc[(t * cov + s) * 4], L->ev[i].c);
+        seen[t] = 1;
+      }
+    }
+    for (uint64_t t = 0; t < T; ++t) {
+      if (!seen[t]) continue;
+      float sum[4] = {0, 0, 0, 0};
+      for (uint32_t s = 0; s < cov; ++s)
+        for (int k = 0; k < 4; ++k) sum[k] = sum[k] + acc[(t * cov + s) * 4 + k];
+      for (int k = 0; k < 4; ++k) colorbuf[4 * t + k] = sum[k] / (float)cov;
+    }
+    free(acc);
+    free(seen);
+    return 0;
+  }
   if (!canonical) {
     for (uint64_t i = 0; i < L->n; ++i) rop_blend(&colorbuf[4 * (uint64_t)L->ev[i].target], L->ev[i].c);
     return 0;
@@ -270,10 +338,23 @@ static void write_frontk(orc_result* res, uint64_t t, uint32_t K, const elem* a,
 
 int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
             const float* rgba, orc_result* res) {
+  return orc_run_cov(cfg, F, target, depth, rgba, NULL, res);
+}
+
+int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
+                const float* rgba, const uint32_t* mask, orc_result* res) {
   init_tables();
   const uint32_t K = cfg->layers;
   const uint64_t T = (uint64_t)cfg->width * cfg->height * cfg->samples;
   if (T == 0 || !res || !res->out) return -1;
+  const uint32_t cov = cfg->coverage_samples > 1 ? cfg->coverage_samples : 0;
+  if (cov) {
+    if (cov != 2 && cov != 4 && cov != 8) return -1;
+    if (cfg->samples != 1 || !mask) return -1;
+    if (cfg->algorithm != ORC_SIMPLE && cfg->algorithm != ORC_LINKEDLIST && cfg->algorithm != ORC_SPINLOCK)
+      return -1; /* Loop and Loop64 have no coverage shading (README.md:33-34) */
+  }
+  const uint32_t full = cov ? (1u << cov) - 1u : 0xFFFFFFFFu;
   if (cfg->algorithm != ORC_WEIGHTED && !valid_layers(K)) return -1;
   for (uint64_t i = 0; i < F; ++i) if (target[i] >= T) return -1;
   if (F >= 0xFFFFFFFFull) return -1;
@@ -300,10 +381,12 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
     for (uint64_t i = 0; i < F; ++i) {
       const uint64_t n = counter + 1; /* :55 newOffset = atomicAdd(counter, 1) + 1 */
       counter++;
+      const uint32_t m = mask ? mask[i] & full : full;
+      if (cov && !m) continue; /* no sample covered: the shader does not run (its node stays unused) */
       if (n >= cap) { /* :59-68 out of memory: tail-blend the shaded colour */
         res->ll_overflow++;
         if (cfg->tail_blend) {
-          if ((rc = tail_push(&tails, (uint32_t)i, target[i], &rgba[4 * i]))) goto ll_free;
+          if ((rc = tail_push_m(&tails, (uint32_t)i, target[i], m, &rgba[4 * i]))) goto ll_free;
           if (res->tail_flag) res->tail_flag[i] = 1;
           res->tail_fragments++;
         }
@@ -315,8 +398,9 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
       node[n].col = orc_encode(&rgba[4 * i]); /* :75-82 */
       node[n].key = orc_depth_key(depth[i]);
       node[n].id = (uint32_t)i;
+      node[n].mask = m; /* :79 storeMask */
     }
-    if ((rc = fold_tail(&tails, colorbuf, T, canonical))) goto ll_free;
+    if ((rc = fold_tail_cov(&tails, colorbuf, T, canonical, cov))) goto ll_free;
     /* composite, :106-172 */
     elem* a = (elem*)malloc(K * sizeof(elem));
     if (!a) { rc = -2; goto ll_free; }
@@ -338,8 +422,8 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
         ++len;
       }
       float c[4] = {0, 0, 0, 0};
-      for (int i = 0; i < frags; ++i) blend_packed(c, a[i].col); /* :162-166 */
-      over_into(c, tailc);                                        /* :170 */
+      blend_front(c, a, frags, cov); /* :147-166 */
+      over_into(c, tailc);           /* :170 */
       if (res->composite) memcpy(&res->composite[4 * t], c, sizeof c);
       if (res->count) res->count[t] = len;
       write_frontk(res, t, K, a, frags, 0);
@@ -360,13 +444,15 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
     for (uint64_t t = 0; t < T; ++t) auxd[t] = EMPTY_U32;
     for (uint64_t i = 0; i < F; ++i) {
       const uint32_t t = target[i];
-      elem x = { orc_depth_key(depth[i]), (uint32_t)i, orc_encode(&rgba[4 * i]) };
+      const uint32_t m = mask ? mask[i] & full : full;
+      if (cov && !m) continue; /* no sample covered: the shader does not run */
+      elem x = { orc_depth_key(depth[i]), (uint32_t)i, orc_encode(&rgba[4 * i]), m };
       if (!spin) {
         /* oitSimple.frag.glsl:74-90 */
         uint32_t slot = count[t]++;
         if (slot < K) { ab[(uint64_t)t * K + slot] = x; continue; }
         if (cfg->tail_blend) {
-          if ((rc = tail_push(&tails, (uint32_t)i, t, &rgba[4 * i]))) goto kb_free;
+          if ((rc = tail_push_m(&tails, (uint32_t)i, t, m, &rgba[4 * i]))) goto kb_free;
           if (res->tail_flag) res->tail_flag[i] = 1;
           res->tail_fragments++;
         }
@@ -383,7 +469,7 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
           ab[(uint64_t)t * K + c] = x;  /* :89-93 */
         } else {
           int furthest = 0;             /* :96-107 */
-          elem maxe = {0u, 0u, 0u};
+          elem maxe = {0u, 0u, 0u, 0u};
           for (uint32_t j = 0; j < K; ++j)
             if (elem_lt(maxe, ab[(uint64_t)t * K + j])) { maxe = ab[(uint64_t)t * K + j]; furthest = (int)j; }
           if (elem_lt(x, maxe)) {       /* :109-117 replace the furthest, tail-blend it */
@@ -402,12 +488,12 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
         if (canonical || !tail_full) orc_decode_premult(tail_e.col, c4);
         else memcpy(c4, &rgba[4 * i], sizeof c4);
         uint32_t pos = canonical ? tail_e.id : (uint32_t)i;
-        if ((rc = tail_push(&tails, pos, t, c4))) goto kb_free;
+        if ((rc = tail_push_m(&tails, pos, t, tail_e.mask, c4))) goto kb_free;
         if (res->tail_flag) res->tail_flag[tail_e.id] = 1;
         res->tail_fragments++;
       }
     }
-    if ((rc = fold_tail(&tails, colorbuf, T, canonical))) goto kb_free;
+    if ((rc = fold_tail_cov(&tails, colorbuf, T, canonical, cov))) goto kb_free;
     {
       elem* a = (elem*)malloc(K * sizeof(elem));
       if (!a) { rc = -2; goto kb_free; }
@@ -417,7 +503,7 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
         for (int i = 0; i < n; ++i) a[i] = ab[(uint64_t)t * K + i];
         bubble_sort(a, n);
         float c[4] = {0, 0, 0, 0};
-        for (int i = 0; i < n; ++i) blend_packed(c, a[i].col);
+        blend_front(c, a, n, cov); /* oitSimple.frag.glsl:142-165 */
         if (res->composite) memcpy(&res->composite[4 * t], c, sizeof c);
         if (res->count) res->count[t] = count[t];
         write_frontk(res, t, K, a, n, 0);
@@ -549,6 +635,7 @@ static int rec_cmp(const void* a, const void* b) {
 int orc_run_fp64(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
                  const float* rgba, int quantize, double* out, double* wb_accum, double* wb_reveal) {
   init_tables();
+  if (cfg->coverage_samples > 1) return -1; /* the exact mirror has no coverage shading */
   const uint64_t T = (uint64_t)cfg->width * cfg->height * cfg->samples;
   if (T == 0 || !out) return -1;
   for (uint64_t i = 0; i < F; ++i) if (target[i] >= T) return -1;
@@ -630,10 +717,21 @@ int orc_run_fp64(const orc_config* cfg, uint64_t F, const uint32_t* target, cons
 int orc_replay_kbuffer(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* rgba,
                        const uint32_t* count, const uint32_t* ab_color, const uint32_t* ab_key,
                        const uint32_t* ab_id, const uint8_t* tail_flag, int tail_quantized, float* out) {
+  return orc_replay_kbuffer_cov(cfg, F, target, rgba, NULL, count, ab_color, ab_key, ab_id, NULL, tail_flag,
+                                tail_quantized, out);
+}
+
+int orc_replay_kbuffer_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* rgba,
+                           const uint32_t* mask, const uint32_t* count, const uint32_t* ab_color,
+                           const uint32_t* ab_key, const uint32_t* ab_id, const uint32_t* ab_mask,
+                           const uint8_t* tail_flag, int tail_quantized, float* out) {
   init_tables();
   const uint32_t K = cfg->layers;
   const uint64_t T = (uint64_t)cfg->width * cfg->height * cfg->samples;
   if (!valid_layers(K) || T == 0) return -1;
+  const uint32_t cov = cfg->coverage_samples > 1 ? cfg->coverage_samples : 0;
+  if (cov && (!mask || !ab_mask)) return -1;
+  const uint32_t full = cov ? (1u << cov) - 1u : 0xFFFFFFFFu;
   float* colorbuf = (float*)malloc(T * 4 * sizeof(float));
   elem* a = (elem*)malloc(K * sizeof(elem));
   if (!colorbuf || !a) { free(colorbuf); free(a); return -2; }
@@ -646,9 +744,10 @@ int orc_replay_kbuffer(const orc_config* cfg, uint64_t F, const uint32_t* target
       float c4[4];
       if (tail_quantized) orc_decode_premult(orc_encode(&rgba[4 * i]), c4);
       else memcpy(c4, &rgba[4 * i], sizeof c4);
-      if (tail_push(&tails, (uint32_t)i, target[i], c4)) { free(tails.ev); free(colorbuf); free(a); return -2; }
+      const uint32_t m = mask ? mask[i] & full : full;
+      if (tail_push_m(&tails, (uint32_t)i, target[i], m, c4)) { free(tails.ev); free(colorbuf); free(a); return -2; }
     }
-    const int frc = fold_tail(&tails, colorbuf, T, 1); /* id order, canonical association */
+    const int frc = fold_tail_cov(&tails, colorbuf, T, 1, cov); /* id order, canonical association */
     free(tails.ev);
     if (frc) { free(colorbuf); free(a); return frc; }
   }
@@ -656,15 +755,40 @@ int orc_replay_kbuffer(const orc_config* cfg, uint64_t F, const uint32_t* target
     int n = (int)(count[t] < K ? count[t] : K);
     for (int i = 0; i < n; ++i) {
       a[i].col = ab_color[t * K + i]; a[i].key = ab_key[t * K + i]; a[i].id = ab_id[t * K + i];
+      a[i].mask = ab_mask ? ab_mask[t * K + i] : full;
     }
     bubble_sort(a, n);
     float c[4] = {0, 0, 0, 0};
-    for (int i = 0; i < n; ++i) blend_packed(c, a[i].col);
+    blend_front(c, a, n, cov);
     float o[4];
     memcpy(o, &colorbuf[4 * t], sizeof o);
     rop_blend(o, c);
     memcpy(&out[4 * t], o, sizeof o);
   }
   free(colorbuf); free(a);
+  return 0;
+}
+
+int orc_finalize_srgb8(const float* in, uint32_t width, uint32_t height, uint32_t samples, uint32_t ss,
+                       uint8_t* out) {
+  init_tables();
+  if (!in || !out || width == 0 || height == 0 || samples == 0 || ss == 0) return -1;
+  const uint64_t W = (uint64_t)width * ss, H = (uint64_t)height * ss;
+  const float n = (float)(samples * ss * ss);
+  for (uint64_t y = 0; y < height; ++y) {
+    for (uint64_t x = 0; x < width; ++x) {
+      float sum[4] = {0, 0, 0, 0};
+      for (uint32_t s = 0; s < samples; ++s)
+        for (uint32_t dy = 0; dy < ss; ++dy)
+          for (uint32_t dx = 0; dx < ss; ++dx) {
+            const float* p = &in[4 * ((s * H + y * ss + dy) * W + x * ss + dx)];
+            for (int k = 0; k < 4; ++k) sum[k] = sum[k] + p[k];
+          }
+      uint8_t* o = &out[4 * (y * width + x)];
+      for (int k = 0; k < 3; ++k) o[k] = (uint8_t)code_rgb(sum[k] / n);
+      const float a = fminf(fmaxf(sum[3] / n, 0.0f), 1.0f);
+      o[3] = (uint8_t)floorf(a * 255.0f + 0.5f);
+    }
+  }
   return 0;
 }

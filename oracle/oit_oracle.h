@@ -51,6 +51,7 @@ typedef struct orc_config {
   uint64_t ll_capacity;  /* LL elements including terminator 0 */
   float background[4];
   float wboit_view_near, wboit_view_span;
+  uint32_t coverage_samples; /* OIT_COVERAGE_SHADING: MSAA samples per pixel (2, 4, 8), 0 or 1 = off */
 } orc_config;
 
 typedef struct orc_result {
@@ -78,6 +79,18 @@ double   orc_srgb_decode_exact(double f);   /* shaderCommon.glsl:84-94 */
 /* Sequential run of one frame. Returns 0 or a negative error. */
 int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
             const float* rgba, orc_result* res);
+/* The same with per-fragment sample masks (gl_SampleMaskIn, required when
+ * cfg->coverage_samples > 1; Simple, Linked List and Spinlock only). */
+int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
+                const float* rgba, const uint32_t* mask, orc_result* res);
+
+/* Display image: the average of `samples` sample planes and of each
+ * ss x ss block of a (width*ss) x (height*ss) premultiplied image (the MSAA
+ * resolve, main.cpp:700-714, and the supersample downsample, :715-731),
+ * then sRGB8 rgb (exact curve, round to nearest) and linear 8-bit alpha,
+ * RGBA byte order. */
+int orc_finalize_srgb8(const float* in, uint32_t width, uint32_t height, uint32_t samples, uint32_t ss,
+                       uint8_t* out);
 
 /* FP64 mirror: per target, all fragments sorted by (depth key, id) and blended
  * front to back in double (quantize=1 uses the packed sRGB8 colour, 0 the
@@ -93,6 +106,11 @@ int orc_run_fp64(const orc_config* cfg, uint64_t F, const uint32_t* target, cons
 int orc_replay_kbuffer(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* rgba,
                        const uint32_t* count, const uint32_t* ab_color, const uint32_t* ab_key,
                        const uint32_t* ab_id, const uint8_t* tail_flag, int tail_quantized, float* out);
+/* With coverage masks (fragment masks and the A-buffer's mask plane). */
+int orc_replay_kbuffer_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* rgba,
+                           const uint32_t* mask, const uint32_t* count, const uint32_t* ab_color,
+                           const uint32_t* ab_key, const uint32_t* ab_id, const uint32_t* ab_mask,
+                           const uint8_t* tail_flag, int tail_quantized, float* out);
 
 #ifdef __cplusplus
 }

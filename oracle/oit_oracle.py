@@ -30,6 +30,7 @@ class OrcConfig(ctypes.Structure):
         ("background", ctypes.c_float * 4),
         ("wboit_view_near", ctypes.c_float),
         ("wboit_view_span", ctypes.c_float),
+        ("coverage_samples", ctypes.c_uint32),
     ]
 
 
@@ -78,6 +79,13 @@ def lib() -> ctypes.CDLL:
         L.orc_replay_kbuffer.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_uint64, P, P, P, P, P, P, P,
                                          ctypes.c_int, P]
         L.orc_replay_kbuffer.restype = ctypes.c_int
+        L.orc_run_cov.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_uint64, P, P, P, P, ctypes.POINTER(OrcResult)]
+        L.orc_run_cov.restype = ctypes.c_int
+        L.orc_replay_kbuffer_cov.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_uint64, P, P, P, P, P, P, P, P, P,
+                                             ctypes.c_int, P]
+        L.orc_replay_kbuffer_cov.restype = ctypes.c_int
+        L.orc_finalize_srgb8.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P]
+        L.orc_finalize_srgb8.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -106,8 +114,10 @@ def decode_premult(packed: int) -> np.ndarray:
 
 
 def make_config(algorithm, layers, tail_blend, width, height, samples=1, tail_order=TAIL_CANONICAL,
-                ll_capacity=0, background=(0.5, 0.5, 0.5, 1.0), wboit_view_near=2.0, wboit_view_span=8.0):
+                ll_capacity=0, background=(0.5, 0.5, 0.5, 1.0), wboit_view_near=2.0, wboit_view_span=8.0,
+                coverage_samples=0):
     c = OrcConfig()
+    c.coverage_samples = coverage_samples
     c.algorithm = algorithm
     c.layers = layers
     c.tail_blend = 1 if tail_blend else 0
@@ -129,9 +139,11 @@ def _arrays(target, depth, rgba):
     return target, depth, rgba
 
 
-def run(cfg: OrcConfig, target, depth, rgba) -> dict:
-    """Sequential run of one frame; returns numpy arrays of every output."""
+def run(cfg: OrcConfig, target, depth, rgba, mask=None) -> dict:
+    """Sequential run of one frame; returns numpy arrays of every output.
+    `mask`: per-fragment sample masks (coverage shading)."""
     target, depth, rgba = _arrays(target, depth, rgba)
+    mask_arr = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint32)
     F = target.shape[0]
     T = cfg.width * cfg.height * cfg.samples
     K = max(cfg.layers, 1)
@@ -150,8 +162,8 @@ def run(cfg: OrcConfig, target, depth, rgba) -> dict:
     for name in ("out", "composite", "colorbuf", "count", "frontk_key", "frontk_val", "tail_flag", "wb_accum",
                  "wb_reveal"):
         setattr(r, name, out[name].ctypes.data)
-    rc = lib().orc_run(ctypes.byref(cfg), F, target.ctypes.data, depth.ctypes.data, rgba.ctypes.data,
-                       ctypes.byref(r))
+    rc = lib().orc_run_cov(ctypes.byref(cfg), F, target.ctypes.data, depth.ctypes.data, rgba.ctypes.data,
+                           None if mask_arr is None else mask_arr.ctypes.data, ctypes.byref(r))
     if rc != 0:
         raise RuntimeError(f"orc_run failed: {rc}")
     out["tail_flag"] = out["tail_flag"][:F]
@@ -174,18 +186,35 @@ def run_fp64(cfg: OrcConfig, target, depth, rgba, quantize=True) -> dict:
     return {"out": out, "wb_accum": acc, "wb_reveal": rev}
 
 
-def replay_kbuffer(cfg: OrcConfig, target, rgba, count, ab_color, ab_key, ab_id, tail_flag, tail_quantized):
-    """Composite of given k-buffer contents ([T, K] arrays) and a given tail set."""
+def replay_kbuffer(cfg: OrcConfig, target, rgba, count, ab_color, ab_key, ab_id, tail_flag, tail_quantized,
+                   mask=None, ab_mask=None):
+    """Composite of given k-buffer contents ([T, K] arrays) and a given tail set
+    (with coverage shading: fragment masks and the A-buffer's mask plane)."""
     target = np.ascontiguousarray(target, dtype=np.uint32)
     rgba = np.ascontiguousarray(rgba, dtype=np.float32).reshape(-1, 4)
     T = cfg.width * cfg.height * cfg.samples
     out = np.zeros((T, 4), np.float32)
     arrs = [np.ascontiguousarray(a, dtype=np.uint32) for a in (count, ab_color, ab_key, ab_id)]
     tf = np.ascontiguousarray(tail_flag, dtype=np.uint8)
-    rc = lib().orc_replay_kbuffer(ctypes.byref(cfg), target.shape[0], target.ctypes.data, rgba.ctypes.data,
-                                  arrs[0].ctypes.data, arrs[1].ctypes.data, arrs[2].ctypes.data,
-                                  arrs[3].ctypes.data, tf.ctypes.data, 1 if tail_quantized else 0,
-                                  out.ctypes.data)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint32)
+    am = None if ab_mask is None else np.ascontiguousarray(ab_mask, dtype=np.uint32)
+    rc = lib().orc_replay_kbuffer_cov(ctypes.byref(cfg), target.shape[0], target.ctypes.data, rgba.ctypes.data,
+                                      None if m is None else m.ctypes.data, arrs[0].ctypes.data,
+                                      arrs[1].ctypes.data, arrs[2].ctypes.data, arrs[3].ctypes.data,
+                                      None if am is None else am.ctypes.data, tf.ctypes.data,
+                                      1 if tail_quantized else 0, out.ctypes.data)
     if rc != 0:
         raise RuntimeError(f"orc_replay_kbuffer failed: {rc}")
+    return out
+
+
+def finalize_srgb8(image, width, height, samples=1, supersample=1) -> np.ndarray:
+    """Display image (MSAA resolve and supersample box filter, then sRGB8):
+    uint8 [height, width, 4] RGBA."""
+    img = np.ascontiguousarray(image, dtype=np.float32).reshape(-1, 4)
+    assert img.shape[0] == width * height * samples * supersample * supersample
+    out = np.zeros((height, width, 4), np.uint8)
+    rc = lib().orc_finalize_srgb8(img.ctypes.data, width, height, samples, supersample, out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_finalize_srgb8 failed: {rc}")
     return out

@@ -8,7 +8,8 @@ import numpy as np
 
 
 def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, samples=1, ll_pool_nodes=0,
-            splits=1, background=(0.5, 0.5, 0.5, 1.0), max_fragments=0, renderer=None, variant=0):
+            splits=1, background=(0.5, 0.5, 0.5, 1.0), max_fragments=0, renderer=None, variant=0,
+            coverage_samples=0, mask=None):
     from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, Renderer
 
     T = width * height * samples
@@ -18,12 +19,14 @@ def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, s
     try:
         cfg = FrameConfig(algorithm=algorithm, layers=layers, tail_blend=bool(tail_blend), width=width,
                           height=height, samples=samples, ll_pool_nodes=ll_pool_nodes,
-                          max_fragments=max_fragments, background=background, variant=variant)
+                          max_fragments=max_fragments, background=background, variant=variant,
+                          coverage_samples=coverage_samples)
         r.configure(cfg)
         F = int(len(target))
         tg = DeviceBuffer.from_numpy(np.ascontiguousarray(target, dtype=np.uint32).reshape(F))
         dp = DeviceBuffer.from_numpy(np.ascontiguousarray(depth, dtype=np.float32).reshape(F))
         cl = DeviceBuffer.from_numpy(np.ascontiguousarray(rgba, dtype=np.float32).reshape(F, 4))
+        mk = None if mask is None else DeviceBuffer.from_numpy(np.ascontiguousarray(mask, dtype=np.uint32).reshape(F))
         out = DeviceBuffer((T, 4), np.float32)
         fk_key = DeviceBuffer((T, K), np.uint32)
         fk_val = DeviceBuffer((T, K), np.uint32)
@@ -32,7 +35,8 @@ def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, s
         bounds = np.linspace(0, F, splits + 1).astype(np.int64)
         for a, b in zip(bounds[:-1], bounds[1:]):
             n = int(b - a)
-            r.append(tg.view(a, n), dp.view(a, n), cl.view(a, n), count=n)
+            r.append(tg.view(a, n), dp.view(a, n), cl.view(a, n), count=n,
+                     coverage=None if mk is None else mk.view(a, n))
         r.resolve_debug(out, fk_key, fk_val, counts)
         st = r.stats()
         return {

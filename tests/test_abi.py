@@ -126,9 +126,9 @@ def test_state_mirrors_reference_antialiasing_rules():
     s.aaType = _abi.AA_SUPER_4X
     cfg = FrameConfig.from_state(s, 64, 32)
     assert (cfg.width, cfg.height, cfg.samples) == (128, 64, 1)
-    s.aaType = _abi.AA_MSAA_4X
-    with pytest.raises(ValueError):
-        FrameConfig.from_state(s, 64, 32)
+    s.aaType = _abi.AA_MSAA_4X  # coverage shading: per-pixel fragments with 4-sample masks
+    cfg = FrameConfig.from_state(s, 64, 32)
+    assert (cfg.width, cfg.height, cfg.samples, cfg.coverage_samples) == (64, 32, 1, 4)
 
 
 def test_library_codec_tables_equal_the_oracle(oracle):

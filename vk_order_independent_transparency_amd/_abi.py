@@ -67,6 +67,8 @@ class OitConfig(ctypes.Structure):
         ("background", ctypes.c_float * 4),
         ("wboit_view_near", ctypes.c_float),
         ("wboit_view_span", ctypes.c_float),
+        ("coverage_samples", ctypes.c_uint32),
+        ("reserved", ctypes.c_uint32),
     ]
 
 
@@ -76,6 +78,7 @@ class OitFragments(ctypes.Structure):
         ("depth", ctypes.c_void_p),
         ("rgba", ctypes.c_void_p),
         ("count", ctypes.c_uint64),
+        ("coverage", ctypes.c_void_p),
     ]
 
 
@@ -212,6 +215,8 @@ SIGNATURES = {
     "oit_resolve": (ctypes.c_int, [_P, _P, _P]),
     "oit_resolve_debug": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "oit_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(OitStats), _P]),
+    "oit_finalize_srgb8": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
+                                          _P, _P]),
     "oit_status_string": (ctypes.c_char_p, [ctypes.c_int]),
     "oit_profile_enable": (ctypes.c_int, [_P, ctypes.c_int]),
     "oit_profile_read": (ctypes.c_int, [_P, ctypes.POINTER(OitProfile)]),

@@ -133,6 +133,11 @@ struct oit_ctx {
   uint64_t cap_ll_head = 0, ll_alloc = 0;
   uint64_t cap_kb_count = 0, cap_kb_lock = 0, cap_kb_depth = 0, cap_kb_elems = 0;
   uint64_t cap_l64 = 0, cap_wb = 0, cap_tl_head = 0;
+  // coverage shading: sample masks of nodes, k-buffer slots and tail entries
+  uint32_t* ll_mask = nullptr;
+  uint32_t* kb_mask = nullptr;
+  uint32_t* tl_mask = nullptr;
+  uint64_t cap_ll_mask = 0, cap_kb_mask = 0, cap_tl_mask = 0;
   // the last stream-ordered call's completion: buffers are freed only after it
   hipEvent_t work_ev = nullptr;
   bool work_pending = false;
@@ -218,6 +223,10 @@ void free_frame_buffers(oit_ctx* c) {
   c->cap_ll_head = c->ll_alloc = 0;
   c->cap_kb_count = c->cap_kb_lock = c->cap_kb_depth = c->cap_kb_elems = 0;
   c->cap_l64 = c->cap_wb = c->cap_tl_head = 0;
+  c->cap_ll_mask = c->cap_kb_mask = c->cap_tl_mask = 0;
+  dev_free(&c->ll_mask);
+  dev_free(&c->kb_mask);
+  dev_free(&c->tl_mask);
   dev_free(&c->ll_head);
   dev_free(&c->ll_cd);
   dev_free(&c->ll_next);
@@ -279,8 +288,11 @@ oit::Common common_of(const oit_ctx* c) {
   k.tail = c->cfg.tail_blend ? 1 : 0;
   k.background = make_float4(c->cfg.background[0], c->cfg.background[1], c->cfg.background[2], c->cfg.background[3]);
   k.tab = c->d_tab;
+  k.cov = c->cfg.coverage_samples > 1 ? c->cfg.coverage_samples : 0;
   return k;
 }
+
+bool coverage(const oit_ctx* c) { return c->cfg.coverage_samples > 1; }
 
 oit::TailBuf tail_of(const oit_ctx* c) {
   oit::TailBuf t;
@@ -291,6 +303,7 @@ oit::TailBuf tail_of(const oit_ctx* c) {
   t.rec = c->tl_rec;
   t.counter = c->d_counters;
   t.cap = c->tl_cap;
+  t.mask = c->cfg.coverage_samples > 1 ? c->tl_mask : nullptr;
   return t;
 }
 
@@ -319,19 +332,24 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
   uint32_t* nn = nullptr;
   float4* nr = nullptr;
   uint64_t* nrec = nullptr;
+  uint32_t* nm = nullptr;
   int rc = dev_alloc(&nn, cap);
   if (!rc) rc = dev_alloc(&nr, cap);
   if (!rc) rc = dev_alloc(&nrec, cap);
+  if (!rc && coverage(c)) rc = dev_alloc(&nm, cap);
   if (rc) {
     dev_free(&nn);
     dev_free(&nr);
     dev_free(&nrec);
+    dev_free(&nm);
     return rc;
   }
   if (c->tl_cap) {
     (void)hipMemcpyAsync(nn, c->tl_next, (uint64_t)c->tl_cap * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     (void)hipMemcpyAsync(nr, c->tl_rgba, (uint64_t)c->tl_cap * sizeof(float4), hipMemcpyDeviceToDevice, s);
     (void)hipMemcpyAsync(nrec, c->tl_rec, (uint64_t)c->tl_cap * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+    if (nm && c->tl_mask)
+      (void)hipMemcpyAsync(nm, c->tl_mask, (uint64_t)c->tl_cap * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     (void)hipStreamSynchronize(s);
   }
   // records this frame has not written read as "not in the tail"
@@ -340,9 +358,12 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
   dev_free(&c->tl_next);
   dev_free(&c->tl_rgba);
   dev_free(&c->tl_rec);
+  dev_free(&c->tl_mask);
   c->tl_next = nn;
   c->tl_rgba = nr;
   c->tl_rec = nrec;
+  c->tl_mask = nm;
+  c->cap_tl_mask = nm ? cap : 0;
   c->tl_cap = (uint32_t)cap;
   return fe == hipSuccess ? OIT_OK : status_of(fe);
 }
@@ -426,6 +447,13 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   if (T >= 0xFFFFFFFFull) return OIT_ERR_INVALID_ARG;
   for (int k = 0; k < 4; ++k)
     if (!std::isfinite(cfg->background[k])) return OIT_ERR_INVALID_ARG;
+  const uint32_t cov = cfg->coverage_samples;
+  if (cov > 1) {  // OIT_COVERAGE_SHADING
+    if (cov != 2 && cov != 4 && cov != 8) return OIT_ERR_INVALID_ARG;
+    if (S != 1) return OIT_ERR_INVALID_ARG;  // per-pixel fragments
+    if (alg == OIT_LOOP64 || alg == OIT_WEIGHTED) return OIT_ERR_UNSUPPORTED;  // README.md:33-34
+    if (alg == OIT_LINKEDLIST && (cfg->variant & OIT_VARIANT_LL_SPLIT_APPEND)) return OIT_ERR_UNSUPPORTED;
+  }
   uint64_t ll_cap = 0;
   if (alg == OIT_LINKEDLIST) {  // oit.cpp:119-126
     ll_cap = cfg->ll_pool_nodes;
@@ -459,6 +487,8 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
     release(c, &c->kb_lock, &c->cap_kb_lock);
     release(c, &c->kb_depth, &c->cap_kb_depth);
   }
+  if (alg != OIT_LINKEDLIST) release(c, &c->ll_mask, &c->cap_ll_mask);
+  if (alg != OIT_SIMPLE && alg != OIT_SPINLOCK) release(c, &c->kb_mask, &c->cap_kb_mask);
   if (alg != OIT_LOOP64) release(c, &c->l64, &c->cap_l64);
   if (alg != OIT_WEIGHTED) {
     uint64_t d = c->cap_wb;
@@ -475,8 +505,10 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
         rc = reserve(c, &c->ll_cd, &d, ll_cap, &ll_grown);
         if (!rc) rc = reserve(c, &c->ll_next, &c->ll_alloc, ll_cap, &ll_grown);
       }
+      if (!rc && cov > 1) rc = reserve(c, &c->ll_mask, &c->cap_ll_mask, ll_cap);
+      if (!rc && cov <= 1) release(c, &c->ll_mask, &c->cap_ll_mask);
       c->ll_cap = ll_cap;
-      c->bytes_abuf = ll_cap * 12u;
+      c->bytes_abuf = ll_cap * (cov > 1 ? 16u : 12u);
       c->bytes_aux = T * 4u;
       break;
     }
@@ -491,7 +523,9 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
       }
       if (!rc && alg == OIT_SPINLOCK) rc = reserve(c, &c->kb_lock, &c->cap_kb_lock, T);
       if (!rc && alg == OIT_SPINLOCK) rc = reserve(c, &c->kb_depth, &c->cap_kb_depth, T);
-      c->bytes_abuf = T * K * 12u;
+      if (!rc && cov > 1) rc = reserve(c, &c->kb_mask, &c->cap_kb_mask, T * K);
+      if (!rc && cov <= 1) release(c, &c->kb_mask, &c->cap_kb_mask);
+      c->bytes_abuf = T * K * (cov > 1 ? 16u : 12u);
       c->bytes_aux = T * (alg == OIT_SPINLOCK ? 12u : 4u);
       break;
     }
@@ -522,12 +556,15 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
       if (!rc) rc = reserve(c, &c->tl_rec, &d3, cap);
       if (!rc) c->tl_cap = (uint32_t)cap;
     }
+    if (!rc && cov > 1 && c->tl_cap) rc = reserve(c, &c->tl_mask, &c->cap_tl_mask, c->tl_cap);
+    if (!rc && cov <= 1) release(c, &c->tl_mask, &c->cap_tl_mask);
   } else if (!rc) {
     release(c, &c->tl_head, &c->cap_tl_head);
     uint64_t d1 = c->tl_cap, d2 = c->tl_cap, d3 = c->tl_cap;
     release(c, &c->tl_next, &d1);
     release(c, &c->tl_rgba, &d2);
     release(c, &c->tl_rec, &d3);
+    release(c, &c->tl_mask, &c->cap_tl_mask);
     c->tl_cap = 0;
   }
   if (rc) {
@@ -586,7 +623,8 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
   out->targets = c->T;
   out->resident_bytes = c->cap_ll_head * 4u + c->ll_alloc * 12u + c->cap_kb_count * 4u + c->cap_kb_lock * 4u +
                         c->cap_kb_depth * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_wb * 20u +
-                        c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u);
+                        c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u) +
+                        (c->cap_ll_mask + c->cap_kb_mask + c->cap_tl_mask) * 4u;
   return OIT_OK;
 }
 
@@ -665,6 +703,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
   if (!c->frame_open || c->resolved) return OIT_ERR_STATE;
   if (f->count == 0) return OIT_OK;
   if (!f->target || !f->depth || !f->rgba) return OIT_ERR_INVALID_ARG;
+  if (coverage(c) && !f->coverage) return OIT_ERR_INVALID_ARG;  // coverage shading needs the sample masks
   if (f->count > kMaxFrameFragments || c->frame_frags + f->count > kMaxFrameFragments) return OIT_ERR_CAPACITY;
   DeviceGuard g(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
@@ -683,12 +722,14 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
   v.rgba = reinterpret_cast<const float4*>(f->rgba);
   v.count = (uint32_t)f->count;
   v.id_base = (uint32_t)c->frame_frags;
+  v.mask = coverage(c) ? f->coverage : nullptr;
   const oit::Common k = common_of(c);
   const oit::TailBuf tb = tail_of(c);
   hipError_t e = hipSuccess;
   switch (alg) {
     case OIT_LINKEDLIST: {
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      ll.mask = coverage(c) ? c->ll_mask : nullptr;
       c->ll_clean = false;
       c->counters_clean = false;
       const uint64_t touched = end + 1 < c->ll_cap ? end + 1 : c->ll_cap;
@@ -705,11 +746,13 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
     }
     case OIT_SIMPLE: {
       oit::KBuf kb{c->kb_count, nullptr, nullptr, c->kb_col, c->kb_key, c->kb_id};
+      kb.mask = coverage(c) ? c->kb_mask : nullptr;
       e = staged(c, kStStore, s, [&] { return oit::launch_simple_store(k, v, kb, tb, s); });
       break;
     }
     case OIT_SPINLOCK: {
       oit::KBuf kb{c->kb_count, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
+      kb.mask = coverage(c) ? c->kb_mask : nullptr;
       // a fragment's record is written by whichever fragment pushes it to the
       // tail, so this append's records start empty
       if (tb.head && tb.rec)
@@ -772,7 +815,12 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
       const bool pair = !(c->cfg.variant & OIT_VARIANT_LL_SINGLE_WALK);
       const bool keys = (c->cfg.variant & OIT_VARIANT_LL_KEYS_ONLY) != 0;
-      e = staged(c, kStResolve, s, [&] { return oit::launch_ll_resolve_reset(k, ll, tbr, o, pair, keys, s); });
+      if (coverage(c)) {
+        ll.mask = c->ll_mask;
+        e = staged(c, kStResolve, s, [&] { return oit::launch_ll_resolve_cov(k, ll, tbr, o, s); });
+      } else {
+        e = staged(c, kStResolve, s, [&] { return oit::launch_ll_resolve_reset(k, ll, tbr, o, pair, keys, s); });
+      }
       if (e == hipSuccess) {
         c->ll_clean = true;
         c->ll_dirty_upto = 0;
@@ -784,7 +832,12 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     case OIT_SIMPLE:
     case OIT_SPINLOCK: {
       oit::KBuf kb{c->kb_count, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
-      e = staged(c, kStResolve, s, [&] { return oit::launch_kbuf_resolve(k, kb, tbr, o, s); });
+      if (coverage(c)) {
+        kb.mask = c->kb_mask;
+        e = staged(c, kStResolve, s, [&] { return oit::launch_kbuf_resolve_cov(k, kb, tbr, o, s); });
+      } else {
+        e = staged(c, kStResolve, s, [&] { return oit::launch_kbuf_resolve(k, kb, tbr, o, s); });
+      }
       break;
     }
     case OIT_LOOP64:
@@ -798,6 +851,19 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
   }
   mark_work(c, s);
   return status_of(e);
+}
+
+int oit_finalize_srgb8(oit_ctx* c, const float* in_rgba, uint32_t width, uint32_t height, uint32_t samples,
+                       uint32_t supersample, uint8_t* out_rgba8, void* stream) {
+  if (!c || !in_rgba || !out_rgba8 || width == 0 || height == 0) return OIT_ERR_INVALID_ARG;
+  if (samples != 1 && samples != 2 && samples != 4 && samples != 8) return OIT_ERR_INVALID_ARG;
+  if (supersample < 1 || supersample > 8) return OIT_ERR_INVALID_ARG;
+  if ((uint64_t)width * height * samples * supersample * supersample >= 0xFFFFFFFFull) return OIT_ERR_INVALID_ARG;
+  if (((uintptr_t)in_rgba & 15u) || ((uintptr_t)out_rgba8 & 3u)) return OIT_ERR_INVALID_ARG;
+  DeviceGuard g(c->device);
+  return status_of(oit::launch_finalize_srgb8(reinterpret_cast<const float4*>(in_rgba), width, height, samples,
+                                              supersample, c->d_tab, reinterpret_cast<uint32_t*>(out_rgba8),
+                                              static_cast<hipStream_t>(stream)));
 }
 
 int oit_resolve(oit_ctx* c, float* out_rgba, void* stream) {

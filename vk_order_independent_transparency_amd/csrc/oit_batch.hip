@@ -96,6 +96,7 @@ int make_plan(const oit_batch_frame* fr, uint32_t n, Plan& p) {
     if ((uint64_t)f.width * f.height * f.samples > 0xFFFFFFFFull) return OIT_ERR_INVALID_ARG;
     if (!f.out || ((uintptr_t)f.out & 15u)) return OIT_ERR_INVALID_ARG;
     if (f.frags.count > kMaxFrameFragments) return OIT_ERR_CAPACITY;
+    if (f.frags.coverage) return OIT_ERR_UNSUPPORTED;  // batches hold sample-shaded or single-sample frames
     if (f.frags.count > 0) {
       if (!f.frags.target || !f.frags.depth || !f.frags.rgba) return OIT_ERR_INVALID_ARG;
       if ((uintptr_t)f.frags.rgba & 15u) return OIT_ERR_INVALID_ARG;

@@ -59,12 +59,13 @@ __device__ __forceinline__ uint32_t tail_value_cap(uint32_t cap) { return (cap +
 // tail_record_own: the lane's own fragment, tail or not (writes the record
 // either way).
 __device__ __forceinline__ void tail_record_own(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
-                                                const TailBuf& tb) {
+                                                const TailBuf& tb, uint32_t mask = 0xFFFFFFFFu) {
   if (lid >= tb.cap) return;
   tb.rec[lid] = tail ? ((uint64_t)t << 32) | tail_value(lid, flag) : kEmpty64;
   if (tail) {
     tb.rgba[lid] = payload;
     tb.next[lid] = kUnset;
+    if (tb.mask) tb.mask[lid] = mask;
   }
 }
 
@@ -72,11 +73,12 @@ __device__ __forceinline__ void tail_record_own(bool tail, uint32_t t, uint32_t 
 // the A-buffer) goes to the tail; records of the range were filled with
 // kEmpty64 before the kernel.
 __device__ __forceinline__ void tail_record_evict(bool tail, uint32_t t, uint32_t lid, uint32_t flag,
-                                                  float4 payload, const TailBuf& tb) {
+                                                  float4 payload, const TailBuf& tb, uint32_t mask = 0xFFFFFFFFu) {
   if (!tail || lid >= tb.cap) return;
   tb.rec[lid] = ((uint64_t)t << 32) | tail_value(lid, flag);
   tb.rgba[lid] = payload;
   tb.next[lid] = kUnset;
+  if (tb.mask) tb.mask[lid] = mask;
 }
 
 // Sorted register array of K (sort key, colour) pairs, ascending; inserting x
@@ -211,6 +213,58 @@ __device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float
     v = nx;
   }
   return over(acc, bg);
+}
+
+// Coverage shading: the ROP blends each tail fragment into the samples of
+// its mask; per sample the newest-first fold over the background, then the
+// sum over samples in order divided by their count (the MSAA resolve). A
+// target without tail fragments keeps the background.
+__device__ __forceinline__ float4 tail_fold_cov(const TailBuf& tb, uint32_t t, float4 bg, uint32_t S) {
+  const uint32_t vcap = tail_value_cap(tb.cap);
+  uint32_t v = tb.head[t];
+  if (v == 0u || v >= vcap) return bg;
+  float4 acc[8];
+#pragma unroll
+  for (int s = 0; s < 8; ++s) acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
+  while (v != 0u && v < vcap) {  // a link outside the arrays ends the walk
+    const uint32_t id = tail_index(v);
+    const uint32_t nx = tb.next[id];
+    const uint32_t m = tb.mask[id];
+    const float4 c = tb.rgba[id];
+#pragma unroll
+    for (int s = 0; s < 8; ++s)
+      if ((uint32_t)s < S && ((m >> s) & 1u)) acc[s] = over(acc[s], c);
+    v = nx;
+  }
+  float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int s = 0; s < 8; ++s) {
+    if ((uint32_t)s < S) {
+      const float4 cb = over(acc[s], bg);
+      sum = make_float4(sum.x + cb.x, sum.y + cb.y, sum.z + cb.z, sum.w + cb.w);
+    }
+  }
+  const float fs = (float)S;
+  return make_float4(sum.x / fs, sum.y / fs, sum.z / fs, sum.w / fs);
+}
+
+// Coverage-shaded front blend (oitLinkedList.frag.glsl:147-160,
+// oitSimple.frag.glsl:142-159): per sample, front to back over the sorted
+// layers whose mask holds it; the sample colours summed in order and divided
+// by the sample count.
+template <int K>
+__device__ __forceinline__ float4 blend_front_cov(const uint32_t* cols, const uint32_t* masks, uint32_t n, uint32_t S,
+                                                  const float* s_dec, const float* s_alpha) {
+  float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t s = 0; s < S; ++s) {
+    float4 sc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if ((uint32_t)k < n && ((masks[k] >> s) & 1u)) sc = over(sc, decode_premult_lds(cols[k], s_dec, s_alpha));
+    sum = make_float4(sum.x + sc.x, sum.y + sc.y, sum.z + sc.z, sum.w + sc.w);
+  }
+  const float fs = (float)S;
+  return make_float4(sum.x / fs, sum.y / fs, sum.z / fs, sum.w / fs);
 }
 
 template <int K>
@@ -364,13 +418,15 @@ struct AppendSrc {
   uint32_t T;      // targets of the frame
   uint32_t tbase;  // head index of the frame's target 0
   uint32_t lid0;   // frame-local id of fragment 0 (colour-pass tail records)
+  const uint32_t* mask = nullptr;  // coverage shading: sample masks (a zero mask drops the fragment)
+  uint32_t full = 0;               // the pixel's sample bits
 };
 
 template <bool kTail, int kU>
 __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* s_thr, const LLBuf& ll,
                                                const TailBuf& tb) {
   const uint32_t lane = lane_id();
-  uint32_t t[kU];
+  uint32_t t[kU], m[kU];
   bool node[kU], ovf[kU];
   float4 col[kU];
   float dep[kU];
@@ -379,6 +435,11 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
     const uint64_t i = f.first + (uint64_t)u * 64u + lane;
     t[u] = i < f.count ? f.target[i] : kEmpty;
     if (t[u] >= f.T) t[u] = kEmpty;
+    m[u] = 0xFFFFFFFFu;
+    if (f.mask && t[u] != kEmpty) {
+      m[u] = f.mask[i] & f.full;
+      if (m[u] == 0u) t[u] = kEmpty;  // no sample covered: the shader does not run
+    }
     const uint64_t n64 = (uint64_t)f.node0 + i;
     node[u] = t[u] != kEmpty && n64 < ll.cap;
     ovf[u] = t[u] != kEmpty && !node[u];
@@ -393,6 +454,7 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
   for (int u = 0; u < kU; ++u) {
     const uint32_t n = (uint32_t)(f.node0 + f.first + (uint64_t)u * 64u + lane);
     if (node[u]) ll.cd[n] = ((uint64_t)depth_key(dep[u]) << 32) | encode_color_lds(col[u], s_thr);
+    if (node[u] && ll.mask) ll.mask[n] = m[u];  // storeMask, oitLinkedList.frag.glsl:79
     const Run r = same_target_run(node[u] ? t[u] : kEmpty);
     if (node[u] && !r.start) st_relaxed(&ll.next[n], n - 1u);
     ins[u] = node[u] && r.end;
@@ -425,7 +487,7 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
     if (om && lane == (uint32_t)__builtin_ctzll(om)) atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
     if constexpr (kTail) {
       const uint32_t lid = (uint32_t)(f.lid0 + f.first + (uint64_t)u * 64u + lane);
-      tail_record_own(ovf[u], t[u], lid, 0u, col[u], tb);
+      tail_record_own(ovf[u], t[u], lid, 0u, col[u], tb, m[u]);
     }
   }
 }
@@ -444,6 +506,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBu
   a.T = c.T;
   a.tbase = 0;
   a.lid0 = f.id_base;
+  a.mask = f.mask;
+  a.full = c.cov > 1u ? (1u << c.cov) - 1u : 0xFFFFFFFFu;
   ll_append_wave<kTail, kU>(a, s_thr, ll, tb);
 }
 
@@ -452,8 +516,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBu
 // tail (before the colour buffer). Resets head[t] and every walked link.
 // Reached by every lane of the wave (wave_max_u32).
 template <int K, bool kTail, bool kPair>
-__device__ __forceinline__ float4 ll_walk_reset(uint32_t t, const LLBuf& ll, const float* s_dec,
-                                                const float* s_alpha, FrontK<K>& a, uint32_t& cnt) {
+__device__ __forceinline__ float4 ll_walk_core(uint32_t t, const LLBuf& ll, const float* s_dec,
+                                               const float* s_alpha, FrontK<K>& a, uint32_t& cnt) {
   a.init();
   float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t p = ll.head[t];
@@ -495,6 +559,14 @@ __device__ __forceinline__ float4 ll_walk_reset(uint32_t t, const LLBuf& ll, con
       p = nx0;
     }
   }
+  return tailc;
+}
+
+// ll_walk_core, then the front K blended over the tail colour.
+template <int K, bool kTail, bool kPair>
+__device__ __forceinline__ float4 ll_walk_reset(uint32_t t, const LLBuf& ll, const float* s_dec,
+                                                const float* s_alpha, FrontK<K>& a, uint32_t& cnt) {
+  const float4 tailc = ll_walk_core<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
   const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
   const uint32_t nmax = wave_max_u32(n);
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -629,6 +701,36 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset_keys(Common c, LLBu
   }
 }
 
+// Coverage-shaded composite of the Linked List (OIT_COVERAGE_SHADING,
+// oitLinkedList.frag.glsl:147-160): the walk and tail as above; the front K
+// blended per sample with their node masks; the colour buffer averaged over
+// samples. Resets the structure like k_ll_resolve_reset.
+template <int K, bool kTail>
+__global__ __launch_bounds__(kBlock) void k_ll_resolve_cov(Common c, LLBuf ll, TailBuf tb, ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, c.tab);
+  if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
+    ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
+    ll.counters[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+    FrontK<K> a;
+    uint32_t cnt;
+    const float4 tailc = ll_walk_core<K, kTail, true>(t, ll, s_dec, s_alpha, a, cnt);
+    const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
+    uint32_t masks[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) masks[k] = (uint32_t)k < n ? ll.mask[(uint32_t)a.sk[k]] : 0u;
+    const float4 col = over(blend_front_cov<K>(a.col, masks, n, c.cov, s_dec, s_alpha), tailc);
+    const float4 d = tb.head ? tail_fold_cov(tb, t, c.background, c.cov) : c.background;
+    o.out[t] = over(col, d);
+    write_debug<K>(o, t, a, n, cnt, true, false);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Batched Linked List: many frames in one pool. Frame f's fragments are pool
 // nodes frag_base + 1 .. frag_base + count, its targets are head entries
@@ -714,6 +816,14 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
     valid = false;
     t = kEmpty;
   }
+  uint32_t m = 0xFFFFFFFFu;
+  if (valid && f.mask) {
+    m = f.mask[i] & ((1u << c.cov) - 1u);
+    if (m == 0u) {  // no sample covered: the shader does not run
+      valid = false;
+      t = kEmpty;
+    }
+  }
   const uint32_t lid = f.id_base + i;
   const Run r = same_target_run(t);
   uint32_t base = 0;
@@ -729,11 +839,12 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
       kb.col[idx] = encode_color_lds(col, s_thr);
       kb.key[idx] = depth_key(f.depth[i]);
       kb.id[idx] = lid;
+      if (kb.mask) kb.mask[idx] = m;  // storeMask, oitSimple.frag.glsl:70
     } else {
       tail = true;
     }
   }
-  if constexpr (kTail) tail_record_own(tail, t, lid, 0u, col, tb);
+  if constexpr (kTail) tail_record_own(tail, t, lid, 0u, col, tb, m);
 }
 
 // ---------------------------------------------------------------------------
@@ -755,11 +866,17 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   bool valid = i < f.count;
   uint32_t t = valid ? f.target[i] : kEmpty;
   if (t >= c.T) valid = false;
+  uint32_t my_mask = 0xFFFFFFFFu;
+  if (valid && f.mask) {
+    my_mask = f.mask[i] & ((1u << c.cov) - 1u);
+    if (my_mask == 0u) valid = false;  // no sample covered: the shader does not run
+  }
   const uint32_t lid = f.id_base + i;
   bool tail = false;
   bool pending = false;
   uint32_t tail_id = lid;
   uint32_t tail_col = 0u;
+  uint32_t tail_mask = my_mask;
   uint32_t my_key = 0u, my_col = 0u;
   if (valid) {
     const uint32_t key = depth_key(f.depth[i]);
@@ -784,6 +901,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
           st_relaxed(&kb.col[idx], my_col);
           st_relaxed(&kb.key[idx], my_key);
           st_relaxed(&kb.id[idx], lid);
+          if (kb.mask) st_relaxed(&kb.mask[idx], my_mask);
         } else {
           uint32_t fur = 0;                               // :96-107
           uint64_t mx = 0;
@@ -801,6 +919,10 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
             const uint64_t idx = (uint64_t)fur * c.T + t;
             tail_col = ld_relaxed(&kb.col[idx]);
             tail_id = (uint32_t)mx;
+            if (kb.mask) {
+              tail_mask = ld_relaxed(&kb.mask[idx]);
+              st_relaxed(&kb.mask[idx], my_mask);
+            }
             st_relaxed(&kb.col[idx], my_col);
             st_relaxed(&kb.key[idx], my_key);
             st_relaxed(&kb.id[idx], lid);
@@ -816,7 +938,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   if constexpr (kTail) {
     float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
     if (tail) payload = decode_premult(tail_col, c.tab);
-    tail_record_evict(tail, valid ? t : kEmpty, tail_id, 0u, payload, tb);
+    tail_record_evict(tail, valid ? t : kEmpty, tail_id, 0u, payload, tb, tail_mask);
   }
 }
 
@@ -845,6 +967,49 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
   if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
   o.out[t] = over(col, d);
   if (kDebug) write_debug<K>(o, t, a, n, count, false, false);
+}
+
+// Coverage-shaded composite of Simple and Spinlock (oitSimple.frag.glsl:
+// 142-159, oitSpinlock.frag.glsl:180-197): the stored layers sorted by (depth
+// key, id) with their slots, then colours and masks gathered by slot and
+// blended per sample.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, TailBuf tb, ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, c.tab);
+  __syncthreads();
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= c.T) return;
+  const uint32_t count = kb.count[t];
+  const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
+  FrontK<K> a;  // (key << 32 | id, slot)
+  a.init();
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if ((uint32_t)k < n) {
+      const uint64_t idx = (uint64_t)k * c.T + t;
+      uint64_t ev;
+      uint32_t evc;
+      a.insert(((uint64_t)kb.key[idx] << 32) | kb.id[idx], (uint32_t)k, ev, evc);
+    }
+  }
+  uint32_t cols[K], masks[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    cols[k] = 0u;
+    masks[k] = 0u;
+    if ((uint32_t)k < n) {
+      const uint64_t idx = (uint64_t)a.col[k] * c.T + t;
+      cols[k] = kb.col[idx];
+      masks[k] = kb.mask[idx];
+      a.col[k] = cols[k];
+    }
+  }
+  const float4 col = blend_front_cov<K>(cols, masks, n, c.cov, s_dec, s_alpha);
+  const float4 d = tb.head ? tail_fold_cov(tb, t, c.background, c.cov) : c.background;
+  o.out[t] = over(col, d);
+  write_debug<K>(o, t, a, n, count, false, false);
 }
 
 // ---------------------------------------------------------------------------
@@ -1060,6 +1225,32 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t frags
     for (uint32_t w = 0; w < kBlock / 64; ++w) total += s_count[w];
     if (total) atomicAdd(tb.counter, total);
   }
+}
+
+// Display image: average of `S` sample planes and of each ss x ss block of a
+// (W*ss) x (H*ss) premultiplied image (MSAA resolve, main.cpp:700-714;
+// supersample downsample, :715-731), summed in sample, row, column order;
+// then sRGB8 rgb (exact thresholds) and linear 8-bit alpha, RGBA bytes.
+__global__ __launch_bounds__(kBlock) void k_finalize_srgb8(const float4* __restrict__ in, uint32_t W, uint32_t H,
+                                                           uint32_t S, uint32_t ss, const CodecTables* tab,
+                                                           uint32_t* __restrict__ out) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, tab);
+  const uint32_t p = blockIdx.x * kBlock + threadIdx.x;
+  if (p >= W * H) return;
+  const uint32_t x = p % W, y = p / W;
+  const uint64_t Wf = (uint64_t)W * ss, Hf = (uint64_t)H * ss;
+  float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+  for (uint32_t s = 0; s < S; ++s)
+    for (uint32_t dy = 0; dy < ss; ++dy)
+      for (uint32_t dx = 0; dx < ss; ++dx) {
+        const float4 v = in[(s * Hf + (uint64_t)y * ss + dy) * Wf + (uint64_t)x * ss + dx];
+        sum = make_float4(sum.x + v.x, sum.y + v.y, sum.z + v.z, sum.w + v.w);
+      }
+  const float n = (float)(S * ss * ss);
+  const float a = fminf(fmaxf(sum.w / n, 0.0f), 1.0f);
+  out[p] = code_rgb_fast(sum.x / n, s_thr) | (code_rgb_fast(sum.y / n, s_thr) << 8) |
+           (code_rgb_fast(sum.z / n, s_thr) << 16) | ((uint32_t)floorf(a * 255.0f + 0.5f) << 24);
 }
 
 // Device self-check of the branch-free encode against the exact table search
@@ -1364,6 +1555,50 @@ hipError_t launch_batch_append(const BatchArgs& a, const LLBuf& ll, uint32_t chu
 hipError_t launch_batch_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, bool pair, bool keys,
                                 uint32_t grid_cap, hipStream_t s) {
   return dispatch_k<BatchResolve>(K, a, ll, tail, pair, keys, grid_cap, s);
+}
+
+namespace {
+
+template <int K>
+struct LLResolveCov {
+  static hipError_t run(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
+    uint32_t blocks = blocks_for(c.T);
+    if (blocks > 8192u) blocks = 8192u;
+    if (blocks == 0) blocks = 1;
+    if (c.tail) hipLaunchKernelGGL((k_ll_resolve_cov<K, true>), dim3(blocks), dim3(kBlock), 0, s, c, ll, tb, o);
+    else hipLaunchKernelGGL((k_ll_resolve_cov<K, false>), dim3(blocks), dim3(kBlock), 0, s, c, ll, tb, o);
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct KBufResolveCov {
+  static hipError_t run(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
+    hipLaunchKernelGGL((k_kbuf_resolve_cov<K>), dim3(blocks_for(c.T)), dim3(kBlock), 0, s, c, kb, tb, o);
+    return hipGetLastError();
+  }
+};
+
+}  // namespace
+
+hipError_t launch_ll_resolve_cov(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o,
+                                 hipStream_t s) {
+  if (c.cov < 2u || c.cov > 8u || !ll.mask) return hipErrorInvalidValue;
+  return dispatch_k<LLResolveCov>(c.K, c, ll, tb, o, s);
+}
+
+hipError_t launch_kbuf_resolve_cov(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o,
+                                   hipStream_t s) {
+  if (c.cov < 2u || c.cov > 8u || !kb.mask) return hipErrorInvalidValue;
+  return dispatch_k<KBufResolveCov>(c.K, c, kb, tb, o, s);
+}
+
+hipError_t launch_finalize_srgb8(const float4* in, uint32_t W, uint32_t H, uint32_t S, uint32_t ss,
+                                 const CodecTables* tab, uint32_t* out, hipStream_t s) {
+  const uint64_t P = (uint64_t)W * H;
+  if (P == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_finalize_srgb8, dim3(blocks_for(P)), dim3(kBlock), 0, s, in, W, H, S, ss, tab, out);
+  return hipGetLastError();
 }
 
 }  // namespace oit

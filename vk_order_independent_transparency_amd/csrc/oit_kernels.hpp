@@ -15,6 +15,7 @@ struct FragView {
   const float4* rgba;
   uint32_t count;    // fragments in this batch
   uint32_t id_base;  // frame-local id of the first fragment
+  const uint32_t* mask = nullptr;  // coverage shading: sample mask per fragment (gl_SampleMaskIn)
 };
 
 // Colour-pass tail lists (per target, descending fragment id: newest first).
@@ -25,6 +26,7 @@ struct TailBuf {
   uint64_t* rec;      // [Fcap] (target << 32) | ((id + 1) << 1 | boundary), kEmpty64 = not in the tail
   uint32_t* counter;  // tail fragments of the frame (counted by the link kernel)
   uint32_t cap;       // Fcap
+  uint32_t* mask = nullptr;  // [Fcap] coverage shading: samples the ROP blend writes
 };
 
 // Per-algorithm A-buffer views.
@@ -35,6 +37,7 @@ struct LLBuf {
   uint64_t cap;       // nodes including the terminator
   uint32_t* overflow; // LL fragments that found the pool full
   uint32_t* counters; // frame counters [16]; the resetting resolve moves them to counters[16..31]
+  uint32_t* mask = nullptr;  // [cap] coverage shading: the node's sample mask
 };
 
 struct KBuf {          // Simple / Spinlock
@@ -44,6 +47,7 @@ struct KBuf {          // Simple / Spinlock
   uint32_t* col;      // [K][T]
   uint32_t* key;      // [K][T]
   uint32_t* id;       // [K][T]
+  uint32_t* mask = nullptr;  // [K][T] coverage shading
 };
 
 struct WBBuf {
@@ -64,6 +68,7 @@ struct Common {
   int tail;                // tail blending on
   float4 background;
   const CodecTables* tab;
+  uint32_t cov = 0;        // coverage shading: samples per pixel (0 = off)
 };
 
 // One frame of a batch as the batched Linked List kernels see it.
@@ -124,6 +129,13 @@ hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& w
 hipError_t launch_wboit_resolve(const Common& c, const WBBuf& wb, const ResolveOut& o, hipStream_t s);
 
 hipError_t launch_tail_link(const TailBuf& tb, uint32_t frags, hipStream_t s);
+// coverage shading (Common::cov samples per pixel, masks in the buffers)
+hipError_t launch_ll_resolve_cov(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o,
+                                 hipStream_t s);
+hipError_t launch_kbuf_resolve_cov(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o,
+                                   hipStream_t s);
+hipError_t launch_finalize_srgb8(const float4* in, uint32_t W, uint32_t H, uint32_t S, uint32_t ss,
+                                 const CodecTables* tab, uint32_t* out, hipStream_t s);
 
 hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t hi, unsigned long long* mismatches,
                                  uint32_t* first_bad, hipStream_t s);

@@ -78,13 +78,13 @@ class FrameConfig:
     wboit_view_near: float = 2.0
     wboit_view_span: float = 8.0
     variant: int = 0
+    coverage_samples: int = 0
 
     @classmethod
     def from_state(cls, state: State, width: int, height: int, **kw) -> "FrameConfig":
         state.recomputeAntialiasingSettings()
-        if state.coverageShading():
-            raise ValueError("coverage shading (MSAA without sample shading) is not provided; use an SSAA mode")
         w, h = width * state.supersample, height * state.supersample
+        cov = state.msaa if state.coverageShading() else 0  # OIT_COVERAGE_SHADING, oit.h:82
         return cls(
             algorithm=state.algorithm,
             layers=state.oitLayers,
@@ -93,6 +93,7 @@ class FrameConfig:
             height=h,
             samples=state.msaa if state.sampleShading else 1,
             ll_nodes_per_pixel=state.linkedListAllocatedPerElement,
+            coverage_samples=cov,
             **kw,
         )
 
@@ -116,6 +117,8 @@ class FrameConfig:
         c.wboit_view_near = self.wboit_view_near
         c.wboit_view_span = self.wboit_view_span
         c.variant = self.variant
+        c.coverage_samples = self.coverage_samples
+        c.reserved = 0
         return c
 
 
@@ -182,9 +185,9 @@ class Renderer:
     def clear(self, stream=None) -> None:
         _abi.check("oit_clear", self._lib.oit_clear(self._ctx, _stream_ptr(stream)))
 
-    def append(self, target, depth, rgba, stream=None, count=None) -> None:
+    def append(self, target, depth, rgba, stream=None, count=None, coverage=None) -> None:
         """target u32 [F], depth f32 [F], rgba f32 [F, 4] premultiplied: DeviceBuffers, or raw
-        device addresses with `count`."""
+        device addresses with `count`; `coverage` u32 [F] sample masks with coverage shading."""
         if count is None:
             n = _rows(target)
             if _rows(depth) != n or _rows(rgba) != n or (n and tuple(rgba.shape[1:]) != (4,)):
@@ -192,7 +195,8 @@ class Renderer:
         else:
             n = int(count)
         f = _abi.OitFragments(_dptr(target) if n else None, _dptr(depth) if n else None,
-                              _dptr(rgba) if n else None, n)
+                              _dptr(rgba) if n else None, n,
+                              _dptr(coverage) if (n and coverage is not None) else None)
         _abi.check("oit_append", self._lib.oit_append(self._ctx, ctypes.byref(f), _stream_ptr(stream)))
 
     def resolve(self, out, stream=None) -> None:
@@ -234,11 +238,17 @@ class Renderer:
         _abi.check("oit_profile_read", self._lib.oit_profile_read(self._ctx, ctypes.byref(out)))
         return {name: (out.ms[i], out.launches[i]) for i, name in enumerate(_abi.PROFILE_STAGE_NAMES)}
 
-    def render(self, target, depth, rgba, out, stream=None, count=None) -> None:
+    def render(self, target, depth, rgba, out, stream=None, count=None, coverage=None) -> None:
         """One frame: clear, one append, resolve (the reference's onRender order, oitRender.cpp:43-148)."""
         self.clear(stream)
-        self.append(target, depth, rgba, stream, count)
+        self.append(target, depth, rgba, stream, count, coverage)
         self.resolve(out, stream)
+
+    def finalize_srgb8(self, image, out, width, height, samples=1, supersample=1, stream=None) -> None:
+        """Display image (oit_finalize_srgb8): `image` premultiplied float4
+        [samples][height*supersample][width*supersample], `out` uint8 [height, width, 4]."""
+        _abi.check("oit_finalize_srgb8", self._lib.oit_finalize_srgb8(self._ctx, _dptr(image), width, height, samples,
+                                                                     supersample, _dptr(out), _stream_ptr(stream)))
 
     def _check_out(self, out) -> None:
         if self.config is None:
