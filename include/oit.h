@@ -40,10 +40,10 @@ extern "C" {
 enum oit_algorithm {
   OIT_SIMPLE     = 0,
   OIT_LINKEDLIST = 1,
-  OIT_LOOP       = 2, /* not provided: OIT_ERR_UNSUPPORTED */
+  OIT_LOOP       = 2, /* two geometry passes: the fragment buffers are read again by oit_resolve */
   OIT_LOOP64     = 3,
   OIT_SPINLOCK   = 4,
-  OIT_INTERLOCK  = 5, /* not provided: OIT_ERR_UNSUPPORTED */
+  OIT_INTERLOCK  = 5, /* ordered critical sections, replayed in primitive order */
   OIT_WEIGHTED   = 6
 };
 
@@ -347,7 +347,7 @@ typedef struct oit_server_config {
 
 typedef struct oit_request {
   oit_scene_params scene;
-  uint32_t algorithm;         /* OIT_SIMPLE, OIT_LINKEDLIST, OIT_LOOP64, OIT_SPINLOCK, OIT_WEIGHTED */
+  uint32_t algorithm;         /* any oit_algorithm */
   uint32_t layers;            /* K */
   uint32_t tail_blend;
   uint32_t reserved;          /* 0 */

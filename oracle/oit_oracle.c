@@ -351,7 +351,8 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
   if (cov) {
     if (cov != 2 && cov != 4 && cov != 8) return -1;
     if (cfg->samples != 1 || !mask) return -1;
-    if (cfg->algorithm != ORC_SIMPLE && cfg->algorithm != ORC_LINKEDLIST && cfg->algorithm != ORC_SPINLOCK)
+    if (cfg->algorithm != ORC_SIMPLE && cfg->algorithm != ORC_LINKEDLIST && cfg->algorithm != ORC_SPINLOCK &&
+        cfg->algorithm != ORC_INTERLOCK)
       return -1; /* Loop and Loop64 have no coverage shading (README.md:33-34) */
   }
   const uint32_t full = cov ? (1u << cov) - 1u : 0xFFFFFFFFu;
@@ -367,7 +368,9 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
   res->tail_fragments = 0;
   if (res->tail_flag) memset(res->tail_flag, 0, F);
   int rc = 0;
-  const int canonical = cfg->tail_order == ORC_TAIL_CANONICAL;
+  /* the ordered interlock blends its tail in primitive order by construction
+   * (oitInterlock.frag.glsl:52-79 ordered critical section + ROP order) */
+  const int canonical = cfg->tail_order == ORC_TAIL_CANONICAL && cfg->algorithm != ORC_INTERLOCK;
 
   if (cfg->algorithm == ORC_LINKEDLIST) {
     /* ---- oitLinkedList.frag.glsl ---- */
@@ -435,8 +438,11 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
     free(a);
   ll_free:
     free(head); free(node); free(next);
-  } else if (cfg->algorithm == ORC_SIMPLE || cfg->algorithm == ORC_SPINLOCK) {
-    const int spin = cfg->algorithm == ORC_SPINLOCK;
+  } else if (cfg->algorithm == ORC_SIMPLE || cfg->algorithm == ORC_SPINLOCK || cfg->algorithm == ORC_INTERLOCK) {
+    /* Interlock (oitInterlock.frag.glsl:103-148) runs the Spinlock's critical
+     * section under the fragment shader interlock: one invocation per pixel at
+     * a time, in primitive order when ordered (oit.h:71) */
+    const int spin = cfg->algorithm == ORC_SPINLOCK || cfg->algorithm == ORC_INTERLOCK;
     uint32_t* count = (uint32_t*)calloc(T, sizeof(uint32_t)); /* imgAux cleared to 0 */
     uint32_t* auxd = (uint32_t*)malloc(T * sizeof(uint32_t)); /* imgAuxDepth cleared to ~0, oitRender.cpp:348 */
     elem*     ab = (elem*)malloc(T * K * sizeof(elem));
@@ -488,7 +494,9 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
         if (canonical || !tail_full) orc_decode_premult(tail_e.col, c4);
         else memcpy(c4, &rgba[4 * i], sizeof c4);
         uint32_t pos = canonical ? tail_e.id : (uint32_t)i;
-        if ((rc = tail_push_m(&tails, pos, t, tail_e.mask, c4))) goto kb_free;
+        /* canonical: the tail fragment at its own position with its own samples;
+         * sequential: the evicting invocation's output, written to its samples */
+        if ((rc = tail_push_m(&tails, pos, t, canonical ? tail_e.mask : x.mask, c4))) goto kb_free;
         if (res->tail_flag) res->tail_flag[tail_e.id] = 1;
         res->tail_fragments++;
       }
@@ -516,6 +524,70 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
     }
   kb_free:
     free(count); free(auxd); free(ab);
+  } else if (cfg->algorithm == ORC_LOOP) {
+    /* ---- oitLoop.frag.glsl: depth pass (:57-100), colour pass (:105-180),
+     * composite (:186-222); buffers cleared to all ones (depths) ---- */
+    uint32_t* zs = (uint32_t*)malloc(T * K * sizeof(uint32_t));
+    uint64_t* cs = (uint64_t*)calloc(T * K, sizeof(uint64_t)); /* (id + 1) << 32 | colour of the last writer */
+    if (!zs || !cs) { free(zs); free(cs); rc = -2; goto done; }
+    for (uint64_t j = 0; j < T * K; ++j) zs[j] = EMPTY_U32;
+    for (uint64_t i = 0; i < F; ++i) { /* depth pass: the K nearest distinct depths, sorted */
+      const uint64_t base = (uint64_t)target[i] * K;
+      uint32_t z = orc_depth_key(depth[i]);
+      uint32_t j = 0;
+      if (z > zs[base + K - 1]) continue;    /* :75-80 pretest */
+      if (z > zs[base + K / 2]) j = K / 2;   /* :82-87 */
+      for (; j < K; ++j) {                   /* :89-99 atomicMin chain */
+        const uint32_t old = zs[base + j];
+        if (z < old) zs[base + j] = z;
+        if (old == EMPTY_U32 || old == z) break;
+        z = old > z ? old : z;
+      }
+    }
+    for (uint64_t i = 0; i < F; ++i) { /* colour pass */
+      const uint64_t base = (uint64_t)target[i] * K;
+      const uint32_t z = orc_depth_key(depth[i]);
+      if (zs[base + K - 1] < z) {            /* :139-150 behind the K-th depth: tail-blend itself */
+        if (cfg->tail_blend) {
+          if ((rc = tail_push(&tails, (uint32_t)i, target[i], &rgba[4 * i]))) { free(zs); free(cs); goto done; }
+          if (res->tail_flag) res->tail_flag[i] = 1;
+          res->tail_fragments++;
+        }
+        continue;
+      }
+      int start = 0, end = (int)K - 1;       /* :154-175 binary search of its depth */
+      while (start < end) {
+        const int mid = (start + end) / 2;
+        if (zs[base + (uint32_t)mid] < z) start = mid + 1; else end = mid;
+      }
+      cs[base + (uint32_t)start] = ((uint64_t)(i + 1) << 32) | orc_encode(&rgba[4 * i]); /* :177, later writes win */
+    }
+    if ((rc = fold_tail(&tails, colorbuf, T, canonical))) { free(zs); free(cs); goto done; }
+    elem* a = (elem*)malloc(K * sizeof(elem));
+    if (!a) { free(zs); free(cs); rc = -2; goto done; }
+    for (uint64_t t = 0; t < T; ++t) {
+      float c[4] = {0, 0, 0, 0};
+      int n = 0;
+      for (uint32_t j = 0; j < K; ++j) {     /* :198-211 */
+        if (zs[t * K + j] == EMPTY_U32) break;
+        ++n;
+      }
+      for (int j = 0; j < n; ++j) {          /* :213-217 */
+        const uint64_t s = cs[t * K + (uint32_t)j];
+        blend_packed(c, (uint32_t)(s & 0xFFFFFFFFu));
+        a[j].key = zs[t * K + (uint32_t)j]; a[j].col = (uint32_t)(s & 0xFFFFFFFFu);
+        a[j].id = (uint32_t)(s >> 32) - 1u; a[j].mask = 0;
+      }
+      if (res->composite) memcpy(&res->composite[4 * t], c, sizeof c);
+      if (res->count) res->count[t] = (uint32_t)n;
+      write_frontk(res, t, K, a, n, 1);
+      float o[4];
+      memcpy(o, &colorbuf[4 * t], sizeof o);
+      rop_blend(o, c);
+      memcpy(&res->out[4 * t], o, sizeof o);
+    }
+    free(a);
+    free(zs); free(cs);
   } else if (cfg->algorithm == ORC_LOOP64) {
     /* ---- oitLoop64.frag.glsl ---- */
     uint64_t* slot = (uint64_t*)malloc(T * K * sizeof(uint64_t)); /* cleared to all ones, oitRender.cpp:307 */

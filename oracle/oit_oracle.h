@@ -39,7 +39,8 @@
 extern "C" {
 #endif
 
-enum { ORC_SIMPLE = 0, ORC_LINKEDLIST = 1, ORC_LOOP64 = 3, ORC_SPINLOCK = 4, ORC_WEIGHTED = 6 };
+enum { ORC_SIMPLE = 0, ORC_LINKEDLIST = 1, ORC_LOOP = 2, ORC_LOOP64 = 3, ORC_SPINLOCK = 4, ORC_INTERLOCK = 5,
+       ORC_WEIGHTED = 6 };
 enum { ORC_TAIL_CANONICAL = 0, ORC_TAIL_SEQUENTIAL = 1 };
 
 typedef struct orc_config {

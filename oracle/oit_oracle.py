@@ -13,7 +13,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liboit_oracle.so")
 
-SIMPLE, LINKEDLIST, LOOP64, SPINLOCK, WEIGHTED = 0, 1, 3, 4, 6
+SIMPLE, LINKEDLIST, LOOP, LOOP64, SPINLOCK, INTERLOCK, WEIGHTED = 0, 1, 2, 3, 4, 5, 6
 TAIL_CANONICAL, TAIL_SEQUENTIAL = 0, 1
 
 

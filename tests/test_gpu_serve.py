@@ -1,6 +1,7 @@
 """The request loop (oit_server_*): every request's image is the same when
 it runs alone and when it runs in a mix of concurrent requests (bit-exact
-for the Linked List and Loop64, within 1e-5 relative for WBOIT, whose
+for the Linked List, Loop64 and the ordered interlock, within 1e-5
+relative for WBOIT, whose
 float atomics accumulate in arrival order), the Linked List result matches
 the oracle, the HBM budget is respected (idle contexts trimmed, oversized
 requests refused) and bad requests are rejected at submission."""
@@ -21,7 +22,7 @@ def _say(*a):
 def _requests(n=14, seed=3):
     reqs, _ = serve.served_workload(seed, n, 1000.0, resolutions=SMALL_RES, spheres=(40, 400),
                                     algorithms=(_abi.OIT_LINKEDLIST, _abi.OIT_LOOP64, _abi.OIT_WEIGHTED,
-                                                _abi.OIT_SIMPLE),
+                                                _abi.OIT_SIMPLE, _abi.OIT_LOOP, _abi.OIT_INTERLOCK),
                                     layers=(4, 8, 16, 32))
     return reqs
 
@@ -55,8 +56,9 @@ def test_mix_equals_alone():
         got = outs[i].numpy()
         if r.algorithm == _abi.OIT_WEIGHTED:
             assert np.allclose(got, alone[i], rtol=1e-5, atol=1e-7), i
-        elif r.algorithm == _abi.OIT_SIMPLE:
-            # Simple keeps the first K arrivals (README.md:23): only pixels within K agree exactly
+        elif r.algorithm in (_abi.OIT_SIMPLE, _abi.OIT_LOOP):
+            # Simple keeps the first K arrivals (README.md:23) and Loop32's shared depths keep the last
+            # colour stored: only where no pixel depends on arrival order do they agree exactly
             pass
         else:
             assert np.array_equal(got.view(np.uint32), alone[i].view(np.uint32)), i
@@ -91,6 +93,10 @@ def test_budget_is_respected():
             need.append(8 * r.layers * T + 4 * T + 28 * F)
         elif r.algorithm == _abi.OIT_SIMPLE:
             need.append(4 * T + 12 * r.layers * T + 4 * T + 28 * F)
+        elif r.algorithm == _abi.OIT_LOOP:
+            need.append(12 * r.layers * T + 4 * T + 28 * F)
+        elif r.algorithm == _abi.OIT_INTERLOCK:
+            need.append(4 * T + 28 * (F + 1))
         else:
             need.append(20 * T)
     budget = int(sorted(need)[-1] * 1.2)  # room for the largest request, not for all at once
@@ -112,5 +118,5 @@ def test_bad_requests_are_rejected():
             s.submit(r)
         assert e.value.status == _abi.OIT_ERR_INVALID_ARG
         with pytest.raises(_abi.OitError) as e:
-            s.submit(serve.Request(r.scene, _abi.OIT_LOOP, 8))
-        assert e.value.status == _abi.OIT_ERR_UNSUPPORTED
+            s.submit(serve.Request(r.scene, 9, 8))  # no such algorithm
+        assert e.value.status == _abi.OIT_ERR_INVALID_ARG

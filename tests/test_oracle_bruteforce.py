@@ -116,3 +116,29 @@ def test_weighted_is_order_independent_to_rounding(oracle):
         perm = list(perm)
         o = oracle.run(cfg, tg, d[perm], c[perm])["out"]
         assert np.abs(o - ref).max() < 1e-6
+
+
+def test_loop32_matches_loop64_front_for_distinct_depths(oracle):
+    """With distinct depths Loop32 keeps the same K nearest as Loop64 and composites the same
+    packed colours, for any arrival order (README.md:74)."""
+    import itertools
+
+    rng = np.random.default_rng(71)
+    for trial in range(40):
+        n = int(rng.integers(1, 7))
+        d = rng.permutation(np.linspace(0.05, 0.95, n)).astype(np.float32)
+        a = rng.uniform(0.1, 0.8, n).astype(np.float32)
+        rgb = rng.uniform(0, 1, (n, 3)).astype(np.float32)
+        c = np.concatenate([rgb * a[:, None], a[:, None]], 1).astype(np.float32)
+        tg = np.zeros(n, np.uint32)
+        K = int(rng.choice([1, 2, 4]))
+        comp = None
+        for perm in itertools.islice(itertools.permutations(range(n)), 24):
+            p = np.array(perm)
+            r32 = oracle.run(oracle.make_config(oracle.LOOP, K, 1, 1, 1), tg[p], d[p], c[p])
+            r64 = oracle.run(oracle.make_config(oracle.LOOP64, K, 1, 1, 1), tg[p], d[p], c[p])
+            assert np.array_equal(r32["frontk_key"], r64["frontk_key"])
+            assert np.array_equal(r32["composite"], r64["composite"])
+            if comp is None:
+                comp = r32["composite"]
+            assert np.array_equal(r32["composite"], comp)  # stable across arrival orders

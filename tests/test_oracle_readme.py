@@ -168,3 +168,38 @@ def test_equal_depth_ties_by_fragment_id(oracle):
         assert list(r["frontk_val"][0][:3]) == [0, 1, 2]
         comp = over(over(q(oracle, C[1]), q(oracle, C[2])), q(oracle, C[3]))
         assert np.array_equal(r["composite"][0], comp)
+
+
+def test_loop32_readme_example(oracle):
+    """README.md:72: K=2, the Simple example's four fragments: the depth pass keeps (0.1, 0.2),
+    the colour pass matches c1 and c2 to them, and (c4,.4), (c3,.3) tail-blend in primitive
+    order with their shaded colours."""
+    tg, d, c = stream([(4, .4), (2, .2), (1, .1), (3, .3)])
+    r = oracle.run(oracle.make_config(oracle.LOOP, 2, 1, 1, 1), tg, d, c)
+    assert list(r["frontk_key"][0]) == [oracle.depth_key(.1), oracle.depth_key(.2)]
+    assert list(r["frontk_val"][0]) == [oracle.encode(premult(C[1])), oracle.encode(premult(C[2]))]
+    assert list(r["tail_flag"]) == [1, 0, 0, 1]
+    assert np.array_equal(r["colorbuf"][0], over(over(premult(C[3]), premult(C[4])), BG))
+    assert np.array_equal(r["composite"][0], over(q(oracle, C[1]), q(oracle, C[2])))
+
+
+def test_loop32_shared_depth_keeps_the_last_colour(oracle):
+    """Two fragments at one depth share one depth slot; the colour store of the later one
+    wins (primitive order) and the earlier colour is neither stored nor tail-blended."""
+    tg, d, c = stream([(1, .3), (2, .3), (3, .6)])
+    r = oracle.run(oracle.make_config(oracle.LOOP, 2, 1, 1, 1), tg, d, c)
+    assert list(r["frontk_key"][0]) == [oracle.depth_key(.3), oracle.depth_key(.6)]
+    assert list(r["frontk_val"][0]) == [oracle.encode(premult(C[2])), oracle.encode(premult(C[3]))]
+    assert list(r["tail_flag"]) == [0, 0, 0]
+
+
+def test_interlock_is_the_spinlock_in_primitive_order(oracle):
+    """README.md:86-96 trace as an ordered interlock: the critical sections run in primitive
+    order, and each invocation's tail output is blended at its own position."""
+    tg, d, c = stream([(3, .3), (2, .2), (4, .4), (1, .1)])
+    r = oracle.run(oracle.make_config(oracle.INTERLOCK, 2, 1, 1, 1), tg, d, c)
+    assert list(r["frontk_val"][0]) == [3, 1]
+    assert list(r["tail_flag"]) == [1, 0, 1, 0]
+    assert np.array_equal(r["colorbuf"][0], over(q(oracle, C[3]), over(premult(C[4]), BG)))
+    rs = oracle.run(oracle.make_config(oracle.SPINLOCK, 2, 1, 1, 1, tail_order=oracle.TAIL_SEQUENTIAL), tg, d, c)
+    assert np.array_equal(rs["out"], r["out"])

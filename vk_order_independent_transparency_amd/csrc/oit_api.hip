@@ -135,6 +135,10 @@ struct oit_ctx {
   uint64_t cap_l64 = 0, cap_wb = 0, cap_tl_head = 0;
   // coverage shading: sample masks of nodes, k-buffer slots and tail entries
   uint32_t* ll_mask = nullptr;
+  float4* il_full = nullptr;   // ordered interlock: shaded colour per node [ll_alloc]
+  uint32_t* l32_z = nullptr;   // Loop32 depth slots [K][T]
+  uint64_t* l32_c = nullptr;   // Loop32 colour slots [K][T]: (id + 1) << 32 | colour
+  uint64_t cap_il_full = 0, cap_l32 = 0;
   uint32_t* kb_mask = nullptr;
   uint32_t* tl_mask = nullptr;
   uint64_t cap_ll_mask = 0, cap_kb_mask = 0, cap_tl_mask = 0;
@@ -225,6 +229,10 @@ void free_frame_buffers(oit_ctx* c) {
   c->cap_l64 = c->cap_wb = c->cap_tl_head = 0;
   c->cap_ll_mask = c->cap_kb_mask = c->cap_tl_mask = 0;
   dev_free(&c->ll_mask);
+  dev_free(&c->il_full);
+  dev_free(&c->l32_z);
+  dev_free(&c->l32_c);
+  c->cap_il_full = c->cap_l32 = 0;
   dev_free(&c->kb_mask);
   dev_free(&c->tl_mask);
   dev_free(&c->ll_head);
@@ -307,7 +315,13 @@ oit::TailBuf tail_of(const oit_ctx* c) {
   return t;
 }
 
-bool uses_tail_lists(const oit_ctx* c) { return c->cfg.tail_blend && c->cfg.algorithm != OIT_WEIGHTED; }
+// The Linked List and the ordered interlock keep fragments in the node pool.
+bool pooled(uint32_t alg) { return alg == OIT_LINKEDLIST || alg == OIT_INTERLOCK; }
+
+// The ordered interlock blends its tail in the resolve, in primitive order.
+bool uses_tail_lists(const oit_ctx* c) {
+  return c->cfg.tail_blend && c->cfg.algorithm != OIT_WEIGHTED && c->cfg.algorithm != OIT_INTERLOCK;
+}
 
 // Makes the per-fragment tail arrays hold `need` fragments, keeping this
 // frame's records (stream-ordered copies; the old buffers are freed after a
@@ -366,6 +380,66 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
   c->cap_tl_mask = nm ? cap : 0;
   c->tl_cap = (uint32_t)cap;
   return fe == hipSuccess ? OIT_OK : status_of(fe);
+}
+
+// Grows the ordered interlock's node pool (it holds every fragment of the
+// frame) to at least `need` nodes, keeping this frame's nodes: stream-ordered
+// copies, new links kUnset, the old buffers freed after the stream drains.
+int ensure_pool_nodes(oit_ctx* c, uint64_t need, hipStream_t s) {
+  if (need <= c->ll_alloc) return OIT_OK;
+  if (need > kMaxPoolNodes) return OIT_ERR_CAPACITY;
+  uint64_t cap = c->ll_alloc ? c->ll_alloc : 1024;
+  while (cap < need) cap *= 2;
+  if (cap > kMaxPoolNodes) cap = kMaxPoolNodes;
+  const bool cov = coverage(c);
+  uint64_t* ncd = nullptr;
+  uint32_t* nnext = nullptr;
+  float4* nfull = nullptr;
+  uint32_t* nmask = nullptr;
+  int rc = dev_alloc(&ncd, cap);
+  if (!rc) rc = dev_alloc(&nnext, cap);
+  if (!rc) rc = dev_alloc(&nfull, cap);
+  if (!rc && cov) rc = dev_alloc(&nmask, cap);
+  if (rc) {
+    dev_free(&ncd);
+    dev_free(&nnext);
+    dev_free(&nfull);
+    dev_free(&nmask);
+    return rc;
+  }
+  const uint64_t old = c->ll_alloc;
+  hipError_t e = hipSuccess;
+  if (old) {
+    e = hipMemcpyAsync(ncd, c->ll_cd, old * sizeof(uint64_t), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(nnext, c->ll_next, old * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess && c->il_full)
+      e = hipMemcpyAsync(nfull, c->il_full, old * sizeof(float4), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess && nmask && c->ll_mask)
+      e = hipMemcpyAsync(nmask, c->ll_mask, old * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
+  }
+  if (e == hipSuccess) e = oit::launch_fill_u32(nnext + old, cap - old, oit::kUnset, s);
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  if (e != hipSuccess) {
+    dev_free(&ncd);
+    dev_free(&nnext);
+    dev_free(&nfull);
+    dev_free(&nmask);
+    return status_of(e);
+  }
+  wait_idle(c);
+  dev_free(&c->ll_cd);
+  dev_free(&c->ll_next);
+  dev_free(&c->il_full);
+  dev_free(&c->ll_mask);
+  c->ll_cd = ncd;
+  c->ll_next = nnext;
+  c->il_full = nfull;
+  c->ll_mask = nmask;
+  c->ll_alloc = c->ll_cap = cap;
+  c->cap_il_full = cap;
+  c->cap_ll_mask = nmask ? cap : 0;
+  c->bytes_abuf = cap * ((cov ? 16u : 12u) + 16u);
+  return OIT_OK;
 }
 
 }  // namespace
@@ -436,9 +510,7 @@ int oit_destroy(oit_ctx* c) {
 int oit_configure(oit_ctx* c, const oit_config* cfg) {
   if (!c || !cfg) return OIT_ERR_INVALID_ARG;
   const uint32_t alg = cfg->algorithm;
-  if (alg == OIT_LOOP || alg == OIT_INTERLOCK) return OIT_ERR_UNSUPPORTED;
-  if (alg != OIT_SIMPLE && alg != OIT_LINKEDLIST && alg != OIT_LOOP64 && alg != OIT_SPINLOCK && alg != OIT_WEIGHTED)
-    return OIT_ERR_INVALID_ARG;
+  if (alg > OIT_WEIGHTED) return OIT_ERR_INVALID_ARG;
   if (alg != OIT_WEIGHTED && !valid_layers(cfg->layers)) return OIT_ERR_INVALID_ARG;
   if (cfg->width == 0 || cfg->height == 0) return OIT_ERR_INVALID_ARG;
   const uint32_t S = cfg->samples ? cfg->samples : 1;
@@ -451,11 +523,11 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   if (cov > 1) {  // OIT_COVERAGE_SHADING
     if (cov != 2 && cov != 4 && cov != 8) return OIT_ERR_INVALID_ARG;
     if (S != 1) return OIT_ERR_INVALID_ARG;  // per-pixel fragments
-    if (alg == OIT_LOOP64 || alg == OIT_WEIGHTED) return OIT_ERR_UNSUPPORTED;  // README.md:33-34
+    if (alg == OIT_LOOP || alg == OIT_LOOP64 || alg == OIT_WEIGHTED) return OIT_ERR_UNSUPPORTED;  // README.md:33-34
     if (alg == OIT_LINKEDLIST && (cfg->variant & OIT_VARIANT_LL_SPLIT_APPEND)) return OIT_ERR_UNSUPPORTED;
   }
   uint64_t ll_cap = 0;
-  if (alg == OIT_LINKEDLIST) {  // oit.cpp:119-126
+  if (pooled(alg)) {  // oit.cpp:119-126 (the interlock's pool also grows to the frame's fragments)
     ll_cap = cfg->ll_pool_nodes;
     if (ll_cap == 0) ll_cap = (uint64_t)(cfg->ll_nodes_per_pixel ? cfg->ll_nodes_per_pixel : 10) * T;
     if (ll_cap < 1) ll_cap = 1;
@@ -469,7 +541,7 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   c->T = (uint32_t)T;
   const uint32_t K = c->cfg.layers;
   // buffers of the other algorithms go; this algorithm's are reused if they fit
-  if (alg != OIT_LINKEDLIST) {
+  if (!pooled(alg)) {
     release(c, &c->ll_head, &c->cap_ll_head);
     uint64_t dummy = c->ll_alloc;
     release(c, &c->ll_cd, &dummy);
@@ -487,7 +559,13 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
     release(c, &c->kb_lock, &c->cap_kb_lock);
     release(c, &c->kb_depth, &c->cap_kb_depth);
   }
-  if (alg != OIT_LINKEDLIST) release(c, &c->ll_mask, &c->cap_ll_mask);
+  if (!pooled(alg)) release(c, &c->ll_mask, &c->cap_ll_mask);
+  if (alg != OIT_INTERLOCK) release(c, &c->il_full, &c->cap_il_full);
+  if (alg != OIT_LOOP) {
+    uint64_t d = c->cap_l32;
+    release(c, &c->l32_z, &d);
+    release(c, &c->l32_c, &c->cap_l32);
+  }
   if (alg != OIT_SIMPLE && alg != OIT_SPINLOCK) release(c, &c->kb_mask, &c->cap_kb_mask);
   if (alg != OIT_LOOP64) release(c, &c->l64, &c->cap_l64);
   if (alg != OIT_WEIGHTED) {
@@ -498,18 +576,30 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   int rc = OIT_OK;
   bool ll_grown = false;
   switch (alg) {
-    case OIT_LINKEDLIST: {
+    case OIT_LINKEDLIST:
+    case OIT_INTERLOCK: {
       rc = reserve(c, &c->ll_head, &c->cap_ll_head, T, &ll_grown);
       if (!rc && ll_cap > c->ll_alloc) {  // node payloads and links share one capacity
         uint64_t d = c->ll_alloc;
         rc = reserve(c, &c->ll_cd, &d, ll_cap, &ll_grown);
         if (!rc) rc = reserve(c, &c->ll_next, &c->ll_alloc, ll_cap, &ll_grown);
       }
-      if (!rc && cov > 1) rc = reserve(c, &c->ll_mask, &c->cap_ll_mask, ll_cap);
+      if (!rc && cov > 1) rc = reserve(c, &c->ll_mask, &c->cap_ll_mask, c->ll_alloc);
       if (!rc && cov <= 1) release(c, &c->ll_mask, &c->cap_ll_mask);
-      c->ll_cap = ll_cap;
-      c->bytes_abuf = ll_cap * (cov > 1 ? 16u : 12u);
+      if (!rc && alg == OIT_INTERLOCK) rc = reserve(c, &c->il_full, &c->cap_il_full, c->ll_alloc);
+      c->ll_cap = alg == OIT_INTERLOCK ? c->ll_alloc : ll_cap;
+      c->bytes_abuf = c->ll_cap * ((cov > 1 ? 16u : 12u) + (alg == OIT_INTERLOCK ? 16u : 0u));
       c->bytes_aux = T * 4u;
+      break;
+    }
+    case OIT_LOOP: {  // oit.cpp:127-133: depth and colour planes
+      if (T * K > c->cap_l32) {
+        uint64_t d = c->cap_l32;
+        rc = reserve(c, &c->l32_z, &d, T * K);
+        if (!rc) rc = reserve(c, &c->l32_c, &c->cap_l32, T * K);
+      }
+      c->bytes_abuf = T * K * 12u;
+      c->bytes_aux = 0;
       break;
     }
     case OIT_SIMPLE:
@@ -572,13 +662,13 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
     return rc;
   }
   c->tl_head_valid = false;
-  if (alg == OIT_LINKEDLIST && ll_grown) {
+  if (pooled(alg) && ll_grown) {
     // a new allocation is filled clean (every head 0, every link kUnset) by
     // the next clear, on its stream
     c->ll_clean = false;
     c->ll_dirty_upto = c->ll_alloc;
   }
-  if (alg != OIT_LINKEDLIST) {
+  if (!pooled(alg)) {
     c->ll_clean = false;
     c->ll_dirty_upto = 0;
   }
@@ -615,6 +705,8 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
   out->ll_capacity = c->ll_cap;
   switch (c->cfg.algorithm) {
     case OIT_LINKEDLIST: out->bytes_per_element = 12; break;
+    case OIT_INTERLOCK: out->bytes_per_element = 28; break;
+    case OIT_LOOP: out->bytes_per_element = 12; break;
     case OIT_SIMPLE:
     case OIT_SPINLOCK: out->bytes_per_element = 12; break;
     case OIT_LOOP64: out->bytes_per_element = 8; break;
@@ -624,7 +716,8 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
   out->resident_bytes = c->cap_ll_head * 4u + c->ll_alloc * 12u + c->cap_kb_count * 4u + c->cap_kb_lock * 4u +
                         c->cap_kb_depth * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_wb * 20u +
                         c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u) +
-                        (c->cap_ll_mask + c->cap_kb_mask + c->cap_tl_mask) * 4u;
+                        (c->cap_ll_mask + c->cap_kb_mask + c->cap_tl_mask) * 4u + c->cap_il_full * 16u +
+                        c->cap_l32 * 12u;
   return OIT_OK;
 }
 
@@ -634,7 +727,7 @@ int oit_clear(oit_ctx* c, void* stream) {
   DeviceGuard g(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   const uint64_t T = c->T, K = c->cfg.layers;
-  const bool ll = c->cfg.algorithm == OIT_LINKEDLIST;
+  const bool ll = pooled(c->cfg.algorithm);
   const bool need_ll_fill = ll && !c->ll_clean;
   const bool need_counters = !ll || !c->counters_clean;
   const bool need_tail = c->tl_head && (!ll || c->tl_next);
@@ -656,6 +749,11 @@ int oit_clear(oit_ctx* c, void* stream) {
     hipError_t r = hipSuccess;
     if (need_counters) r = oit::launch_fill_u32(c->d_counters, 16, 0u, s);
     switch (c->cfg.algorithm) {
+      case OIT_LOOP:  // depths to all ones (oitRender.cpp:242-265), colour slots to 0
+        if (r == hipSuccess) r = oit::launch_fill_u32(c->l32_z, T * K, kEmpty, s);
+        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->l32_c), 2 * T * K, 0u, s);
+        break;
+      case OIT_INTERLOCK:
       case OIT_LINKEDLIST:  // head and counter to 0, oitRender.cpp:206-212; links of a dirty range to kUnset
         // every allocated head: a larger earlier frame may have left some
         if (need_ll_fill && r == hipSuccess) r = oit::launch_fill_u32(c->ll_head, c->cap_ll_head, 0u, s);
@@ -744,6 +842,24 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       }
       break;
     }
+    case OIT_INTERLOCK: {  // nodes of every fragment; the critical sections run in the resolve
+      int rc = ensure_pool_nodes(c, end + 1, s);
+      if (rc) return rc;
+      oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      ll.mask = coverage(c) ? c->ll_mask : nullptr;
+      ll.full = c->il_full;
+      c->ll_clean = false;
+      c->counters_clean = false;
+      if (end + 1 > c->ll_dirty_upto) c->ll_dirty_upto = end + 1;
+      const oit::TailBuf none{};
+      e = staged(c, kStStore, s, [&] { return oit::launch_ll_append(k, v, ll, none, 4, s); });
+      break;
+    }
+    case OIT_LOOP: {  // depth pass now; the colour pass runs over the recorded batches at resolve
+      e = staged(c, kStStore, s, [&] { return oit::launch_loop32_depth(k, v, c->l32_z, s); });
+      c->segments.push_back(v);
+      break;
+    }
     case OIT_SIMPLE: {
       oit::KBuf kb{c->kb_count, nullptr, nullptr, c->kb_col, c->kb_key, c->kb_id};
       kb.mask = coverage(c) ? c->kb_mask : nullptr;
@@ -803,6 +919,12 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
       if (e != hipSuccess) return status_of(e);
     }
   }
+  if (link_tails && alg == OIT_LOOP) {  // the second geometry pass (oitLoop.frag.glsl:105-180)
+    for (const auto& v : c->segments) {
+      e = staged(c, kStClassify, s, [&] { return oit::launch_loop32_color(k, v, c->l32_z, c->l32_c, tb, s); });
+      if (e != hipSuccess) return status_of(e);
+    }
+  }
   if (link_tails && tb.head && tb.next) {
     e = staged(c, kStTailLink, s, [&] { return oit::launch_tail_link(tb, (uint32_t)c->frame_frags, s); });
     if (e != hipSuccess) return status_of(e);
@@ -843,6 +965,22 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     case OIT_LOOP64:
       e = staged(c, kStResolve, s, [&] { return oit::launch_loop64_resolve(k, c->l64, tbr, o, s); });
       break;
+    case OIT_LOOP:
+      e = staged(c, kStResolve, s, [&] { return oit::launch_loop32_resolve(k, c->l32_z, c->l32_c, tbr, o, s); });
+      break;
+    case OIT_INTERLOCK: {
+      oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      ll.mask = coverage(c) ? c->ll_mask : nullptr;
+      ll.full = c->il_full;
+      e = staged(c, kStResolve, s, [&] { return oit::launch_interlock_resolve(k, ll, o, s); });
+      if (e == hipSuccess) {
+        c->ll_clean = true;
+        c->ll_dirty_upto = 0;
+        c->counters_clean = true;
+        c->stats_moved = true;
+      }
+      break;
+    }
     case OIT_WEIGHTED: {
       oit::WBBuf wb{c->wb_accum, c->wb_reveal};
       e = staged(c, kStResolve, s, [&] { return oit::launch_wboit_resolve(k, wb, o, s); });

@@ -455,6 +455,7 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
     const uint32_t n = (uint32_t)(f.node0 + f.first + (uint64_t)u * 64u + lane);
     if (node[u]) ll.cd[n] = ((uint64_t)depth_key(dep[u]) << 32) | encode_color_lds(col[u], s_thr);
     if (node[u] && ll.mask) ll.mask[n] = m[u];  // storeMask, oitLinkedList.frag.glsl:79
+    if (node[u] && ll.full) ll.full[n] = col[u];
     const Run r = same_target_run(node[u] ? t[u] : kEmpty);
     if (node[u] && !r.start) st_relaxed(&ll.next[n], n - 1u);
     ins[u] = node[u] && r.end;
@@ -1010,6 +1011,264 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
   const float4 d = tb.head ? tail_fold_cov(tb, t, c.background, c.cov) : c.background;
   o.out[t] = over(col, d);
   write_debug<K>(o, t, a, n, count, false, false);
+}
+
+// ---------------------------------------------------------------------------
+// Loop32: oitLoop.frag.glsl (the geometry is drawn twice)
+// ---------------------------------------------------------------------------
+
+// Depth pass (:57-100): the K nearest distinct depth keys per target, sorted,
+// by a u32 atomicMin chain that stops at an empty slot or its own key; the two
+// pretests use plain loads (slots only decrease within the pass).
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, uint32_t* __restrict__ zs) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= f.count) return;
+  const uint32_t t = f.target[i];
+  if (t >= c.T) return;
+  uint32_t z = depth_key(f.depth[i]);
+  if (z > zs[(uint64_t)(K - 1) * c.T + t]) return;
+  int j = 0;
+  if (z > zs[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
+  for (; j < K; ++j) {
+    const uint32_t old =
+        __hip_atomic_fetch_min(&zs[(uint64_t)j * c.T + t], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == kEmpty || old == z) break;
+    z = old > z ? old : z;
+  }
+}
+
+// Colour pass (:105-180): behind the K-th depth -> tail-blend the shaded
+// colour; else the slot of its depth by binary search, where the colour of the
+// latest fragment in primitive order is kept ((id + 1) << 32 | colour,
+// atomicMax), the result of the reference's plain store in draw order.
+template <int K, bool kTail>
+__global__ __launch_bounds__(kBlock) void k_loop32_color(Common c, FragView f, const uint32_t* __restrict__ zs,
+                                                         uint64_t* __restrict__ cs, TailBuf tb) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, c.tab);
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  bool valid = i < f.count;
+  uint32_t t = valid ? f.target[i] : kEmpty;
+  if (t >= c.T) {
+    valid = false;
+    t = kEmpty;
+  }
+  bool tail = false;
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid) {
+    col = f.rgba[i];
+    const uint32_t z = depth_key(f.depth[i]);
+    if (zs[(uint64_t)(K - 1) * c.T + t] < z) {
+      tail = true;
+    } else {
+      int start = 0, end = K - 1;
+      while (start < end) {
+        const int mid = (start + end) / 2;
+        if (zs[(uint64_t)mid * c.T + t] < z) start = mid + 1;
+        else end = mid;
+      }
+      const uint64_t v = ((uint64_t)(f.id_base + i + 1u) << 32) | encode_color_lds(col, s_thr);
+      (void)__hip_atomic_fetch_max(&cs[(uint64_t)start * c.T + t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+  }
+  if constexpr (kTail) tail_record_own(tail, t, f.id_base + i, 0u, col, tb);
+}
+
+// Composite (:186-222): the layers up to the first empty depth, front to back,
+// over the colour buffer.
+template <int K, bool kColorTail, bool kDebug>
+__global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint32_t* __restrict__ zs,
+                                                           const uint64_t* __restrict__ cs, TailBuf tb,
+                                                           ResolveOut o) {
+  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
+  if (t >= c.T) return;
+  uint32_t z[K], col[K];
+  uint32_t n = 0;
+  bool alive = true;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    z[k] = zs[(uint64_t)k * c.T + t];
+    alive = alive && z[k] != kEmpty;
+    col[k] = alive ? (uint32_t)cs[(uint64_t)k * c.T + t] : 0u;
+    n += alive ? 1u : 0u;
+  }
+  float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((uint32_t)k < n) acc = over(acc, decode_premult(col[k], c.tab));
+  float4 d = c.background;
+  if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+  o.out[t] = over(acc, d);
+  if (kDebug) {
+    if (o.fk_key) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) o.fk_key[(uint64_t)t * K + k] = (uint32_t)k < n ? z[k] : kEmpty;
+    }
+    if (o.fk_val) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) o.fk_val[(uint64_t)t * K + k] = (uint32_t)k < n ? col[k] : kEmpty;
+    }
+    if (o.counts) o.counts[t] = n;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Ordered interlock: oitInterlock.frag.glsl:52-148 with pixel_interlock_ordered
+// ---------------------------------------------------------------------------
+
+// The critical sections of one pixel run one at a time in primitive order:
+// the fragments are in the pixel's list (Linked List append, newest first),
+// so the list is reversed and the k-buffer logic of :103-148 (early depth
+// bound, fill, replace the furthest) is replayed oldest first in registers;
+// each invocation's tail output (its shaded colour when rejected, the evicted
+// packed colour when it replaces one) is blended by the ROP in that same
+// order. The walk leaves the pool clean (heads 0, links kUnset).
+template <int K, bool kCov>
+__global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll, ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, c.tab);
+  if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
+    ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
+    ll.counters[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+    uint32_t p = ll.head[t];
+    if (p != 0u) ll.head[t] = 0u;
+    uint32_t prev = 0;
+    while (p != 0u && p < ll.cap) {  // reverse: oldest first
+      const uint32_t nx = ll.next[p];
+      ll.next[p] = prev;
+      prev = p;
+      p = nx;
+    }
+    uint64_t sk[K];
+    uint32_t scol[K], smask[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      sk[k] = kEmpty64;
+      scol[k] = 0u;
+      smask[k] = 0u;
+    }
+    uint32_t count = 0;
+    uint32_t bound = kEmpty;  // imgAuxDepth cleared to ~0, oitRender.cpp:348
+    float4 cb = c.background;
+    float4 cbs[kCov ? 8 : 1];
+#pragma unroll
+    for (int s = 0; s < (kCov ? 8 : 1); ++s) cbs[s] = c.background;
+    bool seen = false;
+    p = prev;
+    while (p != 0u && p < ll.cap) {
+      const uint32_t nx = ll.next[p];
+      ll.next[p] = kUnset;
+      const uint64_t cd = ll.cd[p];
+      const uint32_t key = (uint32_t)(cd >> 32);
+      const uint32_t col = (uint32_t)cd;
+      const uint64_t me = ((uint64_t)key << 32) | (p - 1u);
+      const uint32_t mm = kCov ? ll.mask[p] : 0xFFFFFFFFu;
+      bool tail = true;
+      bool own = true;
+      uint32_t tcol = 0u;
+      if (key <= bound) {  // :62-64 early depth
+        if (count < (uint32_t)K) {  // :69-75
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if ((uint32_t)k == count) {
+              sk[k] = me;
+              scol[k] = col;
+              smask[k] = mm;
+            }
+          }
+          tail = false;
+        } else {  // :78-101 the furthest (first index wins), replaced when behind
+          uint64_t mx = 0;
+          uint32_t fur = 0;
+#pragma unroll
+          for (int k = 0; k < K; ++k) {
+            if (sk[k] > mx) {
+              mx = sk[k];
+              fur = (uint32_t)k;
+            }
+          }
+          if (mx > me) {
+            own = false;
+#pragma unroll
+            for (int k = 0; k < K; ++k) {
+              if ((uint32_t)k == fur) {
+                tcol = scol[k];
+                sk[k] = me;
+                scol[k] = col;
+                smask[k] = mm;
+              }
+            }
+            bound = (uint32_t)(mx >> 32);
+          }
+        }
+        ++count;
+      }
+      if (tail && c.tail) {  // :106-108 the invocation's output, PREMULTIPLIED blend in primitive order
+        const float4 tc = own ? ll.full[p] : decode_premult_lds(tcol, s_dec, s_alpha);
+        if constexpr (kCov) {
+#pragma unroll
+          for (int s = 0; s < 8; ++s)
+            if ((uint32_t)s < c.cov && ((mm >> s) & 1u)) cbs[s] = over(tc, cbs[s]);
+          seen = true;
+        } else {
+          cb = over(tc, cb);
+        }
+      }
+      p = nx;
+    }
+    // composite (:176-230): the stored layers sorted by (depth key, id)
+    const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
+#pragma unroll
+    for (int i = 1; i < K; ++i) {
+#pragma unroll
+      for (int j = i; j > 0; --j) {
+        const bool sw = sk[j - 1] > sk[j];
+        const uint64_t a0 = sk[j - 1], a1 = sk[j];
+        const uint32_t c0 = scol[j - 1], c1 = scol[j], m0 = smask[j - 1], m1 = smask[j];
+        sk[j - 1] = sw ? a1 : a0;
+        sk[j] = sw ? a0 : a1;
+        scol[j - 1] = sw ? c1 : c0;
+        scol[j] = sw ? c0 : c1;
+        smask[j - 1] = sw ? m1 : m0;
+        smask[j] = sw ? m0 : m1;
+      }
+    }
+    float4 front = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 d = cb;
+    if constexpr (kCov) {
+      front = blend_front_cov<K>(scol, smask, n, c.cov, s_dec, s_alpha);
+      if (seen) {
+        float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+          if ((uint32_t)s < c.cov) sum = make_float4(sum.x + cbs[s].x, sum.y + cbs[s].y, sum.z + cbs[s].z, sum.w + cbs[s].w);
+        const float fs = (float)c.cov;
+        d = make_float4(sum.x / fs, sum.y / fs, sum.z / fs, sum.w / fs);
+      } else {
+        d = c.background;
+      }
+    } else {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if ((uint32_t)k < n) front = over(front, decode_premult_lds(scol[k], s_dec, s_alpha));
+    }
+    o.out[t] = over(front, d);
+    if (o.fk_key) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) o.fk_key[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)(sk[k] >> 32) : kEmpty;
+    }
+    if (o.fk_val) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) o.fk_val[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)sk[k] : kEmpty;
+    }
+    if (o.counts) o.counts[t] = count;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1591,6 +1850,78 @@ hipError_t launch_kbuf_resolve_cov(const Common& c, const KBuf& kb, const TailBu
                                    hipStream_t s) {
   if (c.cov < 2u || c.cov > 8u || !kb.mask) return hipErrorInvalidValue;
   return dispatch_k<KBufResolveCov>(c.K, c, kb, tb, o, s);
+}
+
+namespace {
+
+template <int K>
+struct Loop32Depth {
+  static hipError_t run(const Common& c, const FragView& f, uint32_t* zs, hipStream_t s) {
+    hipLaunchKernelGGL((k_loop32_depth<K>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, zs);
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct Loop32Color {
+  static hipError_t run(const Common& c, const FragView& f, const uint32_t* zs, uint64_t* cs, const TailBuf& tb,
+                        hipStream_t s) {
+    if (c.tail && tb.head) hipLaunchKernelGGL((k_loop32_color<K, true>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s,
+                                              c, f, zs, cs, tb);
+    else hipLaunchKernelGGL((k_loop32_color<K, false>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, zs, cs,
+                            tb);
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct Loop32Resolve {
+  static hipError_t run(const Common& c, const uint32_t* zs, const uint64_t* cs, const TailBuf& tb,
+                        const ResolveOut& o, hipStream_t s) {
+    const bool dbg = o.fk_key || o.fk_val || o.counts;
+    const bool ctail = c.tail && tb.head != nullptr;
+    const dim3 g(blocks_for(c.T)), b(kBlock);
+#define OIT_L32(CT, DBG) hipLaunchKernelGGL((k_loop32_resolve<K, CT, DBG>), g, b, 0, s, c, zs, cs, tb, o)
+    if (ctail) { if (dbg) OIT_L32(true, true); else OIT_L32(true, false); }
+    else { if (dbg) OIT_L32(false, true); else OIT_L32(false, false); }
+#undef OIT_L32
+    return hipGetLastError();
+  }
+};
+
+template <int K>
+struct InterlockResolve {
+  static hipError_t run(const Common& c, const LLBuf& ll, const ResolveOut& o, hipStream_t s) {
+    uint32_t blocks = blocks_for(c.T);
+    if (blocks > 8192u) blocks = 8192u;
+    if (blocks == 0) blocks = 1;
+    if (c.cov > 1u) hipLaunchKernelGGL((k_interlock_resolve<K, true>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
+    else hipLaunchKernelGGL((k_interlock_resolve<K, false>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
+    return hipGetLastError();
+  }
+};
+
+}  // namespace
+
+hipError_t launch_loop32_depth(const Common& c, const FragView& f, uint32_t* zs, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  return dispatch_k<Loop32Depth>(c.K, c, f, zs, s);
+}
+
+hipError_t launch_loop32_color(const Common& c, const FragView& f, const uint32_t* zs, uint64_t* cs,
+                               const TailBuf& tb, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  return dispatch_k<Loop32Color>(c.K, c, f, zs, cs, tb, s);
+}
+
+hipError_t launch_loop32_resolve(const Common& c, const uint32_t* zs, const uint64_t* cs, const TailBuf& tb,
+                                 const ResolveOut& o, hipStream_t s) {
+  return dispatch_k<Loop32Resolve>(c.K, c, zs, cs, tb, o, s);
+}
+
+hipError_t launch_interlock_resolve(const Common& c, const LLBuf& ll, const ResolveOut& o, hipStream_t s) {
+  if (!ll.full || (c.cov > 1u && !ll.mask)) return hipErrorInvalidValue;
+  return dispatch_k<InterlockResolve>(c.K, c, ll, o, s);
 }
 
 hipError_t launch_finalize_srgb8(const float4* in, uint32_t W, uint32_t H, uint32_t S, uint32_t ss,

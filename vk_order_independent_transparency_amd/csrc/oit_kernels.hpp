@@ -38,6 +38,7 @@ struct LLBuf {
   uint32_t* overflow; // LL fragments that found the pool full
   uint32_t* counters; // frame counters [16]; the resetting resolve moves them to counters[16..31]
   uint32_t* mask = nullptr;  // [cap] coverage shading: the node's sample mask
+  float4* full = nullptr;    // [cap] ordered interlock: the shaded colour (its own tail blend uses it)
 };
 
 struct KBuf {          // Simple / Spinlock
@@ -134,6 +135,16 @@ hipError_t launch_ll_resolve_cov(const Common& c, const LLBuf& ll, const TailBuf
                                  hipStream_t s);
 hipError_t launch_kbuf_resolve_cov(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o,
                                    hipStream_t s);
+// Loop32 (oitLoop.frag.glsl): depth pass per append, colour pass per append
+// at resolve time, composite
+hipError_t launch_loop32_depth(const Common& c, const FragView& f, uint32_t* zs, hipStream_t s);
+hipError_t launch_loop32_color(const Common& c, const FragView& f, const uint32_t* zs, uint64_t* cs,
+                               const TailBuf& tb, hipStream_t s);
+hipError_t launch_loop32_resolve(const Common& c, const uint32_t* zs, const uint64_t* cs, const TailBuf& tb,
+                                 const ResolveOut& o, hipStream_t s);
+// ordered interlock: Linked List append with shaded colours, then per-pixel
+// critical sections replayed in primitive order
+hipError_t launch_interlock_resolve(const Common& c, const LLBuf& ll, const ResolveOut& o, hipStream_t s);
 hipError_t launch_finalize_srgb8(const float4* in, uint32_t W, uint32_t H, uint32_t S, uint32_t ss,
                                  const CodecTables* tab, uint32_t* out, hipStream_t s);
 

@@ -51,10 +51,7 @@ int st(hipError_t e) {
 
 bool valid_layers(uint32_t k) { return k == 1 || k == 2 || k == 4 || k == 8 || k == 16 || k == 32; }
 
-bool supported(uint32_t alg) {
-  return alg == OIT_SIMPLE || alg == OIT_LINKEDLIST || alg == OIT_LOOP64 || alg == OIT_SPINLOCK ||
-         alg == OIT_WEIGHTED;
-}
+bool supported(uint32_t alg) { return alg <= OIT_WEIGHTED; }
 
 // Device bytes a context holds for one request (the sizes oit_configure
 // allocates; tail arrays 28 B per fragment, tail heads 4 B per target).
@@ -64,6 +61,8 @@ uint64_t request_bytes(const oit_request& r, uint64_t F) {
   const bool tail = r.tail_blend != 0;
   switch (r.algorithm) {
     case OIT_LINKEDLIST: return 4 * T + 12 * (F + 1) + (tail ? 4 * T : 0);
+    case OIT_INTERLOCK: return 4 * T + 28 * (F + 1);
+    case OIT_LOOP: return 12 * K * T + (tail ? 4 * T + 28 * F : 0);
     case OIT_LOOP64: return 8 * K * T + (tail ? 4 * T + 28 * F : 0);
     case OIT_SIMPLE: return 4 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
     case OIT_SPINLOCK: return 12 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
@@ -233,7 +232,7 @@ int launch(oit_server* s, Slot& sl) {
   c.width = r.scene.width;
   c.height = r.scene.height;
   c.samples = r.scene.samples;
-  c.ll_pool_nodes = alg == OIT_LINKEDLIST ? F + 1 : 0;
+  c.ll_pool_nodes = (alg == OIT_LINKEDLIST || alg == OIT_INTERLOCK) ? F + 1 : 0;
   c.max_fragments = (r.tail_blend && alg != OIT_LINKEDLIST && alg != OIT_WEIGHTED) ? (F ? F : 1) : 0;
   c.background[0] = c.background[1] = c.background[2] = 0.5f;
   c.background[3] = 1.0f;
@@ -359,7 +358,6 @@ void release(oit_server* s) {
 
 int check_request(const oit_request* r) {
   if (!r) return OIT_ERR_INVALID_ARG;
-  if (r->algorithm == OIT_LOOP || r->algorithm == OIT_INTERLOCK) return OIT_ERR_UNSUPPORTED;
   if (!supported(r->algorithm)) return OIT_ERR_INVALID_ARG;
   if (r->algorithm != OIT_WEIGHTED && !valid_layers(r->layers)) return OIT_ERR_INVALID_ARG;
   if (r->tail_blend > 1u || r->reserved != 0u) return OIT_ERR_INVALID_ARG;
