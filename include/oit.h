@@ -177,6 +177,24 @@ int oit_get_stats(oit_ctx* ctx, oit_stats* out, void* stream);
  * [height][width] RGBA bytes (device memory). */
 int oit_finalize_srgb8(oit_ctx* ctx, const float* in_rgba, uint32_t width, uint32_t height, uint32_t samples,
                        uint32_t supersample, uint8_t* out_rgba8, void* stream);
+/* Opaque draw of the frame (the opaque spheres drawn before the transparent
+ * ones, oitRender.cpp:68-122, with the depth-writing BlendMode::NONE
+ * pipeline, main.cpp:526-533). The frame's first call clears the opaque
+ * planes to depth 1.0 and the background colour (oitRender.cpp:89-91);
+ * each call then writes, per sample target, the depth and colour of the
+ * nearest fragment (depth test LESS: the earliest of equal depths; at or
+ * behind depth 1.0 nothing). With coverage shading `frags->coverage` holds
+ * each fragment's sample mask and the planes are per sample. Once a frame
+ * has an opaque draw, its transparent fragments are depth tested against it
+ * before they are stored (early fragment tests with post_depth_coverage,
+ * oitColorDepthDefines.glsl:24-37; with coverage shading the mask keeps the
+ * samples in front, an empty mask drops the fragment) and the composite
+ * blends over the opaque colour instead of the background. Call after
+ * oit_clear and before the frame's first oit_append (OIT_ERR_STATE
+ * otherwise); a count of 0 still gives the frame its depth-1.0 test.
+ * Linked List nodes stay indexed by transparent fragment id: a culled
+ * fragment's node is left unused. */
+int oit_draw_opaque(oit_ctx* ctx, const oit_fragments* frags, void* stream);
 /* Human-readable name of a status code. */
 const char* oit_status_string(int status);
 

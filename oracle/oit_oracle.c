@@ -253,20 +253,26 @@ static int ev_cmp(const void* a, const void* b) {
  * background. The reference blends into an 8-bit target, so the float
  * association is not fixed by it; this one lets the kernels fold a
  * newest-first list in one walk. */
-static int fold_tail_cov(tail_list* L, float* colorbuf, uint64_t T, int canonical, uint32_t cov);
+static int fold_tail_cov(tail_list* L, float* colorbuf, uint64_t T, int canonical, uint32_t cov,
+                         const float* sbase);
 static int fold_tail(tail_list* L, float* colorbuf, uint64_t T, int canonical) {
-  return fold_tail_cov(L, colorbuf, T, canonical, 0);
+  return fold_tail_cov(L, colorbuf, T, canonical, 0, NULL);
 }
 /* Coverage shading: the ROP blends each event into the samples of its mask;
  * per sample the same fold, then the colour buffer of a target with tail
  * events is the sum of its sample values in sample order divided by the
- * sample count (the MSAA resolve). */
-static int fold_tail_cov(tail_list* L, float* colorbuf, uint64_t T, int canonical, uint32_t cov) {
+ * sample count (the MSAA resolve). `sbase` (an opaque pass): the sample
+ * colours under the fold, entry s * T + t; every target is then resolved
+ * from its samples. Without it each sample starts at colorbuf[t] and a
+ * target without events keeps colorbuf[t]. */
+static int fold_tail_cov(tail_list* L, float* colorbuf, uint64_t T, int canonical, uint32_t cov,
+                         const float* sbase) {
   qsort(L->ev, L->n, sizeof(tail_ev), ev_cmp); /* positions are unique per target in canonical order */
   if (cov > 1) {
     float* acc = (float*)calloc(T * cov * 4, sizeof(float));
     unsigned char* seen = (unsigned char*)calloc(T, 1);
     if (!acc || !seen) { free(acc); free(seen); return -2; }
+    if (sbase) memset(seen, 1, T);
     if (canonical) {
       for (uint64_t i = L->n; i-- > 0;) {
         const uint64_t t = L->ev[i].target;
@@ -276,11 +282,13 @@ static int fold_tail_cov(tail_list* L, float* colorbuf, uint64_t T, int canonica
       }
       for (uint64_t t = 0; t < T; ++t) {
         if (!seen[t]) continue;
-        for (uint32_t s = 0; s < cov; ++s) over_into(&acc[(t * cov + s) * 4], &colorbuf[4 * t]);
+        for (uint32_t s = 0; s < cov; ++s)
+          over_into(&acc[(t * cov + s) * 4], sbase ? &sbase[(s * T + t) * 4] : &colorbuf[4 * t]);
       }
     } else {
       for (uint64_t t = 0; t < T; ++t)
-        for (uint32_t s = 0; s < cov; ++s) memcpy(&acc[(t * cov + s) * 4], &colorbuf[4 * t], 4 * sizeof(float));
+        for (uint32_t s = 0; s < cov; ++s)
+          memcpy(&acc[(t * cov + s) * 4], sbase ? &sbase[(s * T + t) * 4] : &colorbuf[4 * t], 4 * sizeof(float));
       for (uint64_t i = 0; i < L->n; ++i) {
         const uint64_t t = L->ev[i].target;
         for (uint32_t s = 0; s < cov; ++s)
@@ -336,6 +344,53 @@ static void write_frontk(orc_result* res, uint64_t t, uint32_t K, const elem* a,
     for (uint32_t i = 0; i < K; ++i) res->frontk_val[t * K + i] = (int)i < n ? (loop64 ? a[i].col : a[i].id) : EMPTY_U32;
 }
 
+/* Early depth test of a transparent fragment against the frame's opaque
+ * pass (early_fragment_tests with post_depth_coverage,
+ * oitColorDepthDefines.glsl:24-37; VK_COMPARE_OP_LESS, main.cpp:526-528,
+ * no depth write for the transparent pipelines, :541-543): the samples of m
+ * whose opaque depth key is larger than key. 0: the fragment does not exist
+ * (its shader does not run). */
+static uint32_t opq_visible(const orc_config* cfg, uint64_t T, uint32_t cov, uint32_t t, uint32_t key, uint32_t m) {
+  if (!cfg->opaque_key) return m;
+  if (cov) {
+    uint32_t r = 0;
+    for (uint32_t s = 0; s < cov; ++s)
+      if (((m >> s) & 1u) && key < cfg->opaque_key[(uint64_t)s * T + t]) r |= 1u << s;
+    return r;
+  }
+  return key < cfg->opaque_key[t] ? m : 0u;
+}
+
+int orc_opaque_pass(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
+                    const float* rgba, const uint32_t* mask, uint32_t* key_out, float* rgba_out) {
+  init_tables();
+  if (!cfg || !key_out || !rgba_out) return -1;
+  const uint64_t T = (uint64_t)cfg->width * cfg->height * cfg->samples;
+  const uint32_t cov = cfg->coverage_samples > 1 ? cfg->coverage_samples : 0;
+  if (T == 0 || (cov && (cfg->samples != 1 || !mask))) return -1;
+  if (cov && cov != 2 && cov != 4 && cov != 8) return -1;
+  const uint32_t planes = cov ? cov : 1;
+  const uint32_t clear = orc_depth_key(1.0f); /* depth clear, oitRender.cpp:91 */
+  for (uint64_t j = 0; j < T * planes; ++j) {
+    key_out[j] = clear;
+    memcpy(&rgba_out[4 * j], cfg->background, 4 * sizeof(float)); /* colour clear, :90 */
+  }
+  for (uint64_t i = 0; i < F; ++i) if (target[i] >= T) return -1;
+  for (uint64_t i = 0; i < F; ++i) {
+    const uint32_t key = orc_depth_key(depth[i]);
+    const uint32_t m = cov ? mask[i] & ((1u << cov) - 1u) : 1u;
+    for (uint32_t s = 0; s < planes; ++s) {
+      if (!((m >> s) & 1u)) continue;
+      const uint64_t j = (uint64_t)s * T + target[i];
+      if (key < key_out[j]) { /* LESS; a later fragment at an equal depth fails */
+        key_out[j] = key;
+        memcpy(&rgba_out[4 * j], &rgba[4 * i], 4 * sizeof(float));
+      }
+    }
+  }
+  return 0;
+}
+
 int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
             const float* rgba, orc_result* res) {
   return orc_run_cov(cfg, F, target, depth, rgba, NULL, res);
@@ -359,10 +414,14 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
   if (cfg->algorithm != ORC_WEIGHTED && !valid_layers(K)) return -1;
   for (uint64_t i = 0; i < F; ++i) if (target[i] >= T) return -1;
   if (F >= 0xFFFFFFFFull) return -1;
+  if (!cfg->opaque_key != !cfg->opaque_rgba) return -1;
 
   float* colorbuf = (float*)malloc(T * 4 * sizeof(float));
   if (!colorbuf) return -2;
   for (uint64_t t = 0; t < T; ++t) memcpy(&colorbuf[4 * t], cfg->background, 4 * sizeof(float));
+  /* with an opaque pass the colour buffer starts as its colour (per sample with coverage shading) */
+  const float* sbase = cov ? cfg->opaque_rgba : NULL;
+  if (cfg->opaque_rgba && !cov) memcpy(colorbuf, cfg->opaque_rgba, T * 4 * sizeof(float));
   tail_list tails = {0, 0, 0};
   res->ll_overflow = 0;
   res->tail_fragments = 0;
@@ -384,8 +443,9 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
     for (uint64_t i = 0; i < F; ++i) {
       const uint64_t n = counter + 1; /* :55 newOffset = atomicAdd(counter, 1) + 1 */
       counter++;
-      const uint32_t m = mask ? mask[i] & full : full;
-      if (cov && !m) continue; /* no sample covered: the shader does not run (its node stays unused) */
+      const uint32_t m = opq_visible(cfg, T, cov, target[i], orc_depth_key(depth[i]), mask ? mask[i] & full : full);
+      if (!m) continue; /* no sample covered or all behind the opaque depth: the shader does not run
+                           (its node stays unused) */
       if (n >= cap) { /* :59-68 out of memory: tail-blend the shaded colour */
         res->ll_overflow++;
         if (cfg->tail_blend) {
@@ -403,7 +463,7 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
       node[n].id = (uint32_t)i;
       node[n].mask = m; /* :79 storeMask */
     }
-    if ((rc = fold_tail_cov(&tails, colorbuf, T, canonical, cov))) goto ll_free;
+    if ((rc = fold_tail_cov(&tails, colorbuf, T, canonical, cov, sbase))) goto ll_free;
     /* composite, :106-172 */
     elem* a = (elem*)malloc(K * sizeof(elem));
     if (!a) { rc = -2; goto ll_free; }
@@ -450,8 +510,8 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
     for (uint64_t t = 0; t < T; ++t) auxd[t] = EMPTY_U32;
     for (uint64_t i = 0; i < F; ++i) {
       const uint32_t t = target[i];
-      const uint32_t m = mask ? mask[i] & full : full;
-      if (cov && !m) continue; /* no sample covered: the shader does not run */
+      const uint32_t m = opq_visible(cfg, T, cov, t, orc_depth_key(depth[i]), mask ? mask[i] & full : full);
+      if (!m) continue; /* no sample covered or all behind the opaque depth: the shader does not run */
       elem x = { orc_depth_key(depth[i]), (uint32_t)i, orc_encode(&rgba[4 * i]), m };
       if (!spin) {
         /* oitSimple.frag.glsl:74-90 */
@@ -501,7 +561,7 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
         res->tail_fragments++;
       }
     }
-    if ((rc = fold_tail_cov(&tails, colorbuf, T, canonical, cov))) goto kb_free;
+    if ((rc = fold_tail_cov(&tails, colorbuf, T, canonical, cov, sbase))) goto kb_free;
     {
       elem* a = (elem*)malloc(K * sizeof(elem));
       if (!a) { rc = -2; goto kb_free; }
@@ -534,6 +594,7 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
     for (uint64_t i = 0; i < F; ++i) { /* depth pass: the K nearest distinct depths, sorted */
       const uint64_t base = (uint64_t)target[i] * K;
       uint32_t z = orc_depth_key(depth[i]);
+      if (!opq_visible(cfg, T, 0, target[i], z, 1u)) continue;
       uint32_t j = 0;
       if (z > zs[base + K - 1]) continue;    /* :75-80 pretest */
       if (z > zs[base + K / 2]) j = K / 2;   /* :82-87 */
@@ -547,6 +608,7 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
     for (uint64_t i = 0; i < F; ++i) { /* colour pass */
       const uint64_t base = (uint64_t)target[i] * K;
       const uint32_t z = orc_depth_key(depth[i]);
+      if (!opq_visible(cfg, T, 0, target[i], z, 1u)) continue;
       if (zs[base + K - 1] < z) {            /* :139-150 behind the K-th depth: tail-blend itself */
         if (cfg->tail_blend) {
           if ((rc = tail_push(&tails, (uint32_t)i, target[i], &rgba[4 * i]))) { free(zs); free(cs); goto done; }
@@ -596,6 +658,7 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
     for (uint64_t j = 0; j < T * K; ++j) { slot[j] = EMPTY_U64; sid[j] = EMPTY_U32; }
     for (uint64_t i = 0; i < F; ++i) {
       const uint64_t base = (uint64_t)target[i] * K;
+      if (!opq_visible(cfg, T, 0, target[i], orc_depth_key(depth[i]), 1u)) continue;
       uint64_t zcur = ((uint64_t)orc_depth_key(depth[i]) << 32) | orc_encode(&rgba[4 * i]); /* :79 */
       uint32_t zid = (uint32_t)i;
       uint32_t j = 0;
@@ -652,6 +715,7 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
     if (!acc || !rev) { free(acc); free(rev); rc = -2; goto done; }
     for (uint64_t t = 0; t < T; ++t) rev[t] = 1.0f;
     for (uint64_t i = 0; i < F; ++i) {
+      if (!opq_visible(cfg, T, 0, target[i], orc_depth_key(depth[i]), 1u)) continue;
       const float* c = &rgba[4 * i];   /* premultiplied, :55-56 */
       const float view = cfg->wboit_view_near + depth[i] * cfg->wboit_view_span;
       const float depthZ = view * 10.0f;                                   /* :65 */
@@ -673,8 +737,9 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
       const float den = fmaxf(A[3], 1e-5f);                                /* :108 */
       float s[4] = { A[0] / den, A[1] / den, A[2] / den, r };
       const float om = 1.0f - r;                                           /* WEIGHTED_COMPOSITE main.cpp:578-588 */
+      const float* dst = &colorbuf[4 * t];
       float o[4];
-      for (int k = 0; k < 4; ++k) o[k] = s[k] * om + cfg->background[k] * r;
+      for (int k = 0; k < 4; ++k) o[k] = s[k] * om + dst[k] * r;
       memcpy(&res->out[4 * t], o, sizeof o);
       if (res->wb_accum) memcpy(&res->wb_accum[4 * t], A, 4 * sizeof(float));
       if (res->wb_reveal) res->wb_reveal[t] = r;
@@ -710,13 +775,16 @@ int orc_run_fp64(const orc_config* cfg, uint64_t F, const uint32_t* target, cons
   if (cfg->coverage_samples > 1) return -1; /* the exact mirror has no coverage shading */
   const uint64_t T = (uint64_t)cfg->width * cfg->height * cfg->samples;
   if (T == 0 || !out) return -1;
+  if (!cfg->opaque_key != !cfg->opaque_rgba) return -1;
   for (uint64_t i = 0; i < F; ++i) if (target[i] >= T) return -1;
+#define ORC_DST(t, k) ((double)(cfg->opaque_rgba ? cfg->opaque_rgba[4 * (t) + (k)] : cfg->background[k]))
   if (cfg->algorithm == ORC_WEIGHTED) {
     double* acc = (double*)calloc(T * 4, sizeof(double));
     double* rev = (double*)malloc(T * sizeof(double));
     if (!acc || !rev) { free(acc); free(rev); return -2; }
     for (uint64_t t = 0; t < T; ++t) rev[t] = 1.0;
     for (uint64_t i = 0; i < F; ++i) {
+      if (!opq_visible(cfg, T, 0, target[i], orc_depth_key(depth[i]), 1u)) continue;
       const float* c = &rgba[4 * i];
       const double view = (double)cfg->wboit_view_near + (double)depth[i] * (double)cfg->wboit_view_span;
       const double x = view * 10.0 / 200.0;
@@ -733,8 +801,8 @@ int orc_run_fp64(const orc_config* cfg, uint64_t F, const uint32_t* target, cons
     for (uint64_t t = 0; t < T; ++t) {
       const double r = rev[t];
       const double den = acc[4 * t + 3] > 1e-5 ? acc[4 * t + 3] : 1e-5;
-      for (int k = 0; k < 3; ++k) out[4 * t + k] = acc[4 * t + k] / den * (1.0 - r) + (double)cfg->background[k] * r;
-      out[4 * t + 3] = r * (1.0 - r) + (double)cfg->background[3] * r;
+      for (int k = 0; k < 3; ++k) out[4 * t + k] = acc[4 * t + k] / den * (1.0 - r) + ORC_DST(t, k) * r;
+      out[4 * t + 3] = r * (1.0 - r) + ORC_DST(t, 3) * r;
       if (wb_accum) for (int k = 0; k < 4; ++k) wb_accum[4 * t + k] = acc[4 * t + k];
       if (wb_reveal) wb_reveal[t] = r;
     }
@@ -745,16 +813,21 @@ int orc_run_fp64(const orc_config* cfg, uint64_t F, const uint32_t* target, cons
   uint64_t* start = (uint64_t*)calloc(T + 1, sizeof(uint64_t));
   sort_rec* recs = (sort_rec*)malloc((F ? F : 1) * sizeof(sort_rec));
   if (!start || !recs) { free(start); free(recs); return -2; }
-  for (uint64_t i = 0; i < F; ++i) start[target[i] + 1]++;
+  unsigned char* live = (unsigned char*)malloc(F ? F : 1);
+  if (!live) { free(start); free(recs); return -2; }
+  for (uint64_t i = 0; i < F; ++i) live[i] = opq_visible(cfg, T, 0, target[i], orc_depth_key(depth[i]), 1u) != 0;
+  for (uint64_t i = 0; i < F; ++i) if (live[i]) start[target[i] + 1]++;
   for (uint64_t t = 0; t < T; ++t) start[t + 1] += start[t];
   uint64_t* fill = (uint64_t*)malloc((T + 1) * sizeof(uint64_t));
-  if (!fill) { free(start); free(recs); return -2; }
+  if (!fill) { free(start); free(recs); free(live); return -2; }
   memcpy(fill, start, (T + 1) * sizeof(uint64_t));
   for (uint64_t i = 0; i < F; ++i) {
+    if (!live[i]) continue;
     sort_rec r = { orc_depth_key(depth[i]), (uint32_t)i };
     recs[fill[target[i]]++] = r;
   }
   free(fill);
+  free(live);
   for (uint64_t t = 0; t < T; ++t) {
     const uint64_t n = start[t + 1] - start[t];
     if (n > 1) qsort(&recs[start[t]], n, sizeof(sort_rec), rec_cmp);
@@ -776,10 +849,11 @@ int orc_run_fp64(const orc_config* cfg, uint64_t F, const uint32_t* target, cons
       for (int k = 0; k < 4; ++k) c[k] += tt * f[k];
     }
     const double tt = 1.0 - c[3];
-    for (int k = 0; k < 4; ++k) out[4 * t + k] = c[k] + tt * (double)cfg->background[k];
+    for (int k = 0; k < 4; ++k) out[4 * t + k] = c[k] + tt * ORC_DST(t, k);
   }
   free(start); free(recs);
   return 0;
+#undef ORC_DST
 }
 
 /* ------------------------------------------------------------------ */
@@ -808,9 +882,11 @@ int orc_replay_kbuffer_cov(const orc_config* cfg, uint64_t F, const uint32_t* ta
   elem* a = (elem*)malloc(K * sizeof(elem));
   if (!colorbuf || !a) { free(colorbuf); free(a); return -2; }
   for (uint64_t t = 0; t < T; ++t) memcpy(&colorbuf[4 * t], cfg->background, 4 * sizeof(float));
-  if (cfg->tail_blend && tail_flag) {
+  if (!cfg->opaque_key != !cfg->opaque_rgba) { free(colorbuf); free(a); return -1; }
+  if (cfg->opaque_rgba && !cov) memcpy(colorbuf, cfg->opaque_rgba, T * 4 * sizeof(float));
+  if ((cfg->tail_blend && tail_flag) || (cov && cfg->opaque_rgba)) {
     tail_list tails = {0, 0, 0};
-    for (uint64_t i = 0; i < F; ++i) {
+    for (uint64_t i = 0; cfg->tail_blend && tail_flag && i < F; ++i) {
       if (!tail_flag[i]) continue;
       if (target[i] >= T) { free(tails.ev); free(colorbuf); free(a); return -1; }
       float c4[4];
@@ -819,7 +895,7 @@ int orc_replay_kbuffer_cov(const orc_config* cfg, uint64_t F, const uint32_t* ta
       const uint32_t m = mask ? mask[i] & full : full;
       if (tail_push_m(&tails, (uint32_t)i, target[i], m, c4)) { free(tails.ev); free(colorbuf); free(a); return -2; }
     }
-    const int frc = fold_tail_cov(&tails, colorbuf, T, 1, cov); /* id order, canonical association */
+    const int frc = fold_tail_cov(&tails, colorbuf, T, 1, cov, cov ? cfg->opaque_rgba : NULL); /* id order, canonical */
     free(tails.ev);
     if (frc) { free(colorbuf); free(a); return frc; }
   }

@@ -53,6 +53,13 @@ typedef struct orc_config {
   float background[4];
   float wboit_view_near, wboit_view_span;
   uint32_t coverage_samples; /* OIT_COVERAGE_SHADING: MSAA samples per pixel (2, 4, 8), 0 or 1 = off */
+  /* Opaque pass of the frame (orc_opaque_pass), NULL when the frame has none:
+   * depth key and colour per sample target; with coverage shading sample s
+   * of pixel p is entry s * W * H + p. Transparent fragments are depth
+   * tested against it (LESS, samples of the mask) and the composite blends
+   * over its colour instead of the background. */
+  const uint32_t* opaque_key;
+  const float*    opaque_rgba;
 } orc_config;
 
 typedef struct orc_result {
@@ -84,6 +91,15 @@ int orc_run(const orc_config* cfg, uint64_t F, const uint32_t* target, const flo
  * cfg->coverage_samples > 1; Simple, Linked List and Spinlock only). */
 int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
                 const float* rgba, const uint32_t* mask, orc_result* res);
+
+/* Opaque pass (oitRender.cpp:113-122; BlendMode::NONE pipeline with depth
+ * test LESS and depth write, main.cpp:526-533): the planes are cleared to
+ * depth 1.0 and the background colour (oitRender.cpp:89-91), then every
+ * fragment in draw order writes its depth key and colour to each sample
+ * target (its target; with coverage shading the samples of its mask) whose
+ * stored key is larger. Outputs [cov * W * H] or [W * H * samples] entries. */
+int orc_opaque_pass(const orc_config* cfg, uint64_t F, const uint32_t* target, const float* depth,
+                    const float* rgba, const uint32_t* mask, uint32_t* key_out, float* rgba_out);
 
 /* Display image: the average of `samples` sample planes and of each
  * ss x ss block of a (width*ss) x (height*ss) premultiplied image (the MSAA

@@ -31,6 +31,8 @@ class OrcConfig(ctypes.Structure):
         ("wboit_view_near", ctypes.c_float),
         ("wboit_view_span", ctypes.c_float),
         ("coverage_samples", ctypes.c_uint32),
+        ("opaque_key", ctypes.c_void_p),
+        ("opaque_rgba", ctypes.c_void_p),
     ]
 
 
@@ -86,6 +88,8 @@ def lib() -> ctypes.CDLL:
         L.orc_replay_kbuffer_cov.restype = ctypes.c_int
         L.orc_finalize_srgb8.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, P]
         L.orc_finalize_srgb8.restype = ctypes.c_int
+        L.orc_opaque_pass.argtypes = [ctypes.POINTER(OrcConfig), ctypes.c_uint64, P, P, P, P, P, P]
+        L.orc_opaque_pass.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -115,7 +119,9 @@ def decode_premult(packed: int) -> np.ndarray:
 
 def make_config(algorithm, layers, tail_blend, width, height, samples=1, tail_order=TAIL_CANONICAL,
                 ll_capacity=0, background=(0.5, 0.5, 0.5, 1.0), wboit_view_near=2.0, wboit_view_span=8.0,
-                coverage_samples=0):
+                coverage_samples=0, opaque=None):
+    """`opaque`: the frame's opaque pass, (depth keys, colours) per sample
+    target as returned by opaque_pass()."""
     c = OrcConfig()
     c.coverage_samples = coverage_samples
     c.algorithm = algorithm
@@ -128,6 +134,13 @@ def make_config(algorithm, layers, tail_blend, width, height, samples=1, tail_or
         c.background[i] = background[i]
     c.wboit_view_near = wboit_view_near
     c.wboit_view_span = wboit_view_span
+    if opaque is not None:
+        keys = np.ascontiguousarray(opaque[0], dtype=np.uint32)
+        cols = np.ascontiguousarray(opaque[1], dtype=np.float32).reshape(-1, 4)
+        assert keys.shape[0] == cols.shape[0]
+        c._opaque = (keys, cols)  # kept alive with the config
+        c.opaque_key = keys.ctypes.data
+        c.opaque_rgba = cols.ctypes.data
     return c
 
 
@@ -137,6 +150,24 @@ def _arrays(target, depth, rgba):
     rgba = np.ascontiguousarray(rgba, dtype=np.float32).reshape(-1, 4)
     assert target.shape[0] == depth.shape[0] == rgba.shape[0]
     return target, depth, rgba
+
+
+def opaque_pass(cfg: OrcConfig, target, depth, rgba, mask=None) -> tuple[np.ndarray, np.ndarray]:
+    """The frame's opaque pass (depth test LESS with depth write over the
+    opaque stream in draw order): depth keys [n] and colours [n, 4] per sample
+    target, n = W*H*samples, or coverage_samples planes of W*H."""
+    target, depth, rgba = _arrays(target, depth, rgba)
+    planes = cfg.coverage_samples if cfg.coverage_samples > 1 else 1
+    n = cfg.width * cfg.height * cfg.samples * planes
+    keys = np.zeros(n, np.uint32)
+    cols = np.zeros((n, 4), np.float32)
+    m = None if mask is None else np.ascontiguousarray(mask, dtype=np.uint32)
+    rc = lib().orc_opaque_pass(ctypes.byref(cfg), target.shape[0], target.ctypes.data, depth.ctypes.data,
+                               rgba.ctypes.data, None if m is None else m.ctypes.data, keys.ctypes.data,
+                               cols.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_opaque_pass failed: {rc}")
+    return keys, cols
 
 
 def run(cfg: OrcConfig, target, depth, rgba, mask=None) -> dict:

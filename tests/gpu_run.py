@@ -9,7 +9,7 @@ import numpy as np
 
 def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, samples=1, ll_pool_nodes=0,
             splits=1, background=(0.5, 0.5, 0.5, 1.0), max_fragments=0, renderer=None, variant=0,
-            coverage_samples=0, mask=None):
+            coverage_samples=0, mask=None, opaque=None, opaque_splits=1):
     from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, Renderer
 
     T = width * height * samples
@@ -32,6 +32,21 @@ def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, s
         fk_val = DeviceBuffer((T, K), np.uint32)
         counts = DeviceBuffer((T,), np.uint32)
         r.clear()
+        keep = []
+        if opaque is not None:  # (target, depth, rgba, mask or None): the frame's opaque draws
+            otg, od, oc, om = opaque
+            G = int(len(otg))
+            ob = [DeviceBuffer.from_numpy(np.ascontiguousarray(otg, dtype=np.uint32).reshape(G)),
+                  DeviceBuffer.from_numpy(np.ascontiguousarray(od, dtype=np.float32).reshape(G)),
+                  DeviceBuffer.from_numpy(np.ascontiguousarray(oc, dtype=np.float32).reshape(G, 4))]
+            ob.append(None if om is None else
+                      DeviceBuffer.from_numpy(np.ascontiguousarray(om, dtype=np.uint32).reshape(G)))
+            keep.append(ob)
+            obounds = np.linspace(0, G, opaque_splits + 1).astype(np.int64)
+            for a, b in zip(obounds[:-1], obounds[1:]):
+                n = int(b - a)
+                r.draw_opaque(ob[0].view(a, n), ob[1].view(a, n), ob[2].view(a, n), count=n,
+                              coverage=None if ob[3] is None else ob[3].view(a, n))
         bounds = np.linspace(0, F, splits + 1).astype(np.int64)
         for a, b in zip(bounds[:-1], bounds[1:]):
             n = int(b - a)

@@ -212,6 +212,7 @@ SIGNATURES = {
     "oit_trim": (ctypes.c_int, [_P]),
     "oit_clear": (ctypes.c_int, [_P, _P]),
     "oit_append": (ctypes.c_int, [_P, ctypes.POINTER(OitFragments), _P]),
+    "oit_draw_opaque": (ctypes.c_int, [_P, ctypes.POINTER(OitFragments), _P]),
     "oit_resolve": (ctypes.c_int, [_P, _P, _P]),
     "oit_resolve_debug": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
     "oit_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(OitStats), _P]),

@@ -142,6 +142,13 @@ struct oit_ctx {
   uint32_t* kb_mask = nullptr;
   uint32_t* tl_mask = nullptr;
   uint64_t cap_ll_mask = 0, cap_kb_mask = 0, cap_tl_mask = 0;
+  // opaque pass: (depth key << 32 | id) and colour per sample target
+  // [T * max(coverage samples, 1)], valid for the frame once opq_active
+  uint64_t* opq_z = nullptr;
+  float4* opq_rgba = nullptr;
+  uint64_t cap_opq = 0, cap_opq_rgba = 0;
+  bool opq_active = false;
+  uint64_t opq_frags = 0;  // opaque fragments drawn this frame (their ids)
   // the last stream-ordered call's completion: buffers are freed only after it
   hipEvent_t work_ev = nullptr;
   bool work_pending = false;
@@ -235,6 +242,10 @@ void free_frame_buffers(oit_ctx* c) {
   c->cap_il_full = c->cap_l32 = 0;
   dev_free(&c->kb_mask);
   dev_free(&c->tl_mask);
+  dev_free(&c->opq_z);
+  dev_free(&c->opq_rgba);
+  c->cap_opq = c->cap_opq_rgba = 0;
+  c->opq_active = false;
   dev_free(&c->ll_head);
   dev_free(&c->ll_cd);
   dev_free(&c->ll_next);
@@ -297,6 +308,10 @@ oit::Common common_of(const oit_ctx* c) {
   k.background = make_float4(c->cfg.background[0], c->cfg.background[1], c->cfg.background[2], c->cfg.background[3]);
   k.tab = c->d_tab;
   k.cov = c->cfg.coverage_samples > 1 ? c->cfg.coverage_samples : 0;
+  if (c->opq_active) {
+    k.opq_z = c->opq_z;
+    k.opq_rgba = c->opq_rgba;
+  }
   return k;
 }
 
@@ -717,7 +732,7 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
                         c->cap_kb_depth * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_wb * 20u +
                         c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u) +
                         (c->cap_ll_mask + c->cap_kb_mask + c->cap_tl_mask) * 4u + c->cap_il_full * 16u +
-                        c->cap_l32 * 12u;
+                        c->cap_l32 * 12u + c->cap_opq * 8u + c->cap_opq_rgba * 16u;
   return OIT_OK;
 }
 
@@ -739,6 +754,8 @@ int oit_clear(oit_ctx* c, void* stream) {
     c->tl_head_valid = false;
     c->frame_frags = 0;
     c->appends = 0;
+    c->opq_active = false;
+    c->opq_frags = 0;
     c->segments.clear();
     c->frame_open = true;
     c->resolved = false;
@@ -788,11 +805,51 @@ int oit_clear(oit_ctx* c, void* stream) {
   c->tl_head_valid = e == hipSuccess && need_tail;
   c->frame_frags = 0;
   c->appends = 0;
+  c->opq_active = false;
+  c->opq_frags = 0;
   c->segments.clear();
   c->frame_open = e == hipSuccess;
   c->resolved = false;
   mark_work(c, s);
   return status_of(e);
+}
+
+int oit_draw_opaque(oit_ctx* c, const oit_fragments* f, void* stream) {
+  if (!c || !f) return OIT_ERR_INVALID_ARG;
+  if (!c->configured) return OIT_ERR_STATE;
+  if (!c->frame_open || c->resolved || c->appends > 0) return OIT_ERR_STATE;  // opaque first, oitRender.cpp:113-125
+  if (f->count > 0 && (!f->target || !f->depth || !f->rgba)) return OIT_ERR_INVALID_ARG;
+  if (f->count > 0 && coverage(c) && !f->coverage) return OIT_ERR_INVALID_ARG;
+  if (f->count > kMaxFrameFragments || c->opq_frags + f->count > kMaxFrameFragments) return OIT_ERR_CAPACITY;
+  DeviceGuard g(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  hipError_t e = hipSuccess;
+  if (!c->opq_active) {  // the frame's first opaque draw: depth 1.0, clear colour (oitRender.cpp:89-91)
+    const uint64_t n = (uint64_t)c->T * (coverage(c) ? c->cfg.coverage_samples : 1u);
+    int rc = reserve(c, &c->opq_z, &c->cap_opq, n);
+    if (!rc) rc = reserve(c, &c->opq_rgba, &c->cap_opq_rgba, n);
+    if (rc) return rc;
+    const float4 bg = make_float4(c->cfg.background[0], c->cfg.background[1], c->cfg.background[2],
+                                  c->cfg.background[3]);
+    e = staged(c, kStClear, s, [&] { return oit::launch_opaque_clear(c->opq_z, c->opq_rgba, n, bg, s); });
+    if (e != hipSuccess) return status_of(e);
+    c->opq_active = true;
+  }
+  if (f->count > 0) {
+    oit::FragView v;
+    v.target = f->target;
+    v.depth = f->depth;
+    v.rgba = reinterpret_cast<const float4*>(f->rgba);
+    v.count = (uint32_t)f->count;
+    v.id_base = (uint32_t)c->opq_frags;
+    v.mask = coverage(c) ? f->coverage : nullptr;
+    const oit::Common k = common_of(c);
+    e = staged(c, kStStore, s, [&] { return oit::launch_opaque_draw(k, v, c->opq_z, c->opq_rgba, s); });
+    if (e != hipSuccess) return status_of(e);
+    c->opq_frags += f->count;
+  }
+  mark_work(c, s);
+  return OIT_OK;
 }
 
 int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {

@@ -215,14 +215,39 @@ __device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float
   return over(acc, bg);
 }
 
+// Colour buffer of target t before the colour-pass tail: the colour of the
+// frame's opaque pass, else the clear colour.
+__device__ __forceinline__ float4 dest_of(const Common& c, uint32_t t) {
+  return c.opq_rgba ? c.opq_rgba[t] : c.background;
+}
+
+// Early depth test of a transparent fragment against the frame's opaque pass
+// (early_fragment_tests with post_depth_coverage, oitColorDepthDefines.glsl:
+// 24-37; VK_COMPARE_OP_LESS without depth write, main.cpp:526-528,541-543):
+// the samples of mask m whose opaque depth key is larger than key (one test
+// at target t unless coverage shading). Zero: the fragment does not exist.
+__device__ __forceinline__ uint32_t opaque_visible(const uint64_t* z, uint32_t T, uint32_t cov, uint32_t t,
+                                                   uint32_t key, uint32_t m) {
+  if (!z) return m;
+  if (cov > 1u) {
+    uint32_t r = 0u;
+    for (uint32_t s = 0; s < cov; ++s)
+      if (((m >> s) & 1u) && key < (uint32_t)(z[(uint64_t)s * T + t] >> 32)) r |= 1u << s;
+    return r;
+  }
+  return key < (uint32_t)(z[t] >> 32) ? m : 0u;
+}
+
 // Coverage shading: the ROP blends each tail fragment into the samples of
-// its mask; per sample the newest-first fold over the background, then the
-// sum over samples in order divided by their count (the MSAA resolve). A
-// target without tail fragments keeps the background.
-__device__ __forceinline__ float4 tail_fold_cov(const TailBuf& tb, uint32_t t, float4 bg, uint32_t S) {
+// its mask; per sample the newest-first fold over the sample's colour (the
+// opaque pass's, else the clear colour), then the sum over samples in order
+// divided by their count (the MSAA resolve). Without an opaque pass a target
+// without tail fragments keeps the clear colour.
+__device__ __forceinline__ float4 tail_fold_cov(const TailBuf& tb, uint32_t t, const Common& c) {
+  const uint32_t S = c.cov;
   const uint32_t vcap = tail_value_cap(tb.cap);
-  uint32_t v = tb.head[t];
-  if (v == 0u || v >= vcap) return bg;
+  uint32_t v = tb.head ? tb.head[t] : 0u;
+  if ((v == 0u || v >= vcap) && !c.opq_rgba) return c.background;
   float4 acc[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -240,7 +265,7 @@ __device__ __forceinline__ float4 tail_fold_cov(const TailBuf& tb, uint32_t t, f
 #pragma unroll
   for (int s = 0; s < 8; ++s) {
     if ((uint32_t)s < S) {
-      const float4 cb = over(acc[s], bg);
+      const float4 cb = over(acc[s], c.opq_rgba ? c.opq_rgba[(uint64_t)s * c.T + t] : c.background);
       sum = make_float4(sum.x + cb.x, sum.y + cb.y, sum.z + cb.z, sum.w + cb.w);
     }
   }
@@ -321,12 +346,17 @@ __global__ __launch_bounds__(kBlock) void k_ll_store(Common c, FragView f, LLBuf
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const bool valid = i < f.count;
+  bool valid = i < f.count;
   const uint32_t lid = f.id_base + i;
   const uint64_t n = (uint64_t)lid + 1u;
   bool ovf = false;
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t t = kEmpty;
+  if (valid && c.opq_z) {  // culled by the opaque pass: no node, no tail
+    t = f.target[i];
+    valid = t < c.T && opaque_visible(c.opq_z, c.T, 1u, t, depth_key(f.depth[i]), 1u) != 0u;
+    t = kEmpty;
+  }
   if (valid) {
     col = f.rgba[i];
     if (n >= ll.cap) {
@@ -352,7 +382,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_link(Common c, FragView f, LLBuf 
   const uint64_t n64 = (uint64_t)f.id_base + i + 1u;
   bool valid = i < f.count && n64 < ll.cap;
   uint32_t t = valid ? f.target[i] : kEmpty;
-  if (t >= c.T) {
+  if (t >= c.T || (c.opq_z && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(f.depth[i]), 1u))) {
     valid = false;
     t = kEmpty;
   }
@@ -391,8 +421,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
   const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
   float4 col = blend_front<K>(a, n, c.tab);
   col = over(col, tailc);
-  float4 d = c.background;
-  if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+  float4 d = dest_of(c, t);
+  if (kColorTail) d = tail_fold(tb, t, d, 0u);
   o.out[t] = over(col, d);
   if (kDebug) write_debug<K>(o, t, a, n, cnt, true, false);
 }
@@ -420,6 +450,8 @@ struct AppendSrc {
   uint32_t lid0;   // frame-local id of fragment 0 (colour-pass tail records)
   const uint32_t* mask = nullptr;  // coverage shading: sample masks (a zero mask drops the fragment)
   uint32_t full = 0;               // the pixel's sample bits
+  const uint64_t* opq = nullptr;   // the frame's opaque depth planes (early depth test), or null
+  uint32_t cov = 0;                // coverage samples (opaque plane count)
 };
 
 template <bool kTail, int kU>
@@ -439,6 +471,10 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
     if (f.mask && t[u] != kEmpty) {
       m[u] = f.mask[i] & f.full;
       if (m[u] == 0u) t[u] = kEmpty;  // no sample covered: the shader does not run
+    }
+    if (f.opq && t[u] != kEmpty) {  // behind the opaque depth in every covered sample: culled
+      m[u] = opaque_visible(f.opq, f.T, f.cov, t[u], depth_key(f.depth[i]), m[u]);
+      if (m[u] == 0u) t[u] = kEmpty;
     }
     const uint64_t n64 = (uint64_t)f.node0 + i;
     node[u] = t[u] != kEmpty && n64 < ll.cap;
@@ -509,6 +545,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBu
   a.lid0 = f.id_base;
   a.mask = f.mask;
   a.full = c.cov > 1u ? (1u << c.cov) - 1u : 0xFFFFFFFFu;
+  a.opq = c.opq_z;
+  a.cov = c.cov;
   ll_append_wave<kTail, kU>(a, s_thr, ll, tb);
 }
 
@@ -661,8 +699,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
     FrontK<K> a;
     uint32_t cnt;
     const float4 col = ll_walk_reset<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
-    float4 d = c.background;
-    if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+    float4 d = dest_of(c, t);
+    if (kColorTail) d = tail_fold(tb, t, d, 0u);
     o.out[t] = over(col, d);
     if (kDebug) write_debug<K>(o, t, a, cnt < (uint32_t)K ? cnt : (uint32_t)K, cnt, true, false);
   }
@@ -683,8 +721,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset_keys(Common c, LLBu
     FrontKeys<K> a;
     uint32_t cnt;
     const float4 col = ll_walk_reset_keys<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
-    float4 d = c.background;
-    if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+    float4 d = dest_of(c, t);
+    if (kColorTail) d = tail_fold(tb, t, d, 0u);
     o.out[t] = over(col, d);
     if (kDebug) {
       const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
@@ -726,7 +764,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_cov(Common c, LLBuf ll, T
 #pragma unroll
     for (int k = 0; k < K; ++k) masks[k] = (uint32_t)k < n ? ll.mask[(uint32_t)a.sk[k]] : 0u;
     const float4 col = over(blend_front_cov<K>(a.col, masks, n, c.cov, s_dec, s_alpha), tailc);
-    const float4 d = tb.head ? tail_fold_cov(tb, t, c.background, c.cov) : c.background;
+    const float4 d = tail_fold_cov(tb, t, c);
     o.out[t] = over(col, d);
     write_debug<K>(o, t, a, n, cnt, true, false);
   }
@@ -825,6 +863,13 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
       t = kEmpty;
     }
   }
+  if (valid && c.opq_z) {  // early depth test against the opaque pass
+    m = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(f.depth[i]), m);
+    if (m == 0u) {
+      valid = false;
+      t = kEmpty;
+    }
+  }
   const uint32_t lid = f.id_base + i;
   const Run r = same_target_run(t);
   uint32_t base = 0;
@@ -871,6 +916,10 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   if (valid && f.mask) {
     my_mask = f.mask[i] & ((1u << c.cov) - 1u);
     if (my_mask == 0u) valid = false;  // no sample covered: the shader does not run
+  }
+  if (valid && c.opq_z) {  // early depth test against the opaque pass
+    my_mask = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(f.depth[i]), my_mask);
+    if (my_mask == 0u) valid = false;
   }
   const uint32_t lid = f.id_base + i;
   bool tail = false;
@@ -964,8 +1013,8 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
     }
   }
   const float4 col = blend_front<K>(a, n, c.tab);
-  float4 d = c.background;
-  if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+  float4 d = dest_of(c, t);
+  if (kColorTail) d = tail_fold(tb, t, d, 0u);
   o.out[t] = over(col, d);
   if (kDebug) write_debug<K>(o, t, a, n, count, false, false);
 }
@@ -1008,7 +1057,7 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
     }
   }
   const float4 col = blend_front_cov<K>(cols, masks, n, c.cov, s_dec, s_alpha);
-  const float4 d = tb.head ? tail_fold_cov(tb, t, c.background, c.cov) : c.background;
+  const float4 d = tail_fold_cov(tb, t, c);
   o.out[t] = over(col, d);
   write_debug<K>(o, t, a, n, count, false, false);
 }
@@ -1027,6 +1076,7 @@ __global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, u
   const uint32_t t = f.target[i];
   if (t >= c.T) return;
   uint32_t z = depth_key(f.depth[i]);
+  if (!opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) return;
   if (z > zs[(uint64_t)(K - 1) * c.T + t]) return;
   int j = 0;
   if (z > zs[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
@@ -1056,9 +1106,13 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color(Common c, FragView f, c
   }
   bool tail = false;
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+  const uint32_t z = valid ? depth_key(f.depth[i]) : 0u;
+  if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) {
+    valid = false;
+    t = kEmpty;
+  }
   if (valid) {
     col = f.rgba[i];
-    const uint32_t z = depth_key(f.depth[i]);
     if (zs[(uint64_t)(K - 1) * c.T + t] < z) {
       tail = true;
     } else {
@@ -1097,8 +1151,8 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
 #pragma unroll
   for (int k = 0; k < K; ++k)
     if ((uint32_t)k < n) acc = over(acc, decode_premult(col[k], c.tab));
-  float4 d = c.background;
-  if (kColorTail) d = tail_fold(tb, t, c.background, 0u);
+  float4 d = dest_of(c, t);
+  if (kColorTail) d = tail_fold(tb, t, d, 0u);
   o.out[t] = over(acc, d);
   if (kDebug) {
     if (o.fk_key) {
@@ -1155,10 +1209,11 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
     }
     uint32_t count = 0;
     uint32_t bound = kEmpty;  // imgAuxDepth cleared to ~0, oitRender.cpp:348
-    float4 cb = c.background;
+    float4 cb = kCov ? c.background : dest_of(c, t);
     float4 cbs[kCov ? 8 : 1];
 #pragma unroll
-    for (int s = 0; s < (kCov ? 8 : 1); ++s) cbs[s] = c.background;
+    for (int s = 0; s < (kCov ? 8 : 1); ++s)
+      cbs[s] = (kCov && c.opq_rgba && (uint32_t)s < c.cov) ? c.opq_rgba[(uint64_t)s * c.T + t] : c.background;
     bool seen = false;
     p = prev;
     while (p != 0u && p < ll.cap) {
@@ -1243,7 +1298,7 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
     float4 d = cb;
     if constexpr (kCov) {
       front = blend_front_cov<K>(scol, smask, n, c.cov, s_dec, s_alpha);
-      if (seen) {
+      if (seen || c.opq_rgba) {
         float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int s = 0; s < 8; ++s)
@@ -1288,7 +1343,9 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, u
   if (i >= f.count) return;
   const uint32_t t = f.target[i];
   if (t >= c.T) return;
-  uint64_t z = ((uint64_t)depth_key(f.depth[i]) << 32) | encode_color_lds(f.rgba[i], s_thr);
+  const uint32_t key = depth_key(f.depth[i]);
+  if (!opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) return;
+  uint64_t z = ((uint64_t)key << 32) | encode_color_lds(f.rgba[i], s_thr);
   if (z > slots[(uint64_t)(K - 1) * c.T + t]) return;
   int j = 0;
   if (z > slots[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
@@ -1316,6 +1373,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
   bool tail = false;
   uint32_t flag = 0;
   float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
+  if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(f.depth[i]), 1u)) valid = false;
   if (valid) {
     const uint32_t col = encode_color_lds(f.rgba[i], s_thr);
     const uint64_t z = ((uint64_t)depth_key(f.depth[i]) << 32) | col;
@@ -1350,14 +1408,14 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint6
       ++n;
     }
   }
-  float4 d = c.background;
+  float4 d = dest_of(c, t);
   if (kColorTail) {
     uint32_t skip = 0;
     if (n == (uint32_t)K) {
 #pragma unroll
       for (int k = 0; k < K; ++k) skip += s[k] == s[K - 1] ? 1u : 0u;
     }
-    d = tail_fold(tb, t, c.background, skip);
+    d = tail_fold(tb, t, d, skip);
   }
   o.out[t] = over(col, d);
   if (kDebug) {
@@ -1389,6 +1447,7 @@ __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WB
     t = f.target[i];
     if (t >= c.T) t = kEmpty;
   }
+  if (t != kEmpty && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(f.depth[i]), 1u)) t = kEmpty;
   if (t != kEmpty) {
     const float4 col = f.rgba[i];
     const float view = vnear + f.depth[i] * vspan;   // stands in for -viewZ (object.vert.glsl:35)
@@ -1442,7 +1501,7 @@ __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, Re
   const float den = fmaxf(A.w, 1e-5f);
   const float4 s = make_float4(A.x / den, A.y / den, A.z / den, r);
   const float om = 1.0f - r;
-  const float4 bg = c.background;
+  const float4 bg = dest_of(c, t);
   o.out[t] = make_float4(s.x * om + bg.x * r, s.y * om + bg.y * r, s.z * om + bg.z * r, s.w * om + bg.w * r);
   if (o.counts) o.counts[t] = 0u;
 }
@@ -1483,6 +1542,72 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t frags
     uint32_t total = 0;
     for (uint32_t w = 0; w < kBlock / 64; ++w) total += s_count[w];
     if (total) atomicAdd(tb.counter, total);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Opaque pass: the opaque spheres drawn before the transparent ones
+// (oitRender.cpp:68-122) with the BlendMode::NONE pipeline (depth test LESS,
+// depth write, no blending, main.cpp:526-533)
+// ---------------------------------------------------------------------------
+
+// Planes entry of the depth clear 1.0 (oitRender.cpp:91): no fragment id.
+__device__ __forceinline__ uint64_t opaque_clear_z() { return ((uint64_t)depth_key(1.0f) << 32) | 0xFFFFFFFFull; }
+
+__global__ __launch_bounds__(kBlock) void k_opaque_clear(uint64_t* __restrict__ z, float4* __restrict__ rgba,
+                                                         uint64_t n, float4 bg) {
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  const uint64_t zc = opaque_clear_z();
+  for (uint64_t j = (uint64_t)blockIdx.x * kBlock + threadIdx.x; j < n; j += stride) {
+    z[j] = zc;
+    rgba[j] = bg;
+  }
+}
+
+// Depth phase of an opaque draw: (depth key << 32 | id) atomicMin on each
+// covered sample target, so the nearest fragment wins and, among equal
+// depths, the earliest in draw order (a later one fails LESS). A fragment at
+// or behind the clear depth writes nothing.
+template <bool kCov>
+__global__ __launch_bounds__(kBlock) void k_opaque_depth(Common c, FragView f, uint64_t* __restrict__ z) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= f.count) return;
+  const uint32_t t = f.target[i];
+  if (t >= c.T) return;
+  const uint32_t key = depth_key(f.depth[i]);
+  if (key >= depth_key(1.0f)) return;
+  const uint64_t v = ((uint64_t)key << 32) | (f.id_base + i);
+  if constexpr (kCov) {
+    const uint32_t m = f.mask[i] & ((1u << c.cov) - 1u);
+    for (uint32_t s = 0; s < c.cov; ++s)
+      if ((m >> s) & 1u)
+        (void)__hip_atomic_fetch_min(&z[(uint64_t)s * c.T + t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  } else {
+    (void)__hip_atomic_fetch_min(&z[t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Colour phase: the fragment that won a sample target writes its colour
+// there (blending off). Runs after its draw's depth phase; a later draw that
+// takes the target over rewrites the colour in its own colour phase.
+template <bool kCov>
+__global__ __launch_bounds__(kBlock) void k_opaque_color(Common c, FragView f, const uint64_t* __restrict__ z,
+                                                         float4* __restrict__ rgba) {
+  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= f.count) return;
+  const uint32_t t = f.target[i];
+  if (t >= c.T) return;
+  const uint32_t key = depth_key(f.depth[i]);
+  if (key >= depth_key(1.0f)) return;
+  const uint64_t v = ((uint64_t)key << 32) | (f.id_base + i);
+  if constexpr (kCov) {
+    const uint32_t m = f.mask[i] & ((1u << c.cov) - 1u);
+    for (uint32_t s = 0; s < c.cov; ++s) {
+      const uint64_t j = (uint64_t)s * c.T + t;
+      if (((m >> s) & 1u) && z[j] == v) rgba[j] = f.rgba[i];
+    }
+  } else {
+    if (z[t] == v) rgba[t] = f.rgba[i];
   }
 }
 
@@ -1922,6 +2047,28 @@ hipError_t launch_loop32_resolve(const Common& c, const uint32_t* zs, const uint
 hipError_t launch_interlock_resolve(const Common& c, const LLBuf& ll, const ResolveOut& o, hipStream_t s) {
   if (!ll.full || (c.cov > 1u && !ll.mask)) return hipErrorInvalidValue;
   return dispatch_k<InterlockResolve>(c.K, c, ll, o, s);
+}
+
+hipError_t launch_opaque_clear(uint64_t* z, float4* rgba, uint64_t n, float4 bg, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  uint64_t blocks = (n + kBlock - 1) / kBlock;
+  if (blocks > 8192u) blocks = 8192u;
+  hipLaunchKernelGGL(k_opaque_clear, dim3((uint32_t)blocks), dim3(kBlock), 0, s, z, rgba, n, bg);
+  return hipGetLastError();
+}
+
+hipError_t launch_opaque_draw(const Common& c, const FragView& f, uint64_t* z, float4* rgba, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  const dim3 g(blocks_for(f.count)), b(kBlock);
+  if (c.cov > 1u) {
+    if (c.cov > 8u || !f.mask) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_opaque_depth<true>, g, b, 0, s, c, f, z);
+    hipLaunchKernelGGL(k_opaque_color<true>, g, b, 0, s, c, f, z, rgba);
+  } else {
+    hipLaunchKernelGGL(k_opaque_depth<false>, g, b, 0, s, c, f, z);
+    hipLaunchKernelGGL(k_opaque_color<false>, g, b, 0, s, c, f, z, rgba);
+  }
+  return hipGetLastError();
 }
 
 hipError_t launch_finalize_srgb8(const float4* in, uint32_t W, uint32_t H, uint32_t S, uint32_t ss,

@@ -70,6 +70,11 @@ struct Common {
   float4 background;
   const CodecTables* tab;
   uint32_t cov = 0;        // coverage shading: samples per pixel (0 = off)
+  // the frame's opaque pass (null when it has none), per sample target:
+  // (depth key << 32 | id) and colour; with coverage shading sample s of
+  // pixel t is entry s * T + t
+  const uint64_t* opq_z = nullptr;
+  const float4* opq_rgba = nullptr;
 };
 
 // One frame of a batch as the batched Linked List kernels see it.
@@ -145,6 +150,11 @@ hipError_t launch_loop32_resolve(const Common& c, const uint32_t* zs, const uint
 // ordered interlock: Linked List append with shaded colours, then per-pixel
 // critical sections replayed in primitive order
 hipError_t launch_interlock_resolve(const Common& c, const LLBuf& ll, const ResolveOut& o, hipStream_t s);
+// Opaque pass: planes of n sample targets to the clear (depth 1.0, colour
+// bg), then a draw: depth test LESS with depth write (u64 atomicMin of
+// (key, id)), then the winners' colours
+hipError_t launch_opaque_clear(uint64_t* z, float4* rgba, uint64_t n, float4 bg, hipStream_t s);
+hipError_t launch_opaque_draw(const Common& c, const FragView& f, uint64_t* z, float4* rgba, hipStream_t s);
 hipError_t launch_finalize_srgb8(const float4* in, uint32_t W, uint32_t H, uint32_t S, uint32_t ss,
                                  const CodecTables* tab, uint32_t* out, hipStream_t s);
 

@@ -185,6 +185,24 @@ class Renderer:
     def clear(self, stream=None) -> None:
         _abi.check("oit_clear", self._lib.oit_clear(self._ctx, _stream_ptr(stream)))
 
+    def _frags(self, target, depth, rgba, count, coverage) -> _abi.OitFragments:
+        if count is None:
+            n = _rows(target)
+            if _rows(depth) != n or _rows(rgba) != n or (n and tuple(rgba.shape[1:]) != (4,)):
+                raise ValueError("fragment arrays disagree in length")
+        else:
+            n = int(count)
+        return _abi.OitFragments(_dptr(target) if n else None, _dptr(depth) if n else None,
+                                 _dptr(rgba) if n else None, n,
+                                 _dptr(coverage) if (n and coverage is not None) else None)
+
+    def draw_opaque(self, target, depth, rgba, stream=None, count=None, coverage=None) -> None:
+        """The frame's opaque draw (oitRender.cpp:113-122): before any append of the frame.
+        Afterwards the frame's transparent fragments are depth tested against it and the
+        composite blends over its colour. Arrays as for append()."""
+        f = self._frags(target, depth, rgba, count, coverage)
+        _abi.check("oit_draw_opaque", self._lib.oit_draw_opaque(self._ctx, ctypes.byref(f), _stream_ptr(stream)))
+
     def append(self, target, depth, rgba, stream=None, count=None, coverage=None) -> None:
         """target u32 [F], depth f32 [F], rgba f32 [F, 4] premultiplied: DeviceBuffers, or raw
         device addresses with `count`; `coverage` u32 [F] sample masks with coverage shading."""
