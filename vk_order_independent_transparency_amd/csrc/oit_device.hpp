@@ -238,6 +238,23 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
   }
 }
 
+// Per-target lock for critical sections whose shared data is read and
+// written only with agent-scope relaxed atomics (coherent across the XCDs'
+// L2 caches, so the section needs ordering but no L2 writeback or
+// invalidate, which an agent-scope acquire/release would add to every lock
+// operation). Acquire: a relaxed exchange; the section's accesses are issued
+// only after its value has returned. Release: wait until every memory
+// operation of the lane has completed, then a relaxed store of 0.
+__device__ __forceinline__ bool lock_try(uint32_t* l) {
+  return __hip_atomic_exchange(l, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+}
+__device__ __forceinline__ void lock_release(uint32_t* l) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unconstrained
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_store(l, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // list_insert_desc for a lane that already tried the head and learned that a
 // larger element is there: it starts at that element's link word `p0` and
 // reads it before its first atomic.

@@ -897,13 +897,62 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
 // Spinlock: oitSpinlock.frag.glsl
 // ---------------------------------------------------------------------------
 
+// One critical section of oitSpinlock.frag.glsl:86-123 for the fragment
+// (key, col, id, mask) of target t, run by the lane that holds t's lock:
+// fill while fewer than K are stored (:89-93), else replace the furthest
+// (first index wins, :96-107) when the newcomer is nearer (:109-117). Sets
+// the tail output: the evicted element, or the fragment itself.
+template <int K>
+__device__ __forceinline__ void spin_section(const Common& c, const KBuf& kb, uint32_t t, uint32_t key, uint32_t col,
+                                             uint32_t id, uint32_t mask, bool& tail, uint32_t& tail_id,
+                                             uint32_t& tail_col, uint32_t& tail_mask) {
+  const uint32_t cnt = ld_relaxed(&kb.count[t]);
+  st_relaxed(&kb.count[t], cnt + 1u);
+  if (cnt < (uint32_t)K) {
+    const uint64_t idx = (uint64_t)cnt * c.T + t;
+    st_relaxed(&kb.col[idx], col);
+    st_relaxed(&kb.key[idx], key);
+    st_relaxed(&kb.id[idx], id);
+    if (kb.mask) st_relaxed(&kb.mask[idx], mask);
+    return;
+  }
+  uint32_t fur = 0;
+  uint64_t mx = 0;
+#pragma unroll
+  for (int j = 0; j < K; ++j) {
+    const uint64_t idx = (uint64_t)j * c.T + t;
+    const uint64_t v = ((uint64_t)ld_relaxed(&kb.key[idx]) << 32) | ld_relaxed(&kb.id[idx]);
+    if (v > mx) {
+      mx = v;
+      fur = (uint32_t)j;
+    }
+  }
+  const uint64_t me = ((uint64_t)key << 32) | id;
+  tail = true;
+  if (mx > me) {
+    const uint64_t idx = (uint64_t)fur * c.T + t;
+    tail_col = ld_relaxed(&kb.col[idx]);
+    tail_id = (uint32_t)mx;
+    if (kb.mask) {
+      tail_mask = ld_relaxed(&kb.mask[idx]);
+      st_relaxed(&kb.mask[idx], mask);
+    }
+    st_relaxed(&kb.col[idx], col);
+    st_relaxed(&kb.key[idx], key);
+    st_relaxed(&kb.id[idx], id);
+    st_relaxed(&kb.depth[t], (uint32_t)(mx >> 32));
+  }
+}
+
 // Colour pass (oitSpinlock.frag.glsl:49-129). The critical section runs under
 // a per-target lock taken with an atomic exchange; one lane of a wave holds
 // the lock while its neighbours retry on the next trip. Elements are ordered
-// by (depth key, id); the
-// furthest is found by a scan with first-index-wins (:96-107), replaced when
-// the newcomer is nearer (:109-117). Evicted and rejected fragments are
-// recorded for the tail with their packed sRGB8 colour.
+// by (depth key, id); evicted and rejected fragments are recorded for the
+// tail with their packed sRGB8 colour. Two consecutive lanes of one target (a
+// sphere's front and back) that both pass the early depth test run their
+// critical sections back to back under one acquisition by the even lane (the
+// two sections in id order, one of the orders the lock allows), so they do
+// not contend for the lock; the odd lane gets its tail output back by shuffle.
 template <int K, bool kTail>
 __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBuf kb, TailBuf tb) {
   __shared__ float s_thr[257];
@@ -929,61 +978,44 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   uint32_t tail_mask = my_mask;
   uint32_t my_key = 0u, my_col = 0u;
   if (valid) {
-    const uint32_t key = depth_key(f.depth[i]);
-    const uint32_t col = encode_color_lds(f.rgba[i], s_thr);
-    tail_col = col;
-    pending = key <= ld_relaxed(&kb.depth[t]);  // :66-69 early depth test
+    my_key = depth_key(f.depth[i]);
+    my_col = encode_color_lds(f.rgba[i], s_thr);
+    tail_col = my_col;
+    pending = my_key <= ld_relaxed(&kb.depth[t]);  // :66-69 early depth test
     if (!pending) tail = true;
-    my_key = key;
-    my_col = col;
   }
+  // pairs: the even lane of a same-target run takes its odd neighbour along
+  const Run r = same_target_run(pending ? t : kEmpty);
+  const bool lead = pending && ((r.lane - r.start_lane) & 1u) == 0u && r.lane < r.end_lane;
+  const bool follower = pending && ((r.lane - r.start_lane) & 1u) == 1u;
+  const uint32_t p_key = __shfl_down(my_key, 1u), p_col = __shfl_down(my_col, 1u);
+  const uint32_t p_mask = __shfl_down(my_mask, 1u);
+  if (follower) pending = false;
+  bool p_tail = false;
+  uint32_t p_tail_id = lid + 1u, p_tail_col = p_col, p_tail_mask = p_mask;
   // The lock loop (:75-123) is driven by a wave-wide ballot: a lane that takes
   // the lock runs the critical section and releases it in the same trip, and
   // the compiler cannot hoist the exchange into an inner spin that a lock
   // holder of the same wave would wait behind.
   while (__ballot(pending)) {
     if (pending) {
-      if (__hip_atomic_exchange(&kb.lock[t], 1u, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == 0u) {
-        const uint32_t cnt = ld_relaxed(&kb.count[t]);  // :86-87
-        st_relaxed(&kb.count[t], cnt + 1u);
-        if (cnt < (uint32_t)K) {                          // :89-93
-          const uint64_t idx = (uint64_t)cnt * c.T + t;
-          st_relaxed(&kb.col[idx], my_col);
-          st_relaxed(&kb.key[idx], my_key);
-          st_relaxed(&kb.id[idx], lid);
-          if (kb.mask) st_relaxed(&kb.mask[idx], my_mask);
-        } else {
-          uint32_t fur = 0;                               // :96-107
-          uint64_t mx = 0;
-#pragma unroll
-          for (int j = 0; j < K; ++j) {
-            const uint64_t idx = (uint64_t)j * c.T + t;
-            const uint64_t v = ((uint64_t)ld_relaxed(&kb.key[idx]) << 32) | ld_relaxed(&kb.id[idx]);
-            if (v > mx) {
-              mx = v;
-              fur = (uint32_t)j;
-            }
-          }
-          const uint64_t me = ((uint64_t)my_key << 32) | lid;
-          if (mx > me) {                                  // :109-117
-            const uint64_t idx = (uint64_t)fur * c.T + t;
-            tail_col = ld_relaxed(&kb.col[idx]);
-            tail_id = (uint32_t)mx;
-            if (kb.mask) {
-              tail_mask = ld_relaxed(&kb.mask[idx]);
-              st_relaxed(&kb.mask[idx], my_mask);
-            }
-            st_relaxed(&kb.col[idx], my_col);
-            st_relaxed(&kb.key[idx], my_key);
-            st_relaxed(&kb.id[idx], lid);
-            st_relaxed(&kb.depth[t], (uint32_t)(mx >> 32));
-          }
-          tail = true;
-        }
-        __hip_atomic_exchange(&kb.lock[t], 0u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+      if (lock_try(&kb.lock[t])) {
+        spin_section<K>(c, kb, t, my_key, my_col, lid, my_mask, tail, tail_id, tail_col, tail_mask);
+        if (lead)
+          spin_section<K>(c, kb, t, p_key, p_col, lid + 1u, p_mask, p_tail, p_tail_id, p_tail_col, p_tail_mask);
+        lock_release(&kb.lock[t]);
         pending = false;
       }
     }
+  }
+  // the follower's outcome, computed by its lead (lane - 1)
+  const uint32_t b_tail = __shfl_up(p_tail ? 1u : 0u, 1u), b_id = __shfl_up(p_tail_id, 1u);
+  const uint32_t b_col = __shfl_up(p_tail_col, 1u), b_mask = __shfl_up(p_tail_mask, 1u);
+  if (follower) {
+    tail = b_tail != 0u;
+    tail_id = b_id;
+    tail_col = b_col;
+    tail_mask = b_mask;
   }
   if constexpr (kTail) {
     float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1335,25 +1367,57 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
 // decrease within a frame, so a stale value only lets more fragments try),
 // then the atomicMin chain (:109-121). Slots end as the K smallest keys for
 // any arrival order; evictions are classified afterwards.
+// Two consecutive lanes of one target (a sphere's front and back fragment)
+// run one chain that carries both keys: at each slot the smaller carried key
+// goes in, the key that comes out joins the larger one, and the chain ends
+// when both are placed or pushed past slot K-1. The multiset of slots and
+// carried keys is the same as after the two single chains, and the slots
+// stay sorted, so the result is identical; the pair costs one chain instead
+// of two and its lanes do not contend for the same slots.
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, uint64_t* __restrict__ slots) {
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= f.count) return;
-  const uint32_t t = f.target[i];
-  if (t >= c.T) return;
-  const uint32_t key = depth_key(f.depth[i]);
-  if (!opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) return;
-  uint64_t z = ((uint64_t)key << 32) | encode_color_lds(f.rgba[i], s_thr);
-  if (z > slots[(uint64_t)(K - 1) * c.T + t]) return;
+  uint32_t t = i < f.count ? f.target[i] : kEmpty;
+  if (t >= c.T) t = kEmpty;
+  uint64_t z = kEmpty64;
+  if (t != kEmpty) {
+    const uint32_t key = depth_key(f.depth[i]);
+    if (opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) z = ((uint64_t)key << 32) | encode_color_lds(f.rgba[i], s_thr);
+    else t = kEmpty;
+  }
+  const Run r = same_target_run(t);
+  const uint32_t zn_lo = __shfl_down((uint32_t)z, 1u);
+  const uint32_t zn_hi = __shfl_down((uint32_t)(z >> 32), 1u);
+  const uint64_t zn = ((uint64_t)zn_hi << 32) | zn_lo;
+  const bool lead = t != kEmpty && ((r.lane - r.start_lane) & 1u) == 0u;
+  if (!lead) return;  // the odd lanes of a run travel with their even neighbour
+  bool hasb = r.lane < r.end_lane;
+  uint64_t a = z, b = 0;
+  if (hasb) {
+    a = z < zn ? z : zn;
+    b = z < zn ? zn : z;
+  }
+  if (a > slots[(uint64_t)(K - 1) * c.T + t]) return;
   int j = 0;
-  if (z > slots[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
+  if (a > slots[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
   for (; j < K; ++j) {
     const uint64_t old =
-        __hip_atomic_fetch_min(&slots[(uint64_t)j * c.T + t], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == kEmpty64) break;
-    z = old > z ? old : z;
+        __hip_atomic_fetch_min(&slots[(uint64_t)j * c.T + t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == kEmpty64) {  // a placed in an empty slot: b (if any) goes to the next one
+      if (!hasb) break;
+      a = b;
+      hasb = false;
+      continue;
+    }
+    const uint64_t up = old > a ? old : a;  // the key that leaves slot j
+    if (hasb && b < up) {
+      a = b;
+      b = up;
+    } else {
+      a = up;
+    }
   }
 }
 
