@@ -1576,31 +1576,62 @@ __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, Re
 // arrive roughly in id order, so most inserts succeed at the head.
 // ---------------------------------------------------------------------------
 
+// Each wave takes kU chunks of 64 ids per trip: their records are loaded
+// together and their first head attempts are in flight together; only lanes
+// that find a larger value at the head walk.
+template <int kU>
 __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t frags) {
   __shared__ uint32_t s_count[kBlock / 64];
   const uint32_t n = frags < tb.cap ? frags : tb.cap;
   const uint32_t vcap = tail_value_cap(tb.cap);
+  const uint32_t lane = lane_id();
+  const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+  const uint64_t step = (uint64_t)gridDim.x * (kBlock / 64) * 64u * kU;
   uint32_t mine = 0;
-  // every lane runs the same number of trips: runs and inserts are wave-wide
-  for (uint32_t base = blockIdx.x * kBlock; base < n; base += gridDim.x * kBlock) {
-    const uint32_t lid = base + threadIdx.x;
-    const uint64_t rec = lid < n ? tb.rec[lid] : kEmpty64;
-    const bool tail = rec != kEmpty64;
-    const uint32_t t = tail ? (uint32_t)(rec >> 32) : kEmpty;
-    const uint32_t v = (uint32_t)rec;
-    mine += tail ? 1u : 0u;
-    // consecutive ids of one target: chained newest to oldest here, then
-    // published with one insert (no other value of the list lies between)
-    const Run r = same_target_run(t);
-    const uint32_t v_prev = __shfl_up(v, 1u);
-    const uint32_t lo = __shfl(v, (int)r.start_lane);
-    if (tail && !r.start) st_relaxed(&tb.next[lid], v_prev);
-    const bool ins = tail && r.end;
-    list_insert_desc<1, 1>(ins, ins ? &tb.head[t] : nullptr, tb.next, lo, v, vcap);
+  // every lane of a wave runs the same number of trips: runs and inserts are wave-wide
+  for (uint64_t base = wave * 64u * kU; base < n; base += step) {
+    uint64_t rec[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const uint64_t lid = base + (uint64_t)u * 64u + lane;
+      rec[u] = lid < n ? tb.rec[lid] : kEmpty64;
+    }
+    uint32_t t[kU], v[kU], lo[kU];
+    bool ins[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const uint32_t lid = (uint32_t)(base + (uint64_t)u * 64u + lane);
+      const bool tail = rec[u] != kEmpty64;
+      t[u] = tail ? (uint32_t)(rec[u] >> 32) : kEmpty;
+      v[u] = (uint32_t)rec[u];
+      mine += tail ? 1u : 0u;
+      // consecutive ids of one target: chained newest to oldest here, then
+      // published with one insert (no other value of the list lies between)
+      const Run r = same_target_run(t[u]);
+      const uint32_t v_prev = __shfl_up(v[u], 1u);
+      lo[u] = __shfl(v[u], (int)r.start_lane);
+      if (tail && !r.start) st_relaxed(&tb.next[lid], v_prev);
+      ins[u] = tail && r.end;
+    }
+    uint32_t old[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) old[u] = ins[u] ? max_relaxed(&tb.head[t[u]], v[u]) : 0u;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (ins[u] && old[u] < lo[u]) {
+        st_relaxed(&tb.next[tail_index(lo[u])], old[u]);
+        ins[u] = false;
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {  // a larger value was already at the head: continue behind it
+      const bool walk = ins[u] && old[u] < vcap;
+      list_insert_desc_from<1, 1>(walk, walk ? &tb.next[tail_index(old[u])] : nullptr, tb.next, lo[u], v[u], vcap);
+    }
   }
   // tail fragments of the frame: one atomic per block
   for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o);
-  if (lane_id() == 0) s_count[threadIdx.x >> 6] = mine;
+  if (lane == 0) s_count[threadIdx.x >> 6] = mine;
   __syncthreads();
   if (threadIdx.x == 0) {
     uint32_t total = 0;
@@ -1960,10 +1991,11 @@ hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t h
 }
 
 hipError_t launch_tail_link(const TailBuf& tb, uint32_t frags, hipStream_t s) {
-  uint32_t blocks = blocks_for(frags < tb.cap ? frags : tb.cap);
+  const uint64_t n = frags < tb.cap ? frags : tb.cap;
+  uint64_t blocks = (n + 4u * kBlock - 1) / (4u * kBlock);
   if (blocks > 4096u) blocks = 4096u;
   if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tail_link, dim3(blocks), dim3(kBlock), 0, s, tb, frags);
+  hipLaunchKernelGGL(k_tail_link<4>, dim3((uint32_t)blocks), dim3(kBlock), 0, s, tb, frags);
   return hipGetLastError();
 }
 
