@@ -206,8 +206,28 @@ def bench_serve(seed, n, rates, slots):
                "peak_held_GB": round(stt.peak_held_bytes / 2 ** 30, 2), "peak_in_flight": int(stt.peak_in_flight)}
         log(json.dumps(row))
         out.append(row)
+    # the same requests all at once: the slots against one request at a time
+    # (one slot, so each request's launches run alone on the device); both
+    # with warm resident contexts
+    reqs, _ = serve.served_workload(seed, n, rates[-1])
+    burst = {}
+    for name, server in (("slots", srv), ("one_at_a_time", None)):
+        one = server is None
+        if one:
+            server = serve.Server(0, slots=1)
+            server.replay(reqs, [0.0] * len(reqs))
+        done = server.replay(reqs, [0.0] * len(reqs))
+        ok = [c for c in done if c.status == 0]
+        span_us = max(c.t_done_us for c in ok) - min(c.t_submit_us for c in ok)
+        burst[name] = {"G_fragments_per_s": round(sum(c.fragments for c in ok) / span_us / 1e3, 3),
+                       "span_s": round(span_us / 1e6, 4), "completed": len(ok)}
+        if one:
+            server.close()
+    burst["slots"]["slots"] = slots
+    burst["speedup"] = round(burst["slots"]["G_fragments_per_s"] / burst["one_at_a_time"]["G_fragments_per_s"], 3)
+    log("served burst vs one at a time: " + json.dumps(burst))
     srv.close()
-    return out
+    return out, burst
 
 
 def _under_load(served):
@@ -318,10 +338,10 @@ def main():
     algorithms = None if args.no_algorithms else bench_algorithms(tg_d, d_d, c_d, F, p, 10, stream)
 
     batch = None if args.no_batch else bench_batch(1, args.batch_frames)
-    served = None
+    served = served_burst = None
     if not args.no_serve:
-        served = bench_serve(1, args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
-                             args.serve_slots)
+        served, served_burst = bench_serve(1, args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
+                                           args.serve_slots)
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -371,6 +391,7 @@ def main():
         "algorithms": algorithms,
         "batch": batch,
         "serve": served,
+        "serve_vs_one_at_a_time": served_burst,
         "stages": stages,
         "ll_overflow": int(st.ll_overflow),
         "cpu_baseline": cpu,

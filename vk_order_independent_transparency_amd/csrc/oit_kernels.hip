@@ -1100,23 +1100,55 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
 
 // Depth pass (:57-100): the K nearest distinct depth keys per target, sorted,
 // by a u32 atomicMin chain that stops at an empty slot or its own key; the two
-// pretests use plain loads (slots only decrease within the pass).
+// pretests use plain loads (slots only decrease within the pass). Two
+// consecutive lanes of one target (a sphere's front and back) run one chain
+// carrying both keys, as in k_loop64_store, with the set rule: a carried key
+// that meets its equal in a slot, or equals the other carried key, is done.
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, uint32_t* __restrict__ zs) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= f.count) return;
-  const uint32_t t = f.target[i];
-  if (t >= c.T) return;
-  uint32_t z = depth_key(f.depth[i]);
-  if (!opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) return;
-  if (z > zs[(uint64_t)(K - 1) * c.T + t]) return;
+  uint32_t t = i < f.count ? f.target[i] : kEmpty;
+  if (t >= c.T) t = kEmpty;
+  uint32_t z = 0u;
+  if (t != kEmpty) {
+    z = depth_key(f.depth[i]);
+    if (!opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) t = kEmpty;
+  }
+  const Run r = same_target_run(t);
+  const uint32_t zn = __shfl_down(z, 1u);
+  const bool lead = t != kEmpty && ((r.lane - r.start_lane) & 1u) == 0u;
+  if (!lead) return;  // the odd lanes of a run travel with their even neighbour
+  uint32_t a = z, b = 0u;
+  bool hasb = r.lane < r.end_lane && zn != z;
+  if (hasb) {
+    a = z < zn ? z : zn;
+    b = z < zn ? zn : z;
+  }
+  if (a > zs[(uint64_t)(K - 1) * c.T + t]) return;
   int j = 0;
-  if (z > zs[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
+  if (a > zs[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
   for (; j < K; ++j) {
     const uint32_t old =
-        __hip_atomic_fetch_min(&zs[(uint64_t)j * c.T + t], z, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == kEmpty || old == z) break;
-    z = old > z ? old : z;
+        __hip_atomic_fetch_min(&zs[(uint64_t)j * c.T + t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (old == kEmpty || old == a) {  // a is in slot j now (or was already): b goes on
+      if (!hasb) break;
+      a = b;
+      hasb = false;
+      continue;
+    }
+    if (old < a) continue;  // a smaller key holds slot j: a goes on unchanged
+    // a took slot j and old comes out
+    if (!hasb) {
+      a = old;
+    } else if (old == b) {  // the two carried keys are equal: one of them is done
+      a = b;
+      hasb = false;
+    } else if (old < b) {
+      a = old;
+    } else {
+      a = b;
+      b = old;
+    }
   }
 }
 
@@ -1545,12 +1577,12 @@ __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WB
       lr += ul;
     }
   }
-  if (t != kEmpty && r.end) {
-    float* acc = reinterpret_cast<float*>(&wb.accum[t]);
-    unsafeAtomicAdd(acc + 0, wc.x);
-    unsafeAtomicAdd(acc + 1, wc.y);
-    unsafeAtomicAdd(acc + 2, wc.z);
-    unsafeAtomicAdd(acc + 3, wc.w);
+  if (t != kEmpty && r.end) {  // channel planes: each atomic instruction of a wave stays in few cache lines
+    float* acc = reinterpret_cast<float*>(wb.accum);
+    unsafeAtomicAdd(acc + t, wc.x);
+    unsafeAtomicAdd(acc + (uint64_t)c.T + t, wc.y);
+    unsafeAtomicAdd(acc + 2ull * c.T + t, wc.z);
+    unsafeAtomicAdd(acc + 3ull * c.T + t, wc.w);
     unsafeAtomicAdd(&wb.reveal[t], lr);
   }
 }
@@ -1560,7 +1592,8 @@ __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WB
 __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, ResolveOut o) {
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
-  const float4 A = wb.accum[t];
+  const float* acc = reinterpret_cast<const float*>(wb.accum);
+  const float4 A = make_float4(acc[t], acc[(uint64_t)c.T + t], acc[2ull * c.T + t], acc[3ull * c.T + t]);
   const float r = exp2f(wb.reveal[t]);  // product of (1 - alpha), kept as a log2 sum
   const float den = fmaxf(A.w, 1e-5f);
   const float4 s = make_float4(A.x / den, A.y / den, A.z / den, r);

@@ -52,7 +52,7 @@ struct KBuf {          // Simple / Spinlock
 };
 
 struct WBBuf {
-  float4* accum;      // [T]
+  float4* accum;      // 4 T floats: planes of r, g, b, a (plane k at ((float*)accum) + k T)
   float* reveal;      // [T] log2 of the revealage (sum of log2(1 - alpha)); 0 = fully revealed
 };
 
