@@ -72,7 +72,8 @@ def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream):
     rows = []
     for alg, name, K in ((1, "linkedlist", 8), (1, "linkedlist", 16), (3, "loop64", 8), (3, "loop64", 16),
                          (4, "spinlock", 8), (4, "spinlock", 16), (0, "simple", 8), (0, "simple", 16),
-                         (6, "weighted", 1)):
+                         (6, "weighted", 1), (2, "loop32", 8), (2, "loop32", 16), (5, "interlock", 8),
+                         (5, "interlock", 16)):
         with Renderer(0) as r:
             r.configure(FrameConfig(algorithm=alg, layers=K, tail_blend=alg != 6, width=p.width, height=p.height,
                                     ll_pool_nodes=F + F // 2 if alg == 1 else 0, max_fragments=F))
@@ -92,6 +93,10 @@ def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream):
             nbytes = sum(rl.loop64_stage_bytes(F, P, K).values())
         elif alg in (0, 4):
             nbytes = sum(rl.kbuffer_stage_bytes(F, P, K, alg == 4).values())
+        elif alg == 2:
+            nbytes = sum(rl.loop32_stage_bytes(F, P, K).values())
+        elif alg == 5:
+            nbytes = sum(rl.interlock_stage_bytes(F, P).values())
         else:
             nbytes = sum(rl.wboit_stage_bytes(F, P).values())
         gbs, fr = rl.frac(nbytes, ms / 1e3)

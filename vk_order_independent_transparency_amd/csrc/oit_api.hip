@@ -982,13 +982,19 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
       if (e != hipSuccess) return status_of(e);
     }
   }
-  if (link_tails && tb.head && tb.next) {
-    e = staged(c, kStTailLink, s, [&] { return oit::launch_tail_link(tb, (uint32_t)c->frame_frags, s); });
+  // Linked List: only fragments whose node would be at or past the pool's
+  // end (id >= cap - 1) can be tails, and only those have records
+  const uint64_t first_tail = alg == OIT_LINKEDLIST ? (c->ll_cap ? c->ll_cap - 1 : 0) : 0;
+  const bool no_tails = !tb.next || first_tail >= c->frame_frags;
+  if (link_tails && tb.head && !no_tails) {
+    e = staged(c, kStTailLink, s, [&] {
+      return oit::launch_tail_link(tb, (uint32_t)first_tail, (uint32_t)c->frame_frags, s);
+    });
     if (e != hipSuccess) return status_of(e);
   }
   c->resolved = true;
   oit::TailBuf tbr = tb;
-  if (!tb.next) tbr.head = nullptr;  // no tail records can exist
+  if (no_tails) tbr.head = nullptr;  // no tail records can exist
   switch (alg) {
     case OIT_LINKEDLIST: {
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};

@@ -522,9 +522,10 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
   for (int u = 0; u < kU; ++u) {
     const uint64_t om = __ballot(ovf[u]);
     if (om && lane == (uint32_t)__builtin_ctzll(om)) atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
-    if constexpr (kTail) {
-      const uint32_t lid = (uint32_t)(f.lid0 + f.first + (uint64_t)u * 64u + lane);
-      tail_record_own(ovf[u], t[u], lid, 0u, col[u], tb, m[u]);
+    if constexpr (kTail) {  // records exist only where the pool can overflow (node >= cap)
+      const uint64_t i = f.first + (uint64_t)u * 64u + lane;
+      const uint32_t lid = (uint32_t)(f.lid0 + i);
+      if ((uint64_t)f.node0 + i >= ll.cap) tail_record_own(ovf[u], t[u], lid, 0u, col[u], tb, m[u]);
     }
   }
 }
@@ -1613,7 +1614,7 @@ __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, Re
 // together and their first head attempts are in flight together; only lanes
 // that find a larger value at the head walk.
 template <int kU>
-__global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t frags) {
+__global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t first, uint32_t frags) {
   __shared__ uint32_t s_count[kBlock / 64];
   const uint32_t n = frags < tb.cap ? frags : tb.cap;
   const uint32_t vcap = tail_value_cap(tb.cap);
@@ -1622,7 +1623,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t frags
   const uint64_t step = (uint64_t)gridDim.x * (kBlock / 64) * 64u * kU;
   uint32_t mine = 0;
   // every lane of a wave runs the same number of trips: runs and inserts are wave-wide
-  for (uint64_t base = wave * 64u * kU; base < n; base += step) {
+  for (uint64_t base = first + wave * 64u * kU; base < n; base += step) {
     uint64_t rec[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -2023,12 +2024,12 @@ hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t h
   return hipGetLastError();
 }
 
-hipError_t launch_tail_link(const TailBuf& tb, uint32_t frags, hipStream_t s) {
+hipError_t launch_tail_link(const TailBuf& tb, uint32_t first, uint32_t frags, hipStream_t s) {
   const uint64_t n = frags < tb.cap ? frags : tb.cap;
-  uint64_t blocks = (n + 4u * kBlock - 1) / (4u * kBlock);
+  if (first >= n) return hipSuccess;
+  uint64_t blocks = (n - first + 4u * kBlock - 1) / (4u * kBlock);
   if (blocks > 4096u) blocks = 4096u;
-  if (blocks == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_tail_link<4>, dim3((uint32_t)blocks), dim3(kBlock), 0, s, tb, frags);
+  hipLaunchKernelGGL(k_tail_link<4>, dim3((uint32_t)blocks), dim3(kBlock), 0, s, tb, first, frags);
   return hipGetLastError();
 }
 

@@ -62,6 +62,25 @@ def wboit_stage_bytes(F: int, P: int) -> dict:
     }
 
 
+def loop32_stage_bytes(F: int, P: int, K: int) -> dict:
+    """Loop32 (two geometry passes): depth planes 4KP + (id, colour) planes 8KP
+    cleared; depth pass 8F in (target, depth) + at least one 4-byte slot
+    read-modify-write 8F; colour pass 24F in + the K-th depth 4F + one 8-byte
+    slot read-modify-write 16F; composite 12KP + 16P."""
+    return {
+        "clear": 12 * K * P,
+        "store": 8 * F + 8 * F,
+        "classify": 24 * F + 4 * F + 16 * F,
+        "resolve": 12 * K * P + 16 * P,
+    }
+
+
+def interlock_stage_bytes(F: int, P: int) -> dict:
+    """Ordered interlock: the Linked List append plus the shaded colour per
+    node (16F); the resolve walks the nodes like the Linked List composite."""
+    return {"clear": 0, "store": 44 * F + 16 * F, "resolve": 8 * P + 16 * F + 16 * P}
+
+
 def frac(bytes_: float, seconds: float) -> tuple[float, float]:
     """Returns (GB/s, fraction of the HBM peak)."""
     gbs = bytes_ / seconds / 1e9
