@@ -158,28 +158,35 @@ def bench_batch(seed, frames_n):
     for r in rends:
         r.close()
     del seq_out
-    b = Batch(0, concurrent=True, graph=True)
-    b.run(frames, st)
-    st.synchronize()
-    b.tune(frames, st)
-    b.run(frames, st)
-    st.synchronize()
-    e0, e1 = Event(), Event()
-    runs = []
-    for _ in range(5):
-        e0.record(st)
+    def timed(frame_order):
+        b = Batch(0, concurrent=True, graph=True, frame_order=frame_order)
         b.run(frames, st)
-        e1.record(st)
-        e1.synchronize()
-        runs.append(e0.elapsed_ms(e1))
-    bat_ms = min(runs)
-    exact = all(np.array_equal(f.out.numpy().view(np.uint32), w.view(np.uint32)) for f, w in zip(frames, want))
-    t = b.tuning
-    b.close()
+        st.synchronize()
+        b.tune(frames, st)
+        b.run(frames, st)
+        st.synchronize()
+        e0, e1 = Event(), Event()
+        runs = []
+        for _ in range(5):
+            e0.record(st)
+            b.run(frames, st)
+            e1.record(st)
+            e1.synchronize()
+            runs.append(e0.elapsed_ms(e1))
+        ok = all(np.array_equal(f.out.numpy().view(np.uint32), w.view(np.uint32)) for f, w in zip(frames, want))
+        t = b.tuning
+        b.close()
+        return min(runs), ok, t
+
+    order_ms, order_ok, _ = timed(True)
+    bat_ms, exact, t = timed(False)
+    exact = exact and order_ok
+    log(f"batch frame after frame: {order_ms:.3f} ms, interleaved: {bat_ms:.3f} ms")
     log(f"batch: sequential {seq_ms:.3f} ms, batched {bat_ms:.3f} ms, {seq_ms / bat_ms:.2f}x, bit-exact {exact}")
     return {"workload": f"batch_{len(frames)}_frames_seed{seed}", "frames": len(frames), "fragments": total,
             "sequential_ms": round(seq_ms, 4), "batched_ms": round(bat_ms, 4),
             "speedup": round(seq_ms / bat_ms, 3), "G_fragments_per_s": round(total / bat_ms / 1e6, 3),
+            "frame_order_ms": round(order_ms, 4),
             "bit_exact_vs_sequential": bool(exact), "append_chunks": int(t.append_chunks)}
 
 

@@ -306,7 +306,10 @@ typedef struct oit_batch_frame {
 
 enum {
   OIT_BATCH_CONCURRENT = 0x1, /* resolve groups run on forked streams (join before return) */
-  OIT_BATCH_GRAPH      = 0x2  /* a run is captured once per frame set and replayed as a hipGraph */
+  OIT_BATCH_GRAPH      = 0x2, /* a run is captured once per frame set and replayed as a hipGraph */
+  OIT_BATCH_FRAME_ORDER = 0x4 /* append frame after frame; by default the frames' fragment slices are
+                                 interleaved so that each frame advances in draw order with a share of
+                                 the waves in flight (fewer same-pixel inserts at once, same results) */
 };
 
 /* Kernel choices of a batch: append chunks per wave (4 or 8) and, per

@@ -1,7 +1,8 @@
 """Batched Linked List (oit_batch_*) against the single-frame Linked List:
 every frame of a mixed batch (sizes, K, tail flag, sample planes, empty and
 deep clustered frames) must give the single-frame image bit for bit, in
-every launch mode (one stream, forked groups, captured graph), on repeated
+every launch mode (one stream, forked groups, captured graph; frames'
+appends interleaved or frame after frame), on repeated
 runs (the pool is reset by the resolve), after growth and after tuning."""
 import numpy as np
 import pytest
@@ -87,11 +88,13 @@ def mixed():
     return st, fr, want
 
 
-@pytest.mark.parametrize("concurrent,graph", [(False, False), (True, False), (False, True), (True, True)])
-def test_batch_equals_single_frame(mixed, concurrent, graph):
+@pytest.mark.parametrize("concurrent,graph,frame_order", [(False, False, False), (True, False, False),
+                                                          (False, True, False), (True, True, False),
+                                                          (False, False, True), (True, True, True)])
+def test_batch_equals_single_frame(mixed, concurrent, graph, frame_order):
     st, fr, want = mixed
-    _say("batch", concurrent, graph, [f.count for f in fr.frames])
-    with Batch(0, concurrent=concurrent, graph=graph) as b:
+    _say("batch", concurrent, graph, frame_order, [f.count for f in fr.frames])
+    with Batch(0, concurrent=concurrent, graph=graph, frame_order=frame_order) as b:
         sz = b.query(fr.frames)
         assert sz.frames == len(MIXED) and sz.groups == 9
         for rep in range(3):  # later runs reuse the pool the resolve left clean

@@ -47,6 +47,7 @@ VARIANT_LL_KEYS_ONLY = 0x8
 
 BATCH_CONCURRENT = 0x1
 BATCH_GRAPH = 0x2
+BATCH_FRAME_ORDER = 0x4
 
 OIT_PROFILE_STAGES = 6
 PROFILE_STAGE_NAMES = ("clear", "store", "link", "classify", "tail_link", "resolve")

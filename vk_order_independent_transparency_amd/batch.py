@@ -65,9 +65,10 @@ def _frames_c(frames):
 
 
 class Batch:
-    def __init__(self, device: int = 0, concurrent: bool = False, graph: bool = False):
+    def __init__(self, device: int = 0, concurrent: bool = False, graph: bool = False, frame_order: bool = False):
         self.handle = ctypes.c_void_p(0)
-        flags = (_abi.BATCH_CONCURRENT if concurrent else 0) | (_abi.BATCH_GRAPH if graph else 0)
+        flags = ((_abi.BATCH_CONCURRENT if concurrent else 0) | (_abi.BATCH_GRAPH if graph else 0)
+                 | (_abi.BATCH_FRAME_ORDER if frame_order else 0))
         _abi.check("oit_batch_create", _abi.lib().oit_batch_create(ctypes.byref(self.handle), device, flags))
 
     def query(self, frames) -> _abi.OitBatchSizes:

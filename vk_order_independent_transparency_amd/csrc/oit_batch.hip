@@ -30,7 +30,8 @@ using oit::LLBuf;
 namespace {
 
 constexpr uint32_t kTile = 256;        // resolve tile (one block of targets)
-constexpr uint32_t kFragAlign = 512;   // append unit at 8 chunks per wave
+constexpr uint32_t kFragAlign = 512;   // append unit at 8 chunks per wave (one schedule slice)
+constexpr uint32_t kRounds = 1024;     // rounds of the interleaved append schedule
 constexpr uint64_t kMaxPoolSlots = 0xFFFFF000ull;  // node indices stay below kUnset
 constexpr uint64_t kMaxHeads = 0xFFFFFF00ull;
 constexpr uint64_t kMaxFrameFragments = 0x7FFFFFFEull;
@@ -76,6 +77,8 @@ struct Group {
 
 struct Plan {
   std::vector<BatchFrameDev> table;  // group order
+  std::vector<uint32_t> sched;       // interleaved append schedule (BatchArgs::sched)
+  uint32_t rounds = 0;
   std::vector<Group> groups;
   uint64_t pool_slots = 0;  // padded fragments (nodes without the terminator)
   uint64_t heads = 0;       // padded targets
@@ -146,6 +149,31 @@ int make_plan(const oit_batch_frame* fr, uint32_t n, Plan& p) {
   }
   p.pool_slots = frag;
   p.heads = tgt;
+  // interleaved append: up to kRounds rounds; round q takes slices
+  // floor(q S / R) .. floor((q + 1) S / R) - 1 of every frame
+  uint64_t max_slices = 0;
+  std::vector<uint64_t> sl(n);
+  for (uint32_t j = 0; j < n; ++j) {
+    sl[j] = (p.table[j].count + oit::kBatchSlice - 1) / oit::kBatchSlice;
+    max_slices = std::max(max_slices, sl[j]);
+  }
+  const uint32_t R = (uint32_t)std::min<uint64_t>(kRounds, std::max<uint64_t>(max_slices, 1));
+  p.rounds = R;
+  p.sched.assign((size_t)R + 1 + (size_t)R * (n + 1), 0u);
+  for (uint32_t q = 0; q <= R; ++q) {
+    uint64_t start = 0;
+    for (uint32_t j = 0; j < n; ++j) start += sl[j] * q / R;
+    p.sched[q] = (uint32_t)start;
+  }
+  for (uint32_t q = 0; q < R; ++q) {
+    uint32_t* pre = p.sched.data() + R + 1 + (size_t)q * (n + 1);
+    uint64_t acc = 0;
+    for (uint32_t j = 0; j < n; ++j) {
+      pre[j] = (uint32_t)acc;
+      acc += sl[j] * (q + 1) / R - sl[j] * q / R;
+    }
+    pre[n] = (uint32_t)acc;
+  }
   return OIT_OK;
 }
 
@@ -184,6 +212,9 @@ struct oit_batch {
   BatchFrameDev* d_table = nullptr;
   BatchFrameDev* h_table = nullptr;  // pinned staging
   uint32_t table_cap = 0;
+  uint32_t* d_sched = nullptr;  // interleaved append schedule, uploaded with the table
+  uint32_t* h_sched = nullptr;
+  size_t sched_cap = 0;
   hipEvent_t table_ev = nullptr;
   bool table_ev_pending = false;
   std::vector<BatchFrameDev> uploaded;
@@ -212,6 +243,8 @@ void release(oit_batch* b) {
   (void)hipFree(b->counters);
   (void)hipFree(b->d_table);
   if (b->h_table) (void)hipHostFree(b->h_table);
+  (void)hipFree(b->d_sched);
+  if (b->h_sched) (void)hipHostFree(b->h_sched);
   if (b->table_ev) (void)hipEventDestroy(b->table_ev);
   if (b->fork_ev) (void)hipEventDestroy(b->fork_ev);
   for (int i = 0; i < kSide; ++i) {
@@ -306,9 +339,29 @@ int upload_table(oit_batch* b, const Plan& p, hipStream_t s) {
     if ((rc = st(hipEventSynchronize(b->table_ev)))) return rc;
     b->table_ev_pending = false;
   }
+  if (p.sched.size() > b->sched_cap) {
+    if ((rc = st(hipDeviceSynchronize()))) return rc;
+    if (b->gexec) {  // captured launches hold the old pointer
+      (void)hipGraphExecDestroy(b->gexec);
+      b->gexec = nullptr;
+    }
+    (void)hipFree(b->d_sched);
+    if (b->h_sched) (void)hipHostFree(b->h_sched);
+    b->d_sched = nullptr;
+    b->h_sched = nullptr;
+    b->sched_cap = 0;
+    const size_t cap = std::max<size_t>(p.sched.size(), 4096);
+    if ((rc = st(hipMalloc((void**)&b->d_sched, cap * sizeof(uint32_t))))) return rc;
+    if ((rc = st(hipHostMalloc((void**)&b->h_sched, cap * sizeof(uint32_t), hipHostMallocDefault)))) return rc;
+    b->sched_cap = cap;
+  }
   std::memcpy(b->h_table, p.table.data(), p.table.size() * sizeof(BatchFrameDev));
+  std::memcpy(b->h_sched, p.sched.data(), p.sched.size() * sizeof(uint32_t));
   if ((rc = st(hipMemcpyAsync(b->d_table, b->h_table, p.table.size() * sizeof(BatchFrameDev),
                               hipMemcpyHostToDevice, s))))
+    return rc;
+  if ((rc = st(hipMemcpyAsync(b->d_sched, b->h_sched, p.sched.size() * sizeof(uint32_t), hipMemcpyHostToDevice,
+                              s))))
     return rc;
   if ((rc = st(hipEventRecord(b->table_ev, s)))) return rc;
   b->table_ev_pending = true;
@@ -332,6 +385,10 @@ BatchArgs args_for(const oit_batch* b, const Plan& p) {
 int launch_append(const oit_batch* b, const Plan& p, uint32_t chunks, hipStream_t s) {
   BatchArgs a = args_for(b, p);
   a.units = (uint32_t)(p.pool_slots / (64u * chunks));
+  if (!(b->flags & OIT_BATCH_FRAME_ORDER) && !p.table.empty()) {
+    a.sched = b->d_sched;
+    a.rounds = p.rounds;
+  }
   return st(oit::launch_batch_append(a, pool_view(b), chunks, s));
 }
 
@@ -422,7 +479,7 @@ extern "C" {
 int oit_batch_create(oit_batch** out, int device, uint32_t flags) {
   if (!out) return OIT_ERR_INVALID_ARG;
   *out = nullptr;
-  if (flags & ~(uint32_t)(OIT_BATCH_CONCURRENT | OIT_BATCH_GRAPH)) return OIT_ERR_INVALID_ARG;
+  if (flags & ~(uint32_t)(OIT_BATCH_CONCURRENT | OIT_BATCH_GRAPH | OIT_BATCH_FRAME_ORDER)) return OIT_ERR_INVALID_ARG;
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || device < 0 || device >= n) {
     (void)hipGetLastError();

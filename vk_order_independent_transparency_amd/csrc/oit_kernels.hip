@@ -798,8 +798,34 @@ __global__ __launch_bounds__(kBlock) void k_batch_append(BatchArgs b, LLBuf ll) 
   stage_thresholds(s_thr, b.tab);
   const uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
   if (unit >= b.units) return;
-  const uint64_t g = (uint64_t)unit * (64u * kU);
-  const uint32_t f = find_frame<false>(b.frames, b.f0, b.f1, g);
+  uint64_t g;
+  uint32_t f;
+  if (b.sched) {  // interleaved: every frame advances in id order, a share of the waves in flight each
+    constexpr uint32_t per = kBatchSlice / (64u * kU);  // waves per slice
+    const uint32_t slice = unit / per;
+    uint32_t lo = 0, hi = b.rounds;  // last round whose first slice is <= slice
+    while (hi - lo > 1u) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (b.sched[mid] <= slice) lo = mid;
+      else hi = mid;
+    }
+    const uint32_t q = lo;
+    const uint32_t o = slice - b.sched[q];
+    const uint32_t* pre = b.sched + b.rounds + 1u + (uint64_t)q * (b.f1 + 1u);
+    uint32_t a0 = 0, a1 = b.f1;  // last frame whose offset in the round is <= o
+    while (a1 - a0 > 1u) {
+      const uint32_t mid = (a0 + a1) >> 1;
+      if (pre[mid] <= o) a0 = mid;
+      else a1 = mid;
+    }
+    f = a0;
+    const uint64_t slices = ((uint64_t)b.frames[f].count + kBatchSlice - 1u) / kBatchSlice;
+    const uint64_t c = slices * q / b.rounds + (o - pre[f]);
+    g = (uint64_t)b.frames[f].frag_base + c * kBatchSlice + (uint64_t)(unit % per) * (64u * kU);
+  } else {
+    g = (uint64_t)unit * (64u * kU);
+    f = find_frame<false>(b.frames, b.f0, b.f1, g);
+  }
   const BatchFrameDev fr = b.frames[f];
   AppendSrc a;
   a.target = fr.target;

@@ -96,7 +96,15 @@ struct BatchArgs {
   uint32_t units;               // append: waves of 64 * chunks fragments from frames[f0].frag_base
   uint32_t tile0, tiles;        // resolve: tiles of 256 targets
   const CodecTables* tab;
+  // append schedule (null: frame after frame): the frames' 512-fragment
+  // slices in `rounds` rounds; round q holds slices floor(q S_f / rounds) ..
+  // floor((q + 1) S_f / rounds) - 1 of every frame f (S_f its slices), frame
+  // after frame. sched[0 .. rounds] are the rounds' first slices, then per
+  // round the (f1 + 1) prefix offsets of the frames within it
+  const uint32_t* sched = nullptr;
+  uint32_t rounds = 0;
 };
+constexpr uint32_t kBatchSlice = 512;  // fragments of a schedule slice (the frames' alignment)
 
 // sRGB8 codec tables built on the host (exact curve, bisection in double).
 void build_codec_tables(CodecTables* t);
