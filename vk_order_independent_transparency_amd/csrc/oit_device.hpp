@@ -238,6 +238,19 @@ __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32
   }
 }
 
+// Fragment-stream and record reads: every input word is read once per pass,
+// so these loads carry the non-temporal hint and leave the L2 and the
+// last-level cache to what the next kernel reads (at 1920x1080 the Linked
+// List's nodes stay cached for the composite).
+__device__ __forceinline__ uint32_t ld_stream(const uint32_t* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ uint64_t ld_stream(const uint64_t* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float ld_stream(const float* p) { return __builtin_nontemporal_load(p); }
+__device__ __forceinline__ float4 ld_stream(const float4* p) {
+  const float* q = reinterpret_cast<const float*>(p);
+  return make_float4(__builtin_nontemporal_load(q), __builtin_nontemporal_load(q + 1), __builtin_nontemporal_load(q + 2),
+                     __builtin_nontemporal_load(q + 3));
+}
+
 // Per-target lock for critical sections whose shared data is read and
 // written only with agent-scope relaxed atomics (coherent across the XCDs'
 // L2 caches, so the section needs ordering but no L2 writeback or

@@ -353,17 +353,17 @@ __global__ __launch_bounds__(kBlock) void k_ll_store(Common c, FragView f, LLBuf
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t t = kEmpty;
   if (valid && c.opq_z) {  // culled by the opaque pass: no node, no tail
-    t = f.target[i];
-    valid = t < c.T && opaque_visible(c.opq_z, c.T, 1u, t, depth_key(f.depth[i]), 1u) != 0u;
+    t = ld_stream(&f.target[i]);
+    valid = t < c.T && opaque_visible(c.opq_z, c.T, 1u, t, depth_key(ld_stream(&f.depth[i])), 1u) != 0u;
     t = kEmpty;
   }
   if (valid) {
-    col = f.rgba[i];
+    col = ld_stream(&f.rgba[i]);
     if (n >= ll.cap) {
-      t = f.target[i];
+      t = ld_stream(&f.target[i]);
       ovf = t < c.T;
     } else {
-      const uint32_t key = depth_key(f.depth[i]);
+      const uint32_t key = depth_key(ld_stream(&f.depth[i]));
       ll.cd[n] = ((uint64_t)key << 32) | encode_color_lds(col, s_thr);
       ll.next[n] = kUnset;
     }
@@ -381,8 +381,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_link(Common c, FragView f, LLBuf 
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   const uint64_t n64 = (uint64_t)f.id_base + i + 1u;
   bool valid = i < f.count && n64 < ll.cap;
-  uint32_t t = valid ? f.target[i] : kEmpty;
-  if (t >= c.T || (c.opq_z && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(f.depth[i]), 1u))) {
+  uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
+  if (t >= c.T || (c.opq_z && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(ld_stream(&f.depth[i])), 1u))) {
     valid = false;
     t = kEmpty;
   }
@@ -465,15 +465,15 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     const uint64_t i = f.first + (uint64_t)u * 64u + lane;
-    t[u] = i < f.count ? f.target[i] : kEmpty;
+    t[u] = i < f.count ? ld_stream(&f.target[i]) : kEmpty;
     if (t[u] >= f.T) t[u] = kEmpty;
     m[u] = 0xFFFFFFFFu;
     if (f.mask && t[u] != kEmpty) {
-      m[u] = f.mask[i] & f.full;
+      m[u] = ld_stream(&f.mask[i]) & f.full;
       if (m[u] == 0u) t[u] = kEmpty;  // no sample covered: the shader does not run
     }
     if (f.opq && t[u] != kEmpty) {  // behind the opaque depth in every covered sample: culled
-      m[u] = opaque_visible(f.opq, f.T, f.cov, t[u], depth_key(f.depth[i]), m[u]);
+      m[u] = opaque_visible(f.opq, f.T, f.cov, t[u], depth_key(ld_stream(&f.depth[i])), m[u]);
       if (m[u] == 0u) t[u] = kEmpty;
     }
     const uint64_t n64 = (uint64_t)f.node0 + i;
@@ -481,8 +481,8 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
     ovf[u] = t[u] != kEmpty && !node[u];
     col[u] = make_float4(0.f, 0.f, 0.f, 0.f);
     dep[u] = 0.f;
-    if (t[u] != kEmpty) col[u] = f.rgba[i];
-    if (node[u]) dep[u] = f.depth[i];
+    if (t[u] != kEmpty) col[u] = ld_stream(&f.rgba[i]);
+    if (node[u]) dep[u] = ld_stream(&f.depth[i]);
   }
   uint32_t lo[kU], hi[kU];
   bool ins[kU];
@@ -877,21 +877,21 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
-  uint32_t t = valid ? f.target[i] : kEmpty;
+  uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
   if (t >= c.T) {
     valid = false;
     t = kEmpty;
   }
   uint32_t m = 0xFFFFFFFFu;
   if (valid && f.mask) {
-    m = f.mask[i] & ((1u << c.cov) - 1u);
+    m = ld_stream(&f.mask[i]) & ((1u << c.cov) - 1u);
     if (m == 0u) {  // no sample covered: the shader does not run
       valid = false;
       t = kEmpty;
     }
   }
   if (valid && c.opq_z) {  // early depth test against the opaque pass
-    m = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(f.depth[i]), m);
+    m = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(ld_stream(&f.depth[i])), m);
     if (m == 0u) {
       valid = false;
       t = kEmpty;
@@ -906,11 +906,11 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
   bool tail = false;
   if (valid) {
-    col = f.rgba[i];
+    col = ld_stream(&f.rgba[i]);
     if (slot < c.K) {
       const uint64_t idx = (uint64_t)slot * c.T + t;
       kb.col[idx] = encode_color_lds(col, s_thr);
-      kb.key[idx] = depth_key(f.depth[i]);
+      kb.key[idx] = depth_key(ld_stream(&f.depth[i]));
       kb.id[idx] = lid;
       if (kb.mask) kb.mask[idx] = m;  // storeMask, oitSimple.frag.glsl:70
     } else {
@@ -986,15 +986,15 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
-  uint32_t t = valid ? f.target[i] : kEmpty;
+  uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
   if (t >= c.T) valid = false;
   uint32_t my_mask = 0xFFFFFFFFu;
   if (valid && f.mask) {
-    my_mask = f.mask[i] & ((1u << c.cov) - 1u);
+    my_mask = ld_stream(&f.mask[i]) & ((1u << c.cov) - 1u);
     if (my_mask == 0u) valid = false;  // no sample covered: the shader does not run
   }
   if (valid && c.opq_z) {  // early depth test against the opaque pass
-    my_mask = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(f.depth[i]), my_mask);
+    my_mask = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(ld_stream(&f.depth[i])), my_mask);
     if (my_mask == 0u) valid = false;
   }
   const uint32_t lid = f.id_base + i;
@@ -1005,8 +1005,8 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   uint32_t tail_mask = my_mask;
   uint32_t my_key = 0u, my_col = 0u;
   if (valid) {
-    my_key = depth_key(f.depth[i]);
-    my_col = encode_color_lds(f.rgba[i], s_thr);
+    my_key = depth_key(ld_stream(&f.depth[i]));
+    my_col = encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
     tail_col = my_col;
     pending = my_key <= ld_relaxed(&kb.depth[t]);  // :66-69 early depth test
     if (!pending) tail = true;
@@ -1134,11 +1134,11 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, uint32_t* __restrict__ zs) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t t = i < f.count ? f.target[i] : kEmpty;
+  uint32_t t = i < f.count ? ld_stream(&f.target[i]) : kEmpty;
   if (t >= c.T) t = kEmpty;
   uint32_t z = 0u;
   if (t != kEmpty) {
-    z = depth_key(f.depth[i]);
+    z = depth_key(ld_stream(&f.depth[i]));
     if (!opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) t = kEmpty;
   }
   const Run r = same_target_run(t);
@@ -1190,20 +1190,20 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color(Common c, FragView f, c
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
-  uint32_t t = valid ? f.target[i] : kEmpty;
+  uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
   if (t >= c.T) {
     valid = false;
     t = kEmpty;
   }
   bool tail = false;
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t z = valid ? depth_key(f.depth[i]) : 0u;
+  const uint32_t z = valid ? depth_key(ld_stream(&f.depth[i])) : 0u;
   if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) {
     valid = false;
     t = kEmpty;
   }
   if (valid) {
-    col = f.rgba[i];
+    col = ld_stream(&f.rgba[i]);
     if (zs[(uint64_t)(K - 1) * c.T + t] < z) {
       tail = true;
     } else {
@@ -1438,12 +1438,12 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, u
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t t = i < f.count ? f.target[i] : kEmpty;
+  uint32_t t = i < f.count ? ld_stream(&f.target[i]) : kEmpty;
   if (t >= c.T) t = kEmpty;
   uint64_t z = kEmpty64;
   if (t != kEmpty) {
-    const uint32_t key = depth_key(f.depth[i]);
-    if (opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) z = ((uint64_t)key << 32) | encode_color_lds(f.rgba[i], s_thr);
+    const uint32_t key = depth_key(ld_stream(&f.depth[i]));
+    if (opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) z = ((uint64_t)key << 32) | encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
     else t = kEmpty;
   }
   const Run r = same_target_run(t);
@@ -1491,15 +1491,15 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
-  uint32_t t = valid ? f.target[i] : kEmpty;
+  uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
   if (t >= c.T) valid = false;
   bool tail = false;
   uint32_t flag = 0;
   float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(f.depth[i]), 1u)) valid = false;
+  if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(ld_stream(&f.depth[i])), 1u)) valid = false;
   if (valid) {
-    const uint32_t col = encode_color_lds(f.rgba[i], s_thr);
-    const uint64_t z = ((uint64_t)depth_key(f.depth[i]) << 32) | col;
+    const uint32_t col = encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
+    const uint64_t z = ((uint64_t)depth_key(ld_stream(&f.depth[i])) << 32) | col;
     const uint64_t s = slots[(uint64_t)(K - 1) * c.T + t];
     if (z >= s) {
       tail = true;
@@ -1567,13 +1567,13 @@ __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WB
   float4 wc = make_float4(0.f, 0.f, 0.f, 0.f);
   float lr = 0.f;
   if (i < f.count) {
-    t = f.target[i];
+    t = ld_stream(&f.target[i]);
     if (t >= c.T) t = kEmpty;
   }
-  if (t != kEmpty && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(f.depth[i]), 1u)) t = kEmpty;
+  if (t != kEmpty && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(ld_stream(&f.depth[i])), 1u)) t = kEmpty;
   if (t != kEmpty) {
-    const float4 col = f.rgba[i];
-    const float view = vnear + f.depth[i] * vspan;   // stands in for -viewZ (object.vert.glsl:35)
+    const float4 col = ld_stream(&f.rgba[i]);
+    const float view = vnear + ld_stream(&f.depth[i]) * vspan;   // stands in for -viewZ (object.vert.glsl:35)
     const float depthZ = view * 10.0f;               // :65
     const float x = depthZ / 200.0f;
     const float x2 = x * x;
@@ -1654,7 +1654,7 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t first
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
       const uint64_t lid = base + (uint64_t)u * 64u + lane;
-      rec[u] = lid < n ? tb.rec[lid] : kEmpty64;
+      rec[u] = lid < n ? ld_stream(&tb.rec[lid]) : kEmpty64;
     }
     uint32_t t[kU], v[kU], lo[kU];
     bool ins[kU];
@@ -1727,13 +1727,13 @@ template <bool kCov>
 __global__ __launch_bounds__(kBlock) void k_opaque_depth(Common c, FragView f, uint64_t* __restrict__ z) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= f.count) return;
-  const uint32_t t = f.target[i];
+  const uint32_t t = ld_stream(&f.target[i]);
   if (t >= c.T) return;
-  const uint32_t key = depth_key(f.depth[i]);
+  const uint32_t key = depth_key(ld_stream(&f.depth[i]));
   if (key >= depth_key(1.0f)) return;
   const uint64_t v = ((uint64_t)key << 32) | (f.id_base + i);
   if constexpr (kCov) {
-    const uint32_t m = f.mask[i] & ((1u << c.cov) - 1u);
+    const uint32_t m = ld_stream(&f.mask[i]) & ((1u << c.cov) - 1u);
     for (uint32_t s = 0; s < c.cov; ++s)
       if ((m >> s) & 1u)
         (void)__hip_atomic_fetch_min(&z[(uint64_t)s * c.T + t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1750,19 +1750,19 @@ __global__ __launch_bounds__(kBlock) void k_opaque_color(Common c, FragView f, c
                                                          float4* __restrict__ rgba) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= f.count) return;
-  const uint32_t t = f.target[i];
+  const uint32_t t = ld_stream(&f.target[i]);
   if (t >= c.T) return;
-  const uint32_t key = depth_key(f.depth[i]);
+  const uint32_t key = depth_key(ld_stream(&f.depth[i]));
   if (key >= depth_key(1.0f)) return;
   const uint64_t v = ((uint64_t)key << 32) | (f.id_base + i);
   if constexpr (kCov) {
-    const uint32_t m = f.mask[i] & ((1u << c.cov) - 1u);
+    const uint32_t m = ld_stream(&f.mask[i]) & ((1u << c.cov) - 1u);
     for (uint32_t s = 0; s < c.cov; ++s) {
       const uint64_t j = (uint64_t)s * c.T + t;
-      if (((m >> s) & 1u) && z[j] == v) rgba[j] = f.rgba[i];
+      if (((m >> s) & 1u) && z[j] == v) rgba[j] = ld_stream(&f.rgba[i]);
     }
   } else {
-    if (z[t] == v) rgba[t] = f.rgba[i];
+    if (z[t] == v) rgba[t] = ld_stream(&f.rgba[i]);
   }
 }
 
