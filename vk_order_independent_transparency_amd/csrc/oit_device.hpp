@@ -267,6 +267,19 @@ __device__ __forceinline__ void lock_release(uint32_t* l) {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
   __hip_atomic_store(l, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+// The same lock with a counter in the lock word (count << 1 | held): the
+// acquiring atomicOr returns the count, the release stores the new one.
+__device__ __forceinline__ bool lock_try_count(uint32_t* l, uint32_t& count) {
+  const uint32_t old = __hip_atomic_fetch_or(l, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  count = old >> 1;
+  return (old & 1u) == 0u;
+}
+__device__ __forceinline__ void lock_release_count(uint32_t* l, uint32_t count) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unconstrained
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_store(l, count << 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 
 // list_insert_desc for a lane that already tried the head and learned that a
 // larger element is there: it starts at that element's link word `p0` and

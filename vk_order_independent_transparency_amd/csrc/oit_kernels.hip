@@ -931,10 +931,10 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
 // the tail output: the evicted element, or the fragment itself.
 template <int K>
 __device__ __forceinline__ void spin_section(const Common& c, const KBuf& kb, uint32_t t, uint32_t key, uint32_t col,
-                                             uint32_t id, uint32_t mask, bool& tail, uint32_t& tail_id,
-                                             uint32_t& tail_col, uint32_t& tail_mask) {
-  const uint32_t cnt = ld_relaxed(&kb.count[t]);
-  st_relaxed(&kb.count[t], cnt + 1u);
+                                             uint32_t id, uint32_t mask, uint32_t& count, bool& tail,
+                                             uint32_t& tail_id, uint32_t& tail_col, uint32_t& tail_mask) {
+  const uint32_t cnt = count++;  // :86-87; the count travels in the lock word
+  st_relaxed(&kb.count[t], count);
   if (cnt < (uint32_t)K) {
     const uint64_t idx = (uint64_t)cnt * c.T + t;
     st_relaxed(&kb.col[idx], col);
@@ -1026,11 +1026,13 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   // holder of the same wave would wait behind.
   while (__ballot(pending)) {
     if (pending) {
-      if (lock_try(&kb.lock[t])) {
-        spin_section<K>(c, kb, t, my_key, my_col, lid, my_mask, tail, tail_id, tail_col, tail_mask);
+      uint32_t count;
+      if (lock_try_count(&kb.lock[t], count)) {
+        spin_section<K>(c, kb, t, my_key, my_col, lid, my_mask, count, tail, tail_id, tail_col, tail_mask);
         if (lead)
-          spin_section<K>(c, kb, t, p_key, p_col, lid + 1u, p_mask, p_tail, p_tail_id, p_tail_col, p_tail_mask);
-        lock_release(&kb.lock[t]);
+          spin_section<K>(c, kb, t, p_key, p_col, lid + 1u, p_mask, count, p_tail, p_tail_id, p_tail_col,
+                          p_tail_mask);
+        lock_release_count(&kb.lock[t], count);
         pending = false;
       }
     }
@@ -1151,9 +1153,9 @@ __global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, u
     a = z < zn ? z : zn;
     b = z < zn ? zn : z;
   }
-  if (a > zs[(uint64_t)(K - 1) * c.T + t]) return;
-  int j = 0;
-  if (a > zs[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
+  const uint32_t z_last = zs[(uint64_t)(K - 1) * c.T + t], z_mid = zs[(uint64_t)(K / 2) * c.T + t];
+  if (a > z_last) return;
+  int j = a > z_mid ? K / 2 : 0;
   for (; j < K; ++j) {
     const uint32_t old =
         __hip_atomic_fetch_min(&zs[(uint64_t)j * c.T + t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1458,9 +1460,9 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, u
     a = z < zn ? z : zn;
     b = z < zn ? zn : z;
   }
-  if (a > slots[(uint64_t)(K - 1) * c.T + t]) return;
-  int j = 0;
-  if (a > slots[(uint64_t)(K / 2) * c.T + t]) j = K / 2;
+  const uint64_t s_last = slots[(uint64_t)(K - 1) * c.T + t], s_mid = slots[(uint64_t)(K / 2) * c.T + t];
+  if (a > s_last) return;
+  int j = a > s_mid ? K / 2 : 0;
   for (; j < K; ++j) {
     const uint64_t old =
         __hip_atomic_fetch_min(&slots[(uint64_t)j * c.T + t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
