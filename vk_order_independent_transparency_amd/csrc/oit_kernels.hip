@@ -1206,15 +1206,15 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color(Common c, FragView f, c
   }
   if (valid) {
     col = ld_stream(&f.rgba[i]);
-    if (zs[(uint64_t)(K - 1) * c.T + t] < z) {
+    // the slot of z: the number of stored depths below it (slots are sorted;
+    // all K loaded at once instead of a binary search of dependent loads)
+    uint32_t below = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k) below += zs[(uint64_t)k * c.T + t] < z ? 1u : 0u;
+    if (below == (uint32_t)K) {
       tail = true;
     } else {
-      int start = 0, end = K - 1;
-      while (start < end) {
-        const int mid = (start + end) / 2;
-        if (zs[(uint64_t)mid * c.T + t] < z) start = mid + 1;
-        else end = mid;
-      }
+      const int start = (int)below;
       const uint64_t v = ((uint64_t)(f.id_base + i + 1u) << 32) | encode_color_lds(col, s_thr);
       (void)__hip_atomic_fetch_max(&cs[(uint64_t)start * c.T + t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
