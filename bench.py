@@ -77,7 +77,7 @@ def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream):
         with Renderer(0) as r:
             r.configure(FrameConfig(algorithm=alg, layers=K, tail_blend=alg != 6, width=p.width, height=p.height,
                                     ll_pool_nodes=F + F // 2 if alg == 1 else 0, max_fragments=F))
-            for _ in range(2):
+            for _ in range(5):
                 r.render(tg_d, d_d, c_d, out, stream)
             e0, e1 = Event(), Event()
             stream.synchronize()
@@ -357,7 +357,7 @@ def main():
 
     st = r.stats(stream)
     r.close()
-    algorithms = None if args.no_algorithms else bench_algorithms(tg_d, d_d, c_d, F, p, 10, stream)
+    algorithms = None if args.no_algorithms else bench_algorithms(tg_d, d_d, c_d, F, p, 50, stream)
 
     batch = None if args.no_batch else bench_batch(1, args.batch_frames)
     served = served_burst = None
