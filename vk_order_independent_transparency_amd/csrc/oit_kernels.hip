@@ -1153,9 +1153,19 @@ __global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, u
     a = z < zn ? z : zn;
     b = z < zn ? zn : z;
   }
+  constexpr int kProbe = K / 2 < 8 ? K / 2 : 8;  // as in k_loop64_store
+  uint32_t probe[kProbe > 0 ? kProbe : 1];
+#pragma unroll
+  for (int k = 0; k < kProbe; ++k) probe[k] = zs[(uint64_t)k * c.T + t];
   const uint32_t z_last = zs[(uint64_t)(K - 1) * c.T + t], z_mid = zs[(uint64_t)(K / 2) * c.T + t];
   if (a > z_last) return;
-  int j = a > z_mid ? K / 2 : 0;
+  int j = 0;
+  if (a > z_mid) {
+    j = K / 2;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kProbe; ++k) j += probe[k] < a ? 1 : 0;
+  }
   for (; j < K; ++j) {
     const uint32_t old =
         __hip_atomic_fetch_min(&zs[(uint64_t)j * c.T + t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -1460,9 +1470,23 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, u
     a = z < zn ? z : zn;
     b = z < zn ? zn : z;
   }
+  // the pretests, and the first slots loaded with them: slots are sorted and
+  // only decrease, so every loaded slot below a is still below it and the
+  // chain can start past them (a slot that has dropped below a since only
+  // costs a no-op atomic)
+  constexpr int kProbe = K / 2 < 8 ? K / 2 : 8;
+  uint64_t probe[kProbe > 0 ? kProbe : 1];
+#pragma unroll
+  for (int k = 0; k < kProbe; ++k) probe[k] = slots[(uint64_t)k * c.T + t];
   const uint64_t s_last = slots[(uint64_t)(K - 1) * c.T + t], s_mid = slots[(uint64_t)(K / 2) * c.T + t];
   if (a > s_last) return;
-  int j = a > s_mid ? K / 2 : 0;
+  int j = 0;
+  if (a > s_mid) {
+    j = K / 2;
+  } else {
+#pragma unroll
+    for (int k = 0; k < kProbe; ++k) j += probe[k] < a ? 1 : 0;
+  }
   for (; j < K; ++j) {
     const uint64_t old =
         __hip_atomic_fetch_min(&slots[(uint64_t)j * c.T + t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
