@@ -156,18 +156,19 @@ def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream):
     return rows
 
 
-def bench_batch(seed, frames_n):
+def bench_batch(seed, frames_n, resolutions=(256, 512, 1024, 2048)):
     """The 64-frame batched workload: every frame one at a time (clear, append,
     resolve per frame, one context each) against the batched Linked List
     (one append launch, one resolve launch per layer group on forked streams,
     replayed from a graph, variants tuned). Device time from HIP events; the
-    batched images are checked bit for bit against the one-at-a-time ones."""
+    batched images are checked bit for bit against the one-at-a-time ones.
+    `resolutions`: the square frame sizes drawn from."""
     import numpy as np
 
     from vk_order_independent_transparency_amd import (Batch, BatchFrame, DeviceBuffer, Event, FrameConfig,
                                                        OIT_LINKEDLIST, Renderer, Stream, scene)
 
-    work = scene.batch_workload(seed, frames_n)
+    work = scene.batch_workload(seed, frames_n, resolutions)
     st = Stream()
     frames, total = [], 0
     for fr in work:
@@ -233,7 +234,8 @@ def bench_batch(seed, frames_n):
     exact = exact and order_ok
     log(f"batch frame after frame: {order_ms:.3f} ms, interleaved: {bat_ms:.3f} ms")
     log(f"batch: sequential {seq_ms:.3f} ms, batched {bat_ms:.3f} ms, {seq_ms / bat_ms:.2f}x, bit-exact {exact}")
-    return {"workload": f"batch_{len(frames)}_frames_seed{seed}", "frames": len(frames), "fragments": total,
+    sizes = "" if tuple(resolutions) == (256, 512, 1024, 2048) else "_" + "_".join(str(r) for r in resolutions)
+    return {"workload": f"batch_{len(frames)}_frames_seed{seed}{sizes}", "frames": len(frames), "fragments": total,
             "sequential_ms": round(seq_ms, 4), "batched_ms": round(bat_ms, 4),
             "speedup": round(seq_ms / bat_ms, 3), "G_fragments_per_s": round(total / bat_ms / 1e6, 3),
             "frame_order_ms": round(order_ms, 4),
@@ -410,6 +412,9 @@ def main():
     algorithms = None if args.no_algorithms else bench_algorithms(tg_d, d_d, c_d, F, p, 50, stream)
 
     batch = None if args.no_batch else bench_batch(1, args.batch_frames)
+    # the same generator with only 256x256 and 512x512 frames: per-frame launch
+    # and clear costs weigh more than the fragment work
+    batch_small = None if args.no_batch else bench_batch(1, args.batch_frames, (256, 512))
     served = served_burst = None
     if not args.no_serve:
         served, served_burst = bench_serve(1, args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
@@ -465,6 +470,7 @@ def main():
         "latency_ms": _under_load(served),
         "algorithms": algorithms,
         "batch": batch,
+        "batch_small_frames": batch_small,
         "serve": served,
         "serve_vs_one_at_a_time": served_burst,
         "stages": stages,

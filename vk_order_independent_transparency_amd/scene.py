@@ -168,17 +168,18 @@ class BatchFrame:
     layers: int
 
 
-def batch_workload(seed: int = 1, frames: int = 64) -> list[BatchFrame]:
+def batch_workload(seed: int = 1, frames: int = 64, resolutions=(256, 512, 1024, 2048)) -> list[BatchFrame]:
     """Batched workload: `frames` frames, square resolution uniform from
-    {256, 512, 1024, 2048}, sphere count log-uniform in [100, 5000], 20 % of
-    the spheres packed into a 0.1-wide square (local depth complexity up to
-    a few hundred layers), K uniform from {4, 8, 16, 32}; radius [0.01, 0.05]
-    and alpha [0.1, 0.5] as in the 1920x1080 workload. Frame i's scene seed
-    is drawn from `seed`, so the list is a pure function of (seed, frames)."""
+    `resolutions` (default {256, 512, 1024, 2048}), sphere count log-uniform
+    in [100, 5000], 20 % of the spheres packed into a 0.1-wide square (local
+    depth complexity up to a few hundred layers), K uniform from
+    {4, 8, 16, 32}; radius [0.01, 0.05] and alpha [0.1, 0.5] as in the
+    1920x1080 workload. Frame i's scene seed is drawn from `seed`, so the
+    list is a pure function of (seed, frames, resolutions)."""
     rng = _SplitMix64(seed ^ 0xB47C4)
     out = []
     for _ in range(frames):
-        res = rng.choice((256, 512, 1024, 2048))
+        res = rng.choice(tuple(resolutions))
         spheres = rng.log_uniform_int(100, 5000)
         K = rng.choice((4, 8, 16, 32))
         fseed = rng.next()
