@@ -780,8 +780,7 @@ int oit_clear(oit_ctx* c, void* stream) {
       case OIT_SIMPLE:  // count to 0, oitRender.cpp:161-169
         if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_count, T, 0u, s);
         break;
-      case OIT_SPINLOCK:  // count, spin to 0, depth to ~0, oitRender.cpp:343-354
-        if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_count, T, 0u, s);
+      case OIT_SPINLOCK:  // count, spin to 0, depth to ~0, oitRender.cpp:343-354 (the count is in the lock word)
         if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_lock, T, 0u, s);
         if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_depth, T, kEmpty, s);
         break;
@@ -1016,7 +1015,8 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     }
     case OIT_SIMPLE:
     case OIT_SPINLOCK: {
-      oit::KBuf kb{c->kb_count, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
+      oit::KBuf kb{c->kb_count, alg == OIT_SPINLOCK ? c->kb_lock : nullptr, c->kb_depth, c->kb_col, c->kb_key,
+                   c->kb_id};
       if (coverage(c)) {
         kb.mask = c->kb_mask;
         e = staged(c, kStResolve, s, [&] { return oit::launch_kbuf_resolve_cov(k, kb, tbr, o, s); });

@@ -933,8 +933,7 @@ template <int K>
 __device__ __forceinline__ void spin_section(const Common& c, const KBuf& kb, uint32_t t, uint32_t key, uint32_t col,
                                              uint32_t id, uint32_t mask, uint32_t& count, bool& tail,
                                              uint32_t& tail_id, uint32_t& tail_col, uint32_t& tail_mask) {
-  const uint32_t cnt = count++;  // :86-87; the count travels in the lock word
-  st_relaxed(&kb.count[t], count);
+  const uint32_t cnt = count++;  // :86-87; the count lives in the lock word (count << 1 | held)
   if (cnt < (uint32_t)K) {
     const uint64_t idx = (uint64_t)cnt * c.T + t;
     st_relaxed(&kb.col[idx], col);
@@ -976,10 +975,16 @@ __device__ __forceinline__ void spin_section(const Common& c, const KBuf& kb, ui
 // the lock while its neighbours retry on the next trip. Elements are ordered
 // by (depth key, id); evicted and rejected fragments are recorded for the
 // tail with their packed sRGB8 colour. Two consecutive lanes of one target (a
-// sphere's front and back) that both pass the early depth test run their
-// critical sections back to back under one acquisition by the even lane (the
-// two sections in id order, one of the orders the lock allows), so they do
-// not contend for the lock; the odd lane gets its tail output back by shuffle.
+// sphere's front and back) run their critical sections back to back under
+// one acquisition by the even lane (the two sections in id order, one of the
+// orders the lock allows), so they do not contend for the lock; the odd lane
+// gets its tail output back by shuffle. The early depth test (:66-69) reads
+// the bound in the same round trip as the first lock attempt; a fragment it
+// rejects skips its section (a lock taken for nothing is released with the
+// count unchanged). A fragment the test rejects is one its section would
+// reject too (the bound is the key of an evicted element, and the stored
+// elements only get nearer), so the test changes the count beyond K, as in
+// the reference, and never the stored set or the tail.
 template <int K, bool kTail>
 __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBuf kb, TailBuf tb) {
   __shared__ float s_thr[257];
@@ -999,7 +1004,6 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   }
   const uint32_t lid = f.id_base + i;
   bool tail = false;
-  bool pending = false;
   uint32_t tail_id = lid;
   uint32_t tail_col = 0u;
   uint32_t tail_mask = my_mask;
@@ -1008,32 +1012,44 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
     my_key = depth_key(ld_stream(&f.depth[i]));
     my_col = encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
     tail_col = my_col;
-    pending = my_key <= ld_relaxed(&kb.depth[t]);  // :66-69 early depth test
-    if (!pending) tail = true;
   }
   // pairs: the even lane of a same-target run takes its odd neighbour along
-  const Run r = same_target_run(pending ? t : kEmpty);
-  const bool lead = pending && ((r.lane - r.start_lane) & 1u) == 0u && r.lane < r.end_lane;
-  const bool follower = pending && ((r.lane - r.start_lane) & 1u) == 1u;
+  const Run r = same_target_run(valid ? t : kEmpty);
+  const bool lead = valid && ((r.lane - r.start_lane) & 1u) == 0u && r.lane < r.end_lane;
+  const bool follower = valid && ((r.lane - r.start_lane) & 1u) == 1u;
   const uint32_t p_key = __shfl_down(my_key, 1u), p_col = __shfl_down(my_col, 1u);
   const uint32_t p_mask = __shfl_down(my_mask, 1u);
-  if (follower) pending = false;
+  bool pending = valid && !follower;
   bool p_tail = false;
   uint32_t p_tail_id = lid + 1u, p_tail_col = p_col, p_tail_mask = p_mask;
+  bool tested = false, me_pass = false, p_pass = false;
   // The lock loop (:75-123) is driven by a wave-wide ballot: a lane that takes
   // the lock runs the critical section and releases it in the same trip, and
   // the compiler cannot hoist the exchange into an inner spin that a lock
   // holder of the same wave would wait behind.
   while (__ballot(pending)) {
     if (pending) {
+      uint32_t bound = 0u;
+      if (!tested) bound = ld_relaxed(&kb.depth[t]);  // :66-69, issued with the first attempt
       uint32_t count;
-      if (lock_try_count(&kb.lock[t], count)) {
-        spin_section<K>(c, kb, t, my_key, my_col, lid, my_mask, count, tail, tail_id, tail_col, tail_mask);
-        if (lead)
+      const bool got = lock_try_count(&kb.lock[t], count);
+      if (!tested) {
+        tested = true;
+        me_pass = my_key <= bound;
+        p_pass = lead && p_key <= bound;
+        if (!me_pass) tail = true;           // rejected: its own colour goes to the tail
+        if (lead && !p_pass) p_tail = true;  // the follower rejected likewise
+      }
+      if (got) {
+        if (me_pass)
+          spin_section<K>(c, kb, t, my_key, my_col, lid, my_mask, count, tail, tail_id, tail_col, tail_mask);
+        if (p_pass)
           spin_section<K>(c, kb, t, p_key, p_col, lid + 1u, p_mask, count, p_tail, p_tail_id, p_tail_col,
                           p_tail_mask);
         lock_release_count(&kb.lock[t], count);
         pending = false;
+      } else if (!me_pass && !p_pass) {
+        pending = false;  // both rejected by the early test: no section to run
       }
     }
   }
@@ -1060,7 +1076,7 @@ template <int K, bool kColorTail, bool kDebug>
 __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, TailBuf tb, ResolveOut o) {
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
-  const uint32_t count = kb.count[t];
+  const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
   const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
   FrontK<K> a;
   a.init();
@@ -1092,7 +1108,7 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
   __syncthreads();
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
-  const uint32_t count = kb.count[t];
+  const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
   const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
   FrontK<K> a;  // (key << 32 | id, slot)
   a.init();
@@ -1522,15 +1538,21 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
   bool tail = false;
   uint32_t flag = 0;
   float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
-  if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(ld_stream(&f.depth[i])), 1u)) valid = false;
+  const uint32_t key = valid ? depth_key(ld_stream(&f.depth[i])) : 0u;
+  if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) valid = false;
   if (valid) {
-    const uint32_t col = encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
-    const uint64_t z = ((uint64_t)depth_key(ld_stream(&f.depth[i])) << 32) | col;
+    // the key's high half is the depth: a fragment nearer than the K-th
+    // slot's depth is kept without its colour being read (every fragment
+    // of a pixel with fewer than K, where that slot is empty)
     const uint64_t s = slots[(uint64_t)(K - 1) * c.T + t];
-    if (z >= s) {
-      tail = true;
-      flag = z == s ? 1u : 0u;
-      payload = decode_premult(col, c.tab);
+    if (key >= (uint32_t)(s >> 32)) {
+      const uint32_t col = encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
+      const uint64_t z = ((uint64_t)key << 32) | col;
+      if (z >= s) {
+        tail = true;
+        flag = z == s ? 1u : 0u;
+        payload = decode_premult(col, c.tab);
+      }
     }
   }
   tail_record_own(tail, t, f.id_base + i, flag, payload, tb);

@@ -42,8 +42,8 @@ struct LLBuf {
 };
 
 struct KBuf {          // Simple / Spinlock
-  uint32_t* count;    // [T]
-  uint32_t* lock;     // [T] Spinlock only: count << 1 | held (0 = empty, free)
+  uint32_t* count;    // [T] Simple's count (the Spinlock keeps its count in the lock word)
+  uint32_t* lock;     // [T] Spinlock only: count << 1 | held (0 = empty, free); null for Simple
   uint32_t* depth;    // [T] Spinlock early-depth bound
   uint32_t* col;      // [K][T]
   uint32_t* key;      // [K][T]
