@@ -1232,11 +1232,18 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color(Common c, FragView f, c
   }
   if (valid) {
     col = ld_stream(&f.rgba[i]);
-    // the slot of z: the number of stored depths below it (slots are sorted;
-    // all K loaded at once instead of a binary search of dependent loads)
+    // the slot of z: the number of stored depths below it. Slots are sorted
+    // with the empty ones last, so the lower half loaded at once decides
+    // unless all of it lies below z (a pixel holding more than K/2 depths):
+    // only then is the upper half loaded
     uint32_t below = 0;
+    constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
 #pragma unroll
-    for (int k = 0; k < K; ++k) below += zs[(uint64_t)k * c.T + t] < z ? 1u : 0u;
+    for (int k = 0; k < kHalf; ++k) below += zs[(uint64_t)k * c.T + t] < z ? 1u : 0u;
+    if (below == (uint32_t)kHalf) {
+#pragma unroll
+      for (int k = kHalf; k < K; ++k) below += zs[(uint64_t)k * c.T + t] < z ? 1u : 0u;
+    }
     if (below == (uint32_t)K) {
       tail = true;
     } else {
@@ -1259,9 +1266,16 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
   uint32_t z[K], col[K];
   uint32_t n = 0;
   bool alive = true;
+  // depths are sorted with the empty slots last: the upper half is loaded
+  // only where the lower half is full
+  constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
+#pragma unroll
+  for (int k = 0; k < kHalf; ++k) z[k] = zs[(uint64_t)k * c.T + t];
+  const bool upper = z[kHalf - 1] != kEmpty;
+#pragma unroll
+  for (int k = kHalf; k < K; ++k) z[k] = upper ? zs[(uint64_t)k * c.T + t] : kEmpty;
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    z[k] = zs[(uint64_t)k * c.T + t];
     alive = alive && z[k] != kEmpty;
     col[k] = alive ? (uint32_t)cs[(uint64_t)k * c.T + t] : 0u;
     n += alive ? 1u : 0u;
@@ -1565,9 +1579,15 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint6
                                                            ResolveOut o) {
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
+  // slots are sorted with the empty ones last: the upper half is loaded only
+  // where the lower half is full
+  constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
   uint64_t s[K];
 #pragma unroll
-  for (int k = 0; k < K; ++k) s[k] = slots[(uint64_t)k * c.T + t];
+  for (int k = 0; k < kHalf; ++k) s[k] = slots[(uint64_t)k * c.T + t];
+  const bool upper = (uint32_t)(s[kHalf - 1] >> 32) != kEmpty;
+#pragma unroll
+  for (int k = kHalf; k < K; ++k) s[k] = upper ? slots[(uint64_t)k * c.T + t] : kEmpty64;
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t n = 0;
   bool alive = true;
