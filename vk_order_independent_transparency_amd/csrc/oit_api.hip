@@ -618,8 +618,9 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
       break;
     }
     case OIT_SIMPLE:
-    case OIT_SPINLOCK: {  // oit.cpp:102-118
-      rc = reserve(c, &c->kb_count, &c->cap_kb_count, T);
+    case OIT_SPINLOCK: {  // oit.cpp:102-118 (the Spinlock's count lives in its lock word)
+      if (alg == OIT_SIMPLE) rc = reserve(c, &c->kb_count, &c->cap_kb_count, T);
+      else release(c, &c->kb_count, &c->cap_kb_count);
       if (!rc && T * K > c->cap_kb_elems) {
         uint64_t d1 = c->cap_kb_elems, d2 = c->cap_kb_elems;
         rc = reserve(c, &c->kb_col, &d1, T * K);
@@ -631,7 +632,7 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
       if (!rc && cov > 1) rc = reserve(c, &c->kb_mask, &c->cap_kb_mask, T * K);
       if (!rc && cov <= 1) release(c, &c->kb_mask, &c->cap_kb_mask);
       c->bytes_abuf = T * K * (cov > 1 ? 16u : 12u);
-      c->bytes_aux = T * (alg == OIT_SPINLOCK ? 12u : 4u);
+      c->bytes_aux = T * (alg == OIT_SPINLOCK ? 8u : 4u);
       break;
     }
     case OIT_LOOP64: {  // oit.cpp:134-140
@@ -923,7 +924,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       break;
     }
     case OIT_SPINLOCK: {
-      oit::KBuf kb{c->kb_count, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
+      oit::KBuf kb{nullptr, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
       kb.mask = coverage(c) ? c->kb_mask : nullptr;
       // a fragment's record is written by whichever fragment pushes it to the
       // tail, so this append's records start empty
@@ -1015,8 +1016,8 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     }
     case OIT_SIMPLE:
     case OIT_SPINLOCK: {
-      oit::KBuf kb{c->kb_count, alg == OIT_SPINLOCK ? c->kb_lock : nullptr, c->kb_depth, c->kb_col, c->kb_key,
-                   c->kb_id};
+      oit::KBuf kb{alg == OIT_SPINLOCK ? nullptr : c->kb_count, alg == OIT_SPINLOCK ? c->kb_lock : nullptr,
+                   c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
       if (coverage(c)) {
         kb.mask = c->kb_mask;
         e = staged(c, kStResolve, s, [&] { return oit::launch_kbuf_resolve_cov(k, kb, tbr, o, s); });

@@ -65,7 +65,7 @@ uint64_t request_bytes(const oit_request& r, uint64_t F) {
     case OIT_LOOP: return 12 * K * T + (tail ? 4 * T + 28 * F : 0);
     case OIT_LOOP64: return 8 * K * T + (tail ? 4 * T + 28 * F : 0);
     case OIT_SIMPLE: return 4 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
-    case OIT_SPINLOCK: return 12 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
+    case OIT_SPINLOCK: return 8 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
     default: return 20 * T;
   }
 }
