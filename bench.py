@@ -60,40 +60,56 @@ def cpu_baseline(tg, d, c, K, cap, budget_s):
     }
 
 
+def oracle_frame_banded(oit_oracle, tg, d, c, K, W, H, bands, ex):
+    """One Linked List frame of the oracle split over pixel bands: the
+    fragments are bucketed by band (a stable sort of the band index, so each
+    band keeps draw order), each band runs with its targets renumbered on the
+    executor `ex` (the oracle's C calls release the GIL), and the band images
+    are written into the frame's image. Pixels are independent, so the image
+    equals the full-frame run bit for bit."""
+    import numpy as np
+
+    rows = (H + bands - 1) // bands
+    span = np.uint32(W * rows)
+    out = np.empty((W * H, 4), np.float32)
+    band = (tg // span).astype(np.uint16)
+    order = np.argsort(band, kind="stable")
+    starts = np.concatenate([[0], np.cumsum(np.bincount(band, minlength=bands))])
+
+    def work(b):
+        h = min(rows, H - b * rows)
+        if h <= 0:
+            return
+        sel = order[starts[b]:starts[b + 1]]
+        first = b * W * rows
+        if len(sel) == 0:
+            cfg = oit_oracle.make_config(oit_oracle.LINKEDLIST, K, 1, W, h, ll_capacity=1)
+            out[first:first + W * h] = oit_oracle.run(cfg, tg[:0], d[:0], c[:0])["out"]
+            return
+        cfg = oit_oracle.make_config(oit_oracle.LINKEDLIST, K, 1, W, h, ll_capacity=len(sel) + 1)
+        out[first:first + W * h] = oit_oracle.run(cfg, tg[sel] - np.uint32(first), d[sel], c[sel])["out"]
+
+    list(ex.map(work, range(bands)))
+    return out
+
+
 def cpu_baseline_all_cores(tg, d, c, K, budget_s, threads):
-    """The same frame on `threads` host cores: fragments bucketed by pixel
-    band (a stable radix sort of the band index, timed), then the oracle on
-    each band in a thread pool (its C calls release the GIL). Pixels are
-    independent and each band keeps its fragments in draw order, so every
-    band's image is the full-frame result for its pixels."""
+    """The same frame on `threads` host cores (oracle_frame_banded with four
+    pixel bands per thread; the bucketing is timed with the runs)."""
     from concurrent.futures import ThreadPoolExecutor
 
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oit_oracle
-    import numpy as np
 
     oit_oracle.tables()  # tables built once, before the threads
     W, H = 1920, 1080
     bands = threads * 4
-    rows = (H + bands - 1) // bands
-
-    def work(args):
-        b, sel = args
-        if len(sel) == 0:
-            return
-        h = min(rows, H - b * rows)
-        cfg = oit_oracle.make_config(oit_oracle.LINKEDLIST, K, 1, W, h, ll_capacity=len(sel) + 1)
-        oit_oracle.run(cfg, tg[sel] - np.uint32(b * W * rows), d[sel], c[sel])
-
     times = []
     t_start = time.perf_counter()
     with ThreadPoolExecutor(threads) as ex:
         while True:
             t0 = time.perf_counter()
-            band = (tg // np.uint32(W * rows)).astype(np.uint16)
-            order = np.argsort(band, kind="stable")
-            starts = np.concatenate([[0], np.cumsum(np.bincount(band, minlength=bands))])
-            list(ex.map(work, [(b, order[starts[b]:starts[b + 1]]) for b in range(bands)]))
+            oracle_frame_banded(oit_oracle, tg, d, c, K, W, H, bands, ex)
             times.append(time.perf_counter() - t0)
             if time.perf_counter() - t_start > budget_s or len(times) >= 10:
                 break
