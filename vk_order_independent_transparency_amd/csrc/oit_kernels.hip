@@ -1078,18 +1078,24 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
   if (t >= c.T) return;
   const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
   const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
+  // layers past the wave's largest count are skipped with a scalar branch,
+  // and the k-th insert meets at most k occupied slots
+  const uint32_t nmax = wave_max_u32(n);
   FrontK<K> a;
   a.init();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if ((uint32_t)k < n) {
+    if ((uint32_t)k < nmax && (uint32_t)k < n) {
       const uint64_t idx = (uint64_t)k * c.T + t;
       uint64_t ev;
       uint32_t evc;
-      a.insert(((uint64_t)kb.key[idx] << 32) | kb.id[idx], kb.col[idx], ev, evc);
+      a.insert_bounded(((uint64_t)kb.key[idx] << 32) | kb.id[idx], kb.col[idx], ev, evc, k + 1);
     }
   }
-  const float4 col = blend_front<K>(a, n, c.tab);
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult(a.col[k], c.tab));
   float4 d = dest_of(c, t);
   if (kColorTail) d = tail_fold(tb, t, d, 0u);
   o.out[t] = over(col, d);
@@ -1110,15 +1116,16 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
   if (t >= c.T) return;
   const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
   const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
+  const uint32_t nmax = wave_max_u32(n);
   FrontK<K> a;  // (key << 32 | id, slot)
   a.init();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    if ((uint32_t)k < n) {
+    if ((uint32_t)k < nmax && (uint32_t)k < n) {
       const uint64_t idx = (uint64_t)k * c.T + t;
       uint64_t ev;
       uint32_t evc;
-      a.insert(((uint64_t)kb.key[idx] << 32) | kb.id[idx], (uint32_t)k, ev, evc);
+      a.insert_bounded(((uint64_t)kb.key[idx] << 32) | kb.id[idx], (uint32_t)k, ev, evc, k + 1);
     }
   }
   uint32_t cols[K], masks[K];
@@ -1310,7 +1317,9 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
 // bound, fill, replace the furthest) is replayed oldest first in registers;
 // each invocation's tail output (its shaded colour when rejected, the evicted
 // packed colour when it replaces one) is blended by the ROP in that same
-// order. The walk leaves the pool clean (heads 0, links kUnset).
+// order. The walk leaves the pool clean (heads 0, links kUnset). Launched
+// with coverage shading (kCov); without it k_interlock_resolve_sorted below
+// runs the same replay.
 template <int K, bool kCov>
 __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll, ResolveOut o) {
   __shared__ float s_dec[256];
@@ -1454,6 +1463,91 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
     if (o.fk_val) {
 #pragma unroll
       for (int k = 0; k < K; ++k) o.fk_val[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)sk[k] : kEmpty;
+    }
+    if (o.counts) o.counts[t] = count;
+  }
+}
+
+// The same replay without coverage shading, with the k-buffer kept sorted by
+// (depth key, id) in registers (FrontK). Ids make the keys unique, so "first
+// index wins" never decides, the furthest element is the last one, and
+// replacing it when the newcomer is nearer is the insertion that pushes the
+// last one out (:78-101): the set, the evictions and so the tail outputs
+// equal the slot-array replay's, each insert is bounded by the nodes seen so
+// far (a wave-uniform trip count), and the composite needs no sort.
+template <int K>
+__global__ __launch_bounds__(kBlock) void k_interlock_resolve_sorted(Common c, LLBuf ll, ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, c.tab);
+  if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
+    ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
+    ll.counters[threadIdx.x] = 0u;
+  }
+  __syncthreads();
+  const uint32_t stride = gridDim.x * kBlock;
+  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+    uint32_t p = ll.head[t];
+    if (p != 0u) ll.head[t] = 0u;
+    uint32_t prev = 0;
+    while (p != 0u && p < ll.cap) {  // reverse: oldest first
+      const uint32_t nx = ll.next[p];
+      ll.next[p] = prev;
+      prev = p;
+      p = nx;
+    }
+    FrontK<K> a;
+    a.init();
+    uint32_t count = 0;
+    uint32_t bound = kEmpty;  // imgAuxDepth cleared to ~0, oitRender.cpp:348
+    uint32_t trip = 0;        // nodes replayed so far: the same on every active lane
+    float4 cb = dest_of(c, t);
+    p = prev;
+    while (p != 0u && p < ll.cap) {
+      const int w = (int)__builtin_amdgcn_readfirstlane(trip + 1u);
+      ++trip;
+      const uint32_t nx = ll.next[p];
+      ll.next[p] = kUnset;
+      const uint64_t cd = ll.cd[p];
+      const uint32_t key = (uint32_t)(cd >> 32);
+      const uint64_t me = ((uint64_t)key << 32) | (p - 1u);
+      bool tail = true;
+      bool own = true;
+      uint32_t tcol = 0u;
+      if (key <= bound) {  // :62-64 early depth
+        uint64_t ev;
+        uint32_t evc;
+        a.insert_bounded(me, (uint32_t)cd, ev, evc, w);
+        if (count < (uint32_t)K) {  // :69-75 filled a free slot
+          tail = false;
+        } else if (ev != me) {  // :91-101 replaced the furthest
+          own = false;
+          tcol = evc;
+          bound = (uint32_t)(ev >> 32);
+        }
+        ++count;
+      }
+      if (tail && c.tail) {  // :106-108 the invocation's output, PREMULTIPLIED blend in primitive order
+        const float4 tc = own ? ll.full[p] : decode_premult_lds(tcol, s_dec, s_alpha);
+        cb = over(tc, cb);
+      }
+      p = nx;
+    }
+    // composite (:176-230): the stored layers are already sorted
+    const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
+    const uint32_t nmax = wave_max_u32(n);
+    float4 front = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+      if ((uint32_t)k < nmax && (uint32_t)k < n) front = over(front, decode_premult_lds(a.col[k], s_dec, s_alpha));
+    o.out[t] = over(front, cb);
+    if (o.fk_key) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) o.fk_key[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)(a.sk[k] >> 32) : kEmpty;
+    }
+    if (o.fk_val) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) o.fk_val[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)a.sk[k] : kEmpty;
     }
     if (o.counts) o.counts[t] = count;
   }
@@ -2245,7 +2339,7 @@ struct InterlockResolve {
     if (blocks > 8192u) blocks = 8192u;
     if (blocks == 0) blocks = 1;
     if (c.cov > 1u) hipLaunchKernelGGL((k_interlock_resolve<K, true>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
-    else hipLaunchKernelGGL((k_interlock_resolve<K, false>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
+    else hipLaunchKernelGGL((k_interlock_resolve_sorted<K>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
     return hipGetLastError();
   }
 };
