@@ -2215,9 +2215,9 @@ hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t h
 hipError_t launch_tail_link(const TailBuf& tb, uint32_t first, uint32_t frags, hipStream_t s) {
   const uint64_t n = frags < tb.cap ? frags : tb.cap;
   if (first >= n) return hipSuccess;
-  uint64_t blocks = (n - first + 4u * kBlock - 1) / (4u * kBlock);
+  uint64_t blocks = (n - first + 8u * kBlock - 1) / (8u * kBlock);  // eight 64-id chunks per wave and trip
   if (blocks > 4096u) blocks = 4096u;
-  hipLaunchKernelGGL(k_tail_link<4>, dim3((uint32_t)blocks), dim3(kBlock), 0, s, tb, first, frags);
+  hipLaunchKernelGGL(k_tail_link<8>, dim3((uint32_t)blocks), dim3(kBlock), 0, s, tb, first, frags);
   return hipGetLastError();
 }
 
