@@ -243,7 +243,7 @@ int launch(oit_server* s, Slot& sl) {
   if (oit_query_sizes(sl.ctx[alg], &z) == OIT_OK) sl.resident[alg] = z.resident_bytes;
   publish_held(s);
   if ((rc = sl.gen.emit(sl.target, sl.depth, reinterpret_cast<float*>(sl.rgba), sl.cap_target, sl.stream))) return rc;
-  oit_fragments f{sl.target, sl.depth, reinterpret_cast<const float*>(sl.rgba), F};
+  oit_fragments f{sl.target, sl.depth, reinterpret_cast<const float*>(sl.rgba), F, nullptr};
   if ((rc = oit_clear(sl.ctx[alg], sl.stream))) return rc;
   if ((rc = oit_append(sl.ctx[alg], &f, sl.stream))) return rc;
   if ((rc = oit_resolve(sl.ctx[alg], out, sl.stream))) return rc;
