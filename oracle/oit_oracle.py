@@ -11,7 +11,9 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboit_oracle.so")
+# OIT_ORACLE_LIB selects another build of the same source (the sanitizer
+# build of `make -C oracle sanitize`)
+LIB_PATH = os.environ.get("OIT_ORACLE_LIB") or os.path.join(HERE, "liboit_oracle.so")
 
 SIMPLE, LINKEDLIST, LOOP, LOOP64, SPINLOCK, INTERLOCK, WEIGHTED = 0, 1, 2, 3, 4, 5, 6
 TAIL_CANONICAL, TAIL_SEQUENTIAL = 0, 1
