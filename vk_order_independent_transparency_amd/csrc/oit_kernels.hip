@@ -136,6 +136,46 @@ struct FrontK {
   }
 };
 
+// FrontK with a third payload moved alongside (the ordered interlock's sample
+// masks under coverage shading); the same bounded insert.
+template <int K>
+struct FrontKM {
+  uint64_t sk[K];
+  uint32_t col[K];
+  uint32_t msk[K];
+  __device__ __forceinline__ void init() {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      sk[k] = kEmpty64;
+      col[k] = 0u;
+      msk[k] = 0u;
+    }
+  }
+  __device__ __forceinline__ void insert_bounded(uint64_t x, uint32_t xc, uint32_t xm, uint64_t& ev, uint32_t& evc,
+                                                 int w) {
+    const bool fits = x < sk[K - 1];
+    ev = fits ? sk[K - 1] : x;
+    evc = fits ? col[K - 1] : xc;
+#pragma unroll
+    for (int b = ((K - 1) / 4) * 4; b >= 0; b -= 4) {
+      if (b < w) {
+#pragma unroll
+        for (int j = (b + 3 < K - 1 ? b + 3 : K - 1); j >= b && j > 0; --j) {
+          const bool before = x < sk[j - 1];
+          const bool here = x < sk[j];
+          sk[j] = before ? sk[j - 1] : (here ? x : sk[j]);
+          col[j] = before ? col[j - 1] : (here ? xc : col[j]);
+          msk[j] = before ? msk[j - 1] : (here ? xm : msk[j]);
+        }
+      }
+    }
+    const bool first = x < sk[0];
+    sk[0] = first ? x : sk[0];
+    col[0] = first ? xc : col[0];
+    msk[0] = first ? xm : msk[0];
+  }
+};
+
 // Sorted register array of K sort keys only (depth key << 32 | node index),
 // ascending: the colours are reloaded by node index after the walk, which
 // keeps one third fewer registers live during the latency-bound walk.
@@ -1317,9 +1357,17 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
 // bound, fill, replace the furthest) is replayed oldest first in registers;
 // each invocation's tail output (its shaded colour when rejected, the evicted
 // packed colour when it replaces one) is blended by the ROP in that same
-// order. The walk leaves the pool clean (heads 0, links kUnset). Launched
-// with coverage shading (kCov); without it k_interlock_resolve_sorted below
-// runs the same replay.
+// order. The walk leaves the pool clean (heads 0, links kUnset).
+//
+// The k-buffer is kept sorted by (depth key, id) in registers. Ids make the
+// keys unique, so "first index wins" never decides, the furthest element is
+// the last one, and replacing it when the newcomer is nearer is the
+// insertion that pushes the last one out (:78-101): the set, the evictions
+// and so the tail outputs equal those of the slot array the shader keeps,
+// each insert is bounded by the nodes seen so far (a wave-uniform trip
+// count), and the composite needs no sort. With coverage shading (kCov) the
+// stored layers carry their sample masks, each invocation's tail output goes
+// to the samples of its own mask, and the front K are blended per sample.
 template <int K, bool kCov>
 __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll, ResolveOut o) {
   __shared__ float s_dec[256];
@@ -1341,68 +1389,40 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
       prev = p;
       p = nx;
     }
-    uint64_t sk[K];
-    uint32_t scol[K], smask[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) {
-      sk[k] = kEmpty64;
-      scol[k] = 0u;
-      smask[k] = 0u;
-    }
+    FrontKM<K> a;
+    a.init();
     uint32_t count = 0;
     uint32_t bound = kEmpty;  // imgAuxDepth cleared to ~0, oitRender.cpp:348
+    uint32_t trip = 0;        // nodes replayed so far: the same on every active lane
     float4 cb = kCov ? c.background : dest_of(c, t);
-    float4 cbs[kCov ? 8 : 1];
+    float4 cbs[kCov ? 8 : 1];  // coverage shading: the colour buffer per sample
 #pragma unroll
     for (int s = 0; s < (kCov ? 8 : 1); ++s)
       cbs[s] = (kCov && c.opq_rgba && (uint32_t)s < c.cov) ? c.opq_rgba[(uint64_t)s * c.T + t] : c.background;
     bool seen = false;
     p = prev;
     while (p != 0u && p < ll.cap) {
+      const int w = (int)__builtin_amdgcn_readfirstlane(trip + 1u);
+      ++trip;
       const uint32_t nx = ll.next[p];
       ll.next[p] = kUnset;
       const uint64_t cd = ll.cd[p];
       const uint32_t key = (uint32_t)(cd >> 32);
-      const uint32_t col = (uint32_t)cd;
       const uint64_t me = ((uint64_t)key << 32) | (p - 1u);
       const uint32_t mm = kCov ? ll.mask[p] : 0xFFFFFFFFu;
       bool tail = true;
       bool own = true;
       uint32_t tcol = 0u;
       if (key <= bound) {  // :62-64 early depth
-        if (count < (uint32_t)K) {  // :69-75
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
-            if ((uint32_t)k == count) {
-              sk[k] = me;
-              scol[k] = col;
-              smask[k] = mm;
-            }
-          }
+        uint64_t ev;
+        uint32_t evc;
+        a.insert_bounded(me, (uint32_t)cd, mm, ev, evc, w);
+        if (count < (uint32_t)K) {  // :69-75 filled a free slot
           tail = false;
-        } else {  // :78-101 the furthest (first index wins), replaced when behind
-          uint64_t mx = 0;
-          uint32_t fur = 0;
-#pragma unroll
-          for (int k = 0; k < K; ++k) {
-            if (sk[k] > mx) {
-              mx = sk[k];
-              fur = (uint32_t)k;
-            }
-          }
-          if (mx > me) {
-            own = false;
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-              if ((uint32_t)k == fur) {
-                tcol = scol[k];
-                sk[k] = me;
-                scol[k] = col;
-                smask[k] = mm;
-              }
-            }
-            bound = (uint32_t)(mx >> 32);
-          }
+        } else if (ev != me) {  // :91-101 replaced the furthest
+          own = false;
+          tcol = evc;
+          bound = (uint32_t)(ev >> 32);
         }
         ++count;
       }
@@ -1419,128 +1439,27 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
       }
       p = nx;
     }
-    // composite (:176-230): the stored layers sorted by (depth key, id)
+    // composite (:176-230): the stored layers are already sorted
     const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
-#pragma unroll
-    for (int i = 1; i < K; ++i) {
-#pragma unroll
-      for (int j = i; j > 0; --j) {
-        const bool sw = sk[j - 1] > sk[j];
-        const uint64_t a0 = sk[j - 1], a1 = sk[j];
-        const uint32_t c0 = scol[j - 1], c1 = scol[j], m0 = smask[j - 1], m1 = smask[j];
-        sk[j - 1] = sw ? a1 : a0;
-        sk[j] = sw ? a0 : a1;
-        scol[j - 1] = sw ? c1 : c0;
-        scol[j] = sw ? c0 : c1;
-        smask[j - 1] = sw ? m1 : m0;
-        smask[j] = sw ? m0 : m1;
-      }
-    }
     float4 front = make_float4(0.f, 0.f, 0.f, 0.f);
     float4 d = cb;
     if constexpr (kCov) {
-      front = blend_front_cov<K>(scol, smask, n, c.cov, s_dec, s_alpha);
-      if (seen || c.opq_rgba) {
+      front = blend_front_cov<K>(a.col, a.msk, n, c.cov, s_dec, s_alpha);
+      if (seen || c.opq_rgba) {  // the MSAA resolve of the per-sample colour buffer
         float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
         for (int s = 0; s < 8; ++s)
           if ((uint32_t)s < c.cov) sum = make_float4(sum.x + cbs[s].x, sum.y + cbs[s].y, sum.z + cbs[s].z, sum.w + cbs[s].w);
         const float fs = (float)c.cov;
         d = make_float4(sum.x / fs, sum.y / fs, sum.z / fs, sum.w / fs);
-      } else {
-        d = c.background;
       }
     } else {
+      const uint32_t nmax = wave_max_u32(n);
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if ((uint32_t)k < n) front = over(front, decode_premult_lds(scol[k], s_dec, s_alpha));
+        if ((uint32_t)k < nmax && (uint32_t)k < n) front = over(front, decode_premult_lds(a.col[k], s_dec, s_alpha));
     }
     o.out[t] = over(front, d);
-    if (o.fk_key) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) o.fk_key[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)(sk[k] >> 32) : kEmpty;
-    }
-    if (o.fk_val) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) o.fk_val[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)sk[k] : kEmpty;
-    }
-    if (o.counts) o.counts[t] = count;
-  }
-}
-
-// The same replay without coverage shading, with the k-buffer kept sorted by
-// (depth key, id) in registers (FrontK). Ids make the keys unique, so "first
-// index wins" never decides, the furthest element is the last one, and
-// replacing it when the newcomer is nearer is the insertion that pushes the
-// last one out (:78-101): the set, the evictions and so the tail outputs
-// equal the slot-array replay's, each insert is bounded by the nodes seen so
-// far (a wave-uniform trip count), and the composite needs no sort.
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_interlock_resolve_sorted(Common c, LLBuf ll, ResolveOut o) {
-  __shared__ float s_dec[256];
-  __shared__ float s_alpha[256];
-  stage_decode(s_dec, s_alpha, c.tab);
-  if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
-    ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
-    ll.counters[threadIdx.x] = 0u;
-  }
-  __syncthreads();
-  const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
-    uint32_t p = ll.head[t];
-    if (p != 0u) ll.head[t] = 0u;
-    uint32_t prev = 0;
-    while (p != 0u && p < ll.cap) {  // reverse: oldest first
-      const uint32_t nx = ll.next[p];
-      ll.next[p] = prev;
-      prev = p;
-      p = nx;
-    }
-    FrontK<K> a;
-    a.init();
-    uint32_t count = 0;
-    uint32_t bound = kEmpty;  // imgAuxDepth cleared to ~0, oitRender.cpp:348
-    uint32_t trip = 0;        // nodes replayed so far: the same on every active lane
-    float4 cb = dest_of(c, t);
-    p = prev;
-    while (p != 0u && p < ll.cap) {
-      const int w = (int)__builtin_amdgcn_readfirstlane(trip + 1u);
-      ++trip;
-      const uint32_t nx = ll.next[p];
-      ll.next[p] = kUnset;
-      const uint64_t cd = ll.cd[p];
-      const uint32_t key = (uint32_t)(cd >> 32);
-      const uint64_t me = ((uint64_t)key << 32) | (p - 1u);
-      bool tail = true;
-      bool own = true;
-      uint32_t tcol = 0u;
-      if (key <= bound) {  // :62-64 early depth
-        uint64_t ev;
-        uint32_t evc;
-        a.insert_bounded(me, (uint32_t)cd, ev, evc, w);
-        if (count < (uint32_t)K) {  // :69-75 filled a free slot
-          tail = false;
-        } else if (ev != me) {  // :91-101 replaced the furthest
-          own = false;
-          tcol = evc;
-          bound = (uint32_t)(ev >> 32);
-        }
-        ++count;
-      }
-      if (tail && c.tail) {  // :106-108 the invocation's output, PREMULTIPLIED blend in primitive order
-        const float4 tc = own ? ll.full[p] : decode_premult_lds(tcol, s_dec, s_alpha);
-        cb = over(tc, cb);
-      }
-      p = nx;
-    }
-    // composite (:176-230): the stored layers are already sorted
-    const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
-    const uint32_t nmax = wave_max_u32(n);
-    float4 front = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-    for (int k = 0; k < K; ++k)
-      if ((uint32_t)k < nmax && (uint32_t)k < n) front = over(front, decode_premult_lds(a.col[k], s_dec, s_alpha));
-    o.out[t] = over(front, cb);
     if (o.fk_key) {
 #pragma unroll
       for (int k = 0; k < K; ++k) o.fk_key[(uint64_t)t * K + k] = (uint32_t)k < n ? (uint32_t)(a.sk[k] >> 32) : kEmpty;
@@ -2338,8 +2257,10 @@ struct InterlockResolve {
     uint32_t blocks = blocks_for(c.T);
     if (blocks > 8192u) blocks = 8192u;
     if (blocks == 0) blocks = 1;
-    if (c.cov > 1u) hipLaunchKernelGGL((k_interlock_resolve<K, true>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
-    else hipLaunchKernelGGL((k_interlock_resolve_sorted<K>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
+    if (c.cov > 1u)
+      hipLaunchKernelGGL((k_interlock_resolve<K, true>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
+    else
+      hipLaunchKernelGGL((k_interlock_resolve<K, false>), dim3(blocks), dim3(kBlock), 0, s, c, ll, o);
     return hipGetLastError();
   }
 };
