@@ -83,11 +83,11 @@ def test_several_appends_per_frame(oracle, alg):
         _same(f"out splits={splits}", g["out"], o["out"])
 
 
-@pytest.mark.parametrize("S", [2, 4])
+@pytest.mark.parametrize("S", [2, 4, 8])
 def test_interlock_coverage(oracle, S):
     p, tg, d, c = deep_scene(seed=91)
     m = np.random.default_rng(91 + S).integers(1, 1 << S, len(tg)).astype(np.uint32)
-    for K in (4, 16):
+    for K in (1, 4, 16, 32):
         g = run_gpu(INTERLOCK, K, 1, p.width, p.height, tg, d, c, coverage_samples=S, mask=m)
         o = oracle.run(oracle.make_config(oracle.INTERLOCK, K, 1, p.width, p.height, coverage_samples=S), tg, d, c,
                        mask=m)
