@@ -290,6 +290,25 @@ def bench_serve(seed, n, rates, slots):
     # (one slot, so each request's launches run alone on the device); both
     # with warm resident contexts
     reqs, _ = serve.served_workload(seed, n, rates[-1])
+    burst = serve_burst(serve, srv, reqs, slots)
+    log("served burst vs one at a time: " + json.dumps(burst))
+    srv.close()
+    # the same generator with 256x256 and 512x512 scenes only: requests whose
+    # device work is short, so the per-request steps (count, sizing, launches)
+    # weigh more
+    small, _ = serve.served_workload(seed, n, rates[-1], resolutions=((256, 256), (512, 512)))
+    srv = serve.Server(0, slots=slots)
+    srv.replay(small, [0.0] * len(small))
+    burst_small = serve_burst(serve, srv, small, slots)
+    log("served small requests, burst vs one at a time: " + json.dumps(burst_small))
+    srv.close()
+    return out, burst, burst_small
+
+
+def serve_burst(serve, srv, reqs, slots):
+    """All of `reqs` submitted at once on the warm server `srv` against the
+    same requests on a warm one-slot server: fragments per second over each
+    busy span."""
     burst = {}
     for name, server in (("slots", srv), ("one_at_a_time", None)):
         one = server is None
@@ -305,9 +324,7 @@ def bench_serve(seed, n, rates, slots):
             server.close()
     burst["slots"]["slots"] = slots
     burst["speedup"] = round(burst["slots"]["G_fragments_per_s"] / burst["one_at_a_time"]["G_fragments_per_s"], 3)
-    log("served burst vs one at a time: " + json.dumps(burst))
-    srv.close()
-    return out, burst
+    return burst
 
 
 def _under_load(served):
@@ -431,10 +448,11 @@ def main():
     # the same generator with only 256x256 and 512x512 frames: per-frame launch
     # and clear costs weigh more than the fragment work
     batch_small = None if args.no_batch else bench_batch(1, args.batch_frames, (256, 512))
-    served = served_burst = None
+    served = served_burst = served_small = None
     if not args.no_serve:
-        served, served_burst = bench_serve(1, args.serve_requests, [float(x) for x in args.serve_rates.split(",")],
-                                           args.serve_slots)
+        served, served_burst, served_small = bench_serve(1, args.serve_requests,
+                                                         [float(x) for x in args.serve_rates.split(",")],
+                                                         args.serve_slots)
 
     cpu = cpu_all = None
     if not args.no_cpu_baseline:
@@ -489,6 +507,7 @@ def main():
         "batch_small_frames": batch_small,
         "serve": served,
         "serve_vs_one_at_a_time": served_burst,
+        "serve_small_requests_vs_one_at_a_time": served_small,
         "stages": stages,
         "ll_overflow": int(st.ll_overflow),
         "cpu_baseline": cpu,
