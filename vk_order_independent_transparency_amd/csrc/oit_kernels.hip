@@ -1114,6 +1114,10 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
 // blended front to back, over the colour buffer.
 template <int K, bool kColorTail, bool kDebug>
 __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, TailBuf tb, ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, c.tab);
+  __syncthreads();
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
   const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
@@ -1135,7 +1139,7 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < K; ++k)
-    if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult(a.col[k], c.tab));
+    if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
   float4 d = dest_of(c, t);
   if (kColorTail) d = tail_fold(tb, t, d, 0u);
   o.out[t] = over(col, d);
@@ -1308,6 +1312,10 @@ template <int K, bool kColorTail, bool kDebug>
 __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint32_t* __restrict__ zs,
                                                            const uint64_t* __restrict__ cs, TailBuf tb,
                                                            ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, c.tab);
+  __syncthreads();
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
   uint32_t z[K], col[K];
@@ -1330,7 +1338,7 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < K; ++k)
-    if ((uint32_t)k < n) acc = over(acc, decode_premult(col[k], c.tab));
+    if ((uint32_t)k < n) acc = over(acc, decode_premult_lds(col[k], s_dec, s_alpha));
   float4 d = dest_of(c, t);
   if (kColorTail) d = tail_fold(tb, t, d, 0u);
   o.out[t] = over(acc, d);
@@ -1590,6 +1598,10 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
 template <int K, bool kColorTail, bool kDebug>
 __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint64_t* __restrict__ slots, TailBuf tb,
                                                            ResolveOut o) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, c.tab);
+  __syncthreads();
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
   // slots are sorted with the empty ones last: the upper half is loaded only
@@ -1608,7 +1620,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint6
   for (int k = 0; k < K; ++k) {
     alive = alive && (uint32_t)(s[k] >> 32) != kEmpty;
     if (alive) {
-      col = over(col, decode_premult((uint32_t)s[k], c.tab));
+      col = over(col, decode_premult_lds((uint32_t)s[k], s_dec, s_alpha));
       ++n;
     }
   }
