@@ -502,27 +502,34 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
   bool node[kU], ovf[kU];
   float4 col[kU];
   float dep[kU];
+  // every field of every chunk is loaded before any is looked at, so the
+  // loads share one round trip (a load guarded by a test of the target
+  // would wait for it)
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     const uint64_t i = f.first + (uint64_t)u * 64u + lane;
-    t[u] = i < f.count ? ld_stream(&f.target[i]) : kEmpty;
+    const bool in = i < f.count;
+    t[u] = in ? ld_stream(&f.target[i]) : kEmpty;
+    dep[u] = in ? ld_stream(&f.depth[i]) : 0.f;
+    col[u] = in ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    m[u] = (in && f.mask) ? ld_stream(&f.mask[i]) : 0xFFFFFFFFu;
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t i = f.first + (uint64_t)u * 64u + lane;
     if (t[u] >= f.T) t[u] = kEmpty;
-    m[u] = 0xFFFFFFFFu;
     if (f.mask && t[u] != kEmpty) {
-      m[u] = ld_stream(&f.mask[i]) & f.full;
+      m[u] &= f.full;
       if (m[u] == 0u) t[u] = kEmpty;  // no sample covered: the shader does not run
     }
     if (f.opq && t[u] != kEmpty) {  // behind the opaque depth in every covered sample: culled
-      m[u] = opaque_visible(f.opq, f.T, f.cov, t[u], depth_key(ld_stream(&f.depth[i])), m[u]);
+      m[u] = opaque_visible(f.opq, f.T, f.cov, t[u], depth_key(dep[u]), m[u]);
       if (m[u] == 0u) t[u] = kEmpty;
     }
     const uint64_t n64 = (uint64_t)f.node0 + i;
     node[u] = t[u] != kEmpty && n64 < ll.cap;
     ovf[u] = t[u] != kEmpty && !node[u];
-    col[u] = make_float4(0.f, 0.f, 0.f, 0.f);
-    dep[u] = 0.f;
-    if (t[u] != kEmpty) col[u] = ld_stream(&f.rgba[i]);
-    if (node[u]) dep[u] = ld_stream(&f.depth[i]);
+    if (t[u] == kEmpty) col[u] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
   uint32_t lo[kU], hi[kU];
   bool ins[kU];
@@ -1125,15 +1132,29 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
   // layers past the wave's largest count are skipped with a scalar branch,
   // and the k-th insert meets at most k occupied slots
   const uint32_t nmax = wave_max_u32(n);
+  // every stored layer's planes are loaded before the first insert, so the
+  // loads share one round trip instead of one each
+  uint32_t lk[K], li[K], lc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    lk[k] = 0u;
+    li[k] = 0u;
+    lc[k] = 0u;
+    if ((uint32_t)k < nmax && (uint32_t)k < n) {
+      const uint64_t idx = (uint64_t)k * c.T + t;
+      lk[k] = kb.key[idx];
+      li[k] = kb.id[idx];
+      lc[k] = kb.col[idx];
+    }
+  }
   FrontK<K> a;
   a.init();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if ((uint32_t)k < nmax && (uint32_t)k < n) {
-      const uint64_t idx = (uint64_t)k * c.T + t;
       uint64_t ev;
       uint32_t evc;
-      a.insert_bounded(((uint64_t)kb.key[idx] << 32) | kb.id[idx], kb.col[idx], ev, evc, k + 1);
+      a.insert_bounded(((uint64_t)lk[k] << 32) | li[k], lc[k], ev, evc, k + 1);
     }
   }
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1161,15 +1182,25 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
   const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
   const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
   const uint32_t nmax = wave_max_u32(n);
+  uint32_t lk[K], li[K];  // loaded before the first insert: one round trip
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    lk[k] = 0u;
+    li[k] = 0u;
+    if ((uint32_t)k < nmax && (uint32_t)k < n) {
+      const uint64_t idx = (uint64_t)k * c.T + t;
+      lk[k] = kb.key[idx];
+      li[k] = kb.id[idx];
+    }
+  }
   FrontK<K> a;  // (key << 32 | id, slot)
   a.init();
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     if ((uint32_t)k < nmax && (uint32_t)k < n) {
-      const uint64_t idx = (uint64_t)k * c.T + t;
       uint64_t ev;
       uint32_t evc;
-      a.insert_bounded(((uint64_t)kb.key[idx] << 32) | kb.id[idx], (uint32_t)k, ev, evc, k + 1);
+      a.insert_bounded(((uint64_t)lk[k] << 32) | li[k], (uint32_t)k, ev, evc, k + 1);
     }
   }
   uint32_t cols[K], masks[K];
