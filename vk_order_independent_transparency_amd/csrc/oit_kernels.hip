@@ -924,21 +924,24 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
+  // the fragment's fields in one round trip, before any is tested
   uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
+  const float dv = valid ? ld_stream(&f.depth[i]) : 0.f;
+  float4 col = valid ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t m = (valid && f.mask) ? ld_stream(&f.mask[i]) : 0xFFFFFFFFu;
   if (t >= c.T) {
     valid = false;
     t = kEmpty;
   }
-  uint32_t m = 0xFFFFFFFFu;
   if (valid && f.mask) {
-    m = ld_stream(&f.mask[i]) & ((1u << c.cov) - 1u);
+    m &= (1u << c.cov) - 1u;
     if (m == 0u) {  // no sample covered: the shader does not run
       valid = false;
       t = kEmpty;
     }
   }
   if (valid && c.opq_z) {  // early depth test against the opaque pass
-    m = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(ld_stream(&f.depth[i])), m);
+    m = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(dv), m);
     if (m == 0u) {
       valid = false;
       t = kEmpty;
@@ -950,14 +953,12 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
   if (valid && r.start) base = atomicAdd(&kb.count[t], r.end_lane - r.lane + 1u);
   base = __shfl(base, (int)r.start_lane);
   const uint32_t slot = base + (r.lane - r.start_lane);
-  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
   bool tail = false;
   if (valid) {
-    col = ld_stream(&f.rgba[i]);
     if (slot < c.K) {
       const uint64_t idx = (uint64_t)slot * c.T + t;
       kb.col[idx] = encode_color_lds(col, s_thr);
-      kb.key[idx] = depth_key(ld_stream(&f.depth[i]));
+      kb.key[idx] = depth_key(dv);
       kb.id[idx] = lid;
       if (kb.mask) kb.mask[idx] = m;  // storeMask, oitSimple.frag.glsl:70
     } else {
@@ -1038,15 +1039,18 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
-  uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
+  // the fragment's fields in one round trip, before any is tested
+  const uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
+  const float dv = valid ? ld_stream(&f.depth[i]) : 0.f;
+  const float4 cv = valid ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t my_mask = (valid && f.mask) ? ld_stream(&f.mask[i]) : 0xFFFFFFFFu;
   if (t >= c.T) valid = false;
-  uint32_t my_mask = 0xFFFFFFFFu;
   if (valid && f.mask) {
-    my_mask = ld_stream(&f.mask[i]) & ((1u << c.cov) - 1u);
+    my_mask &= (1u << c.cov) - 1u;
     if (my_mask == 0u) valid = false;  // no sample covered: the shader does not run
   }
   if (valid && c.opq_z) {  // early depth test against the opaque pass
-    my_mask = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(ld_stream(&f.depth[i])), my_mask);
+    my_mask = opaque_visible(c.opq_z, c.T, c.cov, t, depth_key(dv), my_mask);
     if (my_mask == 0u) valid = false;
   }
   const uint32_t lid = f.id_base + i;
@@ -1056,8 +1060,8 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   uint32_t tail_mask = my_mask;
   uint32_t my_key = 0u, my_col = 0u;
   if (valid) {
-    my_key = depth_key(ld_stream(&f.depth[i]));
-    my_col = encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
+    my_key = depth_key(dv);
+    my_col = encode_color_lds(cv, s_thr);
     tail_col = my_col;
   }
   // pairs: the even lane of a same-target run takes its odd neighbour along
@@ -1234,11 +1238,13 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
 template <int K>
 __global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, uint32_t* __restrict__ zs) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t t = i < f.count ? ld_stream(&f.target[i]) : kEmpty;
+  const bool in = i < f.count;
+  uint32_t t = in ? ld_stream(&f.target[i]) : kEmpty;
+  const float dv = in ? ld_stream(&f.depth[i]) : 0.f;  // with the target: one round trip
   if (t >= c.T) t = kEmpty;
   uint32_t z = 0u;
   if (t != kEmpty) {
-    z = depth_key(ld_stream(&f.depth[i]));
+    z = depth_key(dv);
     if (!opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) t = kEmpty;
   }
   const Run r = same_target_run(t);
@@ -1300,20 +1306,23 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color(Common c, FragView f, c
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
+  // the fragment's fields in one round trip, before any is tested
   uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
+  const float dv = valid ? ld_stream(&f.depth[i]) : 0.f;
+  const float4 cv = valid ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
   if (t >= c.T) {
     valid = false;
     t = kEmpty;
   }
   bool tail = false;
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t z = valid ? depth_key(ld_stream(&f.depth[i])) : 0u;
+  const uint32_t z = valid ? depth_key(dv) : 0u;
   if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) {
     valid = false;
     t = kEmpty;
   }
   if (valid) {
-    col = ld_stream(&f.rgba[i]);
+    col = cv;
     // the slot of z: the number of stored depths below it. Slots are sorted
     // with the empty ones last, so the lower half loaded at once decides
     // unless all of it lies below z (a pixel holding more than K/2 depths):
@@ -1532,12 +1541,16 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, u
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t t = i < f.count ? ld_stream(&f.target[i]) : kEmpty;
+  // the fragment's fields in one round trip, before any is tested
+  const bool in = i < f.count;
+  uint32_t t = in ? ld_stream(&f.target[i]) : kEmpty;
+  const float dv = in ? ld_stream(&f.depth[i]) : 0.f;
+  const float4 cv = in ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
   if (t >= c.T) t = kEmpty;
   uint64_t z = kEmpty64;
   if (t != kEmpty) {
-    const uint32_t key = depth_key(ld_stream(&f.depth[i]));
-    if (opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) z = ((uint64_t)key << 32) | encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
+    const uint32_t key = depth_key(dv);
+    if (opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) z = ((uint64_t)key << 32) | encode_color_lds(cv, s_thr);
     else t = kEmpty;
   }
   const Run r = same_target_run(t);
@@ -1600,11 +1613,12 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
   uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
+  const float dv = valid ? ld_stream(&f.depth[i]) : 0.f;  // with the target: one round trip
   if (t >= c.T) valid = false;
   bool tail = false;
   uint32_t flag = 0;
   float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t key = valid ? depth_key(ld_stream(&f.depth[i])) : 0u;
+  const uint32_t key = valid ? depth_key(dv) : 0u;
   if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) valid = false;
   if (valid) {
     // the key's high half is the depth: a fragment nearer than the K-th
@@ -1687,17 +1701,18 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint6
 // reveal *= (1 - a) (ZERO, ONE_MINUS_SRC_COLOR) as a compare-and-swap loop.
 __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WBBuf wb, float vnear, float vspan) {
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  uint32_t t = kEmpty;
   float4 wc = make_float4(0.f, 0.f, 0.f, 0.f);
   float lr = 0.f;
-  if (i < f.count) {
-    t = ld_stream(&f.target[i]);
-    if (t >= c.T) t = kEmpty;
-  }
-  if (t != kEmpty && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(ld_stream(&f.depth[i])), 1u)) t = kEmpty;
+  // the fragment's fields in one round trip, before any is tested
+  const bool in = i < f.count;
+  uint32_t t = in ? ld_stream(&f.target[i]) : kEmpty;
+  const float dv = in ? ld_stream(&f.depth[i]) : 0.f;
+  const float4 cv = in ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  if (t >= c.T) t = kEmpty;
+  if (t != kEmpty && !opaque_visible(c.opq_z, c.T, 1u, t, depth_key(dv), 1u)) t = kEmpty;
   if (t != kEmpty) {
-    const float4 col = ld_stream(&f.rgba[i]);
-    const float view = vnear + ld_stream(&f.depth[i]) * vspan;   // stands in for -viewZ (object.vert.glsl:35)
+    const float4 col = cv;
+    const float view = vnear + dv * vspan;   // stands in for -viewZ (object.vert.glsl:35)
     const float depthZ = view * 10.0f;               // :65
     const float x = depthZ / 200.0f;
     const float x2 = x * x;
