@@ -1366,15 +1366,20 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
   constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
 #pragma unroll
   for (int k = 0; k < kHalf; ++k) z[k] = zs[(uint64_t)k * c.T + t];
-  const bool upper = z[kHalf - 1] != kEmpty;
 #pragma unroll
-  for (int k = kHalf; k < K; ++k) z[k] = upper ? zs[(uint64_t)k * c.T + t] : kEmpty;
+  for (int k = kHalf; k < K; ++k) z[k] = kEmpty;
+  if (z[kHalf - 1] != kEmpty) {
+#pragma unroll
+    for (int k = kHalf; k < K; ++k) z[k] = zs[(uint64_t)k * c.T + t];
+  }
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     alive = alive && z[k] != kEmpty;
-    col[k] = alive ? (uint32_t)cs[(uint64_t)k * c.T + t] : 0u;
     n += alive ? 1u : 0u;
   }
+  // the stored layers' colours, loaded together once n is known
+#pragma unroll
+  for (int k = 0; k < K; ++k) col[k] = (uint32_t)k < n ? (uint32_t)cs[(uint64_t)k * c.T + t] : 0u;
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < K; ++k)
