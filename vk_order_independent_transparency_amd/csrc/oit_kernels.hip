@@ -233,16 +233,18 @@ __device__ __forceinline__ float4 blend_front(const FrontK<K>& a, uint32_t n, co
 // the A-buffer and not blended (Loop64 identical keys, see
 // k_loop64_classify): the ones with the lowest ids, i.e. the last flagged
 // entries of the walk.
-__device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float4 bg, uint32_t skip) {
+// tail_fold with the list's head value already loaded (a composite issues
+// that load with its own first loads instead of after them).
+__device__ __forceinline__ float4 tail_fold_from(const TailBuf& tb, uint32_t head, float4 bg, uint32_t skip) {
   const uint32_t vcap = tail_value_cap(tb.cap);
   uint32_t keep = 0xFFFFFFFFu;  // flagged entries to blend
   if (skip > 0u) {
     uint32_t flagged = 0;
-    for (uint32_t v = tb.head[t]; v != 0u && v < vcap; v = tb.next[tail_index(v)]) flagged += v & 1u;
+    for (uint32_t v = head; v != 0u && v < vcap; v = tb.next[tail_index(v)]) flagged += v & 1u;
     keep = flagged > skip ? flagged - skip : 0u;
   }
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  uint32_t v = tb.head[t];
+  uint32_t v = head;
   while (v != 0u && v < vcap) {  // a link outside the arrays ends the walk
     const uint32_t id = tail_index(v);
     const uint32_t nx = tb.next[id];
@@ -253,6 +255,10 @@ __device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float
     v = nx;
   }
   return over(acc, bg);
+}
+
+__device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float4 bg, uint32_t skip) {
+  return tail_fold_from(tb, tb.head[t], bg, skip);
 }
 
 // Colour buffer of target t before the colour-pass tail: the colour of the
@@ -1132,6 +1138,7 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
   const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
+  const uint32_t th = kColorTail ? tb.head[t] : 0u;                   // tail list head, with the count
   const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
   // layers past the wave's largest count are skipped with a scalar branch,
   // and the k-th insert meets at most k occupied slots
@@ -1166,7 +1173,7 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
   for (int k = 0; k < K; ++k)
     if ((uint32_t)k < nmax && (uint32_t)k < n) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
   float4 d = dest_of(c, t);
-  if (kColorTail) d = tail_fold(tb, t, d, 0u);
+  if (kColorTail) d = tail_fold_from(tb, th, d, 0u);
   o.out[t] = over(col, d);
   if (kDebug) write_debug<K>(o, t, a, n, count, false, false);
 }
@@ -1364,6 +1371,7 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
   // depths are sorted with the empty slots last: the upper half is loaded
   // only where the lower half is full
   constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
+  const uint32_t th = kColorTail ? tb.head[t] : 0u;  // tail list head, with the depths
 #pragma unroll
   for (int k = 0; k < kHalf; ++k) z[k] = zs[(uint64_t)k * c.T + t];
 #pragma unroll
@@ -1385,7 +1393,7 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
   for (int k = 0; k < K; ++k)
     if ((uint32_t)k < n) acc = over(acc, decode_premult_lds(col[k], s_dec, s_alpha));
   float4 d = dest_of(c, t);
-  if (kColorTail) d = tail_fold(tb, t, d, 0u);
+  if (kColorTail) d = tail_fold_from(tb, th, d, 0u);
   o.out[t] = over(acc, d);
   if (kDebug) {
     if (o.fk_key) {
@@ -1657,6 +1665,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint6
   // slots are sorted with the empty ones last: the upper half is loaded only
   // where the lower half is full
   constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
+  const uint32_t th = kColorTail ? tb.head[t] : 0u;  // tail list head, with the slots
   uint64_t s[K];
 #pragma unroll
   for (int k = 0; k < kHalf; ++k) s[k] = slots[(uint64_t)k * c.T + t];
@@ -1681,7 +1690,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint6
 #pragma unroll
       for (int k = 0; k < K; ++k) skip += s[k] == s[K - 1] ? 1u : 0u;
     }
-    d = tail_fold(tb, t, d, skip);
+    d = tail_fold_from(tb, th, d, skip);
   }
   o.out[t] = over(col, d);
   if (kDebug) {
