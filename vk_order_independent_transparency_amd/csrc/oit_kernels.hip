@@ -1429,10 +1429,20 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
 // count), and the composite needs no sort. With coverage shading (kCov) the
 // stored layers carry their sample masks, each invocation's tail output goes
 // to the samples of its own mask, and the front K are blended per sample.
+//
+// The first walk keeps the newest kStack nodes of each list (index, depth
+// and colour, mask) in LDS, one column per thread, and reverses only the
+// older ones in place; the replay takes the reversed part first, then the
+// LDS entries newest last. A list of up to kStack nodes is thus read from
+// memory once, the deeper part of a longer one twice.
 template <int K, bool kCov>
 __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll, ResolveOut o) {
+  constexpr int kStack = 8;
   __shared__ float s_dec[256];
   __shared__ float s_alpha[256];
+  __shared__ uint32_t s_id[kStack][kBlock];
+  __shared__ uint64_t s_cd[kStack][kBlock];
+  __shared__ uint32_t s_mk[kCov ? kStack : 1][kBlock];
   stage_decode(s_dec, s_alpha, c.tab);
   if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
     ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
@@ -1441,15 +1451,28 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
   __syncthreads();
   const uint32_t stride = gridDim.x * kBlock;
   for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+    const uint32_t tx = threadIdx.x;
     uint32_t p = ll.head[t];
     if (p != 0u) ll.head[t] = 0u;
     uint32_t prev = 0;
-    while (p != 0u && p < ll.cap) {  // reverse: oldest first
+    uint32_t depth = 0;
+    while (p != 0u && p < ll.cap) {
       const uint32_t nx = ll.next[p];
-      ll.next[p] = prev;
-      prev = p;
+      if (depth < (uint32_t)kStack) {  // newest nodes: to LDS, link released now
+        const uint64_t cdv = ll.cd[p];
+        const uint32_t mv = kCov ? ll.mask[p] : 0u;
+        ll.next[p] = kUnset;
+        s_id[depth][tx] = p;
+        s_cd[depth][tx] = cdv;
+        if constexpr (kCov) s_mk[depth][tx] = mv;
+      } else {  // older nodes: reversed in place, oldest first
+        ll.next[p] = prev;
+        prev = p;
+      }
+      ++depth;
       p = nx;
     }
+    uint32_t rem = depth < (uint32_t)kStack ? depth : (uint32_t)kStack;  // LDS entries left
     FrontKM<K> a;
     a.init();
     uint32_t count = 0;
@@ -1462,15 +1485,29 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
       cbs[s] = (kCov && c.opq_rgba && (uint32_t)s < c.cov) ? c.opq_rgba[(uint64_t)s * c.T + t] : c.background;
     bool seen = false;
     p = prev;
-    while (p != 0u && p < ll.cap) {
+    for (;;) {  // one node per trip, oldest first
+      uint32_t node;
+      uint64_t cd;
+      uint32_t mm;
+      if (p != 0u) {  // the reversed part (only nodes below the capacity were linked)
+        node = p;
+        const uint32_t nx = ll.next[p];
+        ll.next[p] = kUnset;
+        cd = ll.cd[p];
+        mm = kCov ? ll.mask[p] : 0xFFFFFFFFu;
+        p = nx;
+      } else if (rem > 0u) {
+        --rem;
+        node = s_id[rem][tx];
+        cd = s_cd[rem][tx];
+        mm = kCov ? s_mk[rem][tx] : 0xFFFFFFFFu;
+      } else {
+        break;
+      }
       const int w = (int)__builtin_amdgcn_readfirstlane(trip + 1u);
       ++trip;
-      const uint32_t nx = ll.next[p];
-      ll.next[p] = kUnset;
-      const uint64_t cd = ll.cd[p];
       const uint32_t key = (uint32_t)(cd >> 32);
-      const uint64_t me = ((uint64_t)key << 32) | (p - 1u);
-      const uint32_t mm = kCov ? ll.mask[p] : 0xFFFFFFFFu;
+      const uint64_t me = ((uint64_t)key << 32) | (node - 1u);
       bool tail = true;
       bool own = true;
       uint32_t tcol = 0u;
@@ -1488,7 +1525,7 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
         ++count;
       }
       if (tail && c.tail) {  // :106-108 the invocation's output, PREMULTIPLIED blend in primitive order
-        const float4 tc = own ? ll.full[p] : decode_premult_lds(tcol, s_dec, s_alpha);
+        const float4 tc = own ? ll.full[node] : decode_premult_lds(tcol, s_dec, s_alpha);
         if constexpr (kCov) {
 #pragma unroll
           for (int s = 0; s < 8; ++s)
@@ -1498,7 +1535,6 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
           cb = over(tc, cb);
         }
       }
-      p = nx;
     }
     // composite (:176-230): the stored layers are already sorted
     const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
