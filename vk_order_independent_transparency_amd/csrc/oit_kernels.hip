@@ -1430,9 +1430,10 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
 // stored layers carry their sample masks, each invocation's tail output goes
 // to the samples of its own mask, and the front K are blended per sample.
 //
-// The first walk keeps the newest kStack nodes of each list (index, depth
-// and colour, mask) in LDS, one column per thread, and reverses only the
-// older ones in place; the replay takes the reversed part first, then the
+// The first walk (two consecutive nodes per round trip where the list has
+// them) keeps the newest kStack nodes of each list (index, depth and
+// colour, mask) in LDS, one column per thread, and reverses only the older
+// ones in place; the replay takes the reversed part first, then the
 // LDS entries newest last. A list of up to kStack nodes is thus read from
 // memory once, the deeper part of a longer one twice.
 template <int K, bool kCov>
@@ -1456,21 +1457,35 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
     if (p != 0u) ll.head[t] = 0u;
     uint32_t prev = 0;
     uint32_t depth = 0;
-    while (p != 0u && p < ll.cap) {
-      const uint32_t nx = ll.next[p];
+    auto take = [&](uint32_t node, uint64_t cdv, uint32_t mv) {
       if (depth < (uint32_t)kStack) {  // newest nodes: to LDS, link released now
-        const uint64_t cdv = ll.cd[p];
-        const uint32_t mv = kCov ? ll.mask[p] : 0u;
-        ll.next[p] = kUnset;
-        s_id[depth][tx] = p;
+        ll.next[node] = kUnset;
+        s_id[depth][tx] = node;
         s_cd[depth][tx] = cdv;
         if constexpr (kCov) s_mk[depth][tx] = mv;
       } else {  // older nodes: reversed in place, oldest first
-        ll.next[p] = prev;
-        prev = p;
+        ll.next[node] = prev;
+        prev = node;
       }
       ++depth;
-      p = nx;
+    };
+    // node p-1 is loaded with p: a sphere's front and back fragment are
+    // consecutive nodes chained p -> p-1, so one round trip takes both
+    while (p != 0u && p < ll.cap) {
+      const uint32_t q = p - 1u;
+      const uint32_t nx0 = ll.next[p];
+      const uint32_t nx1 = ll.next[q];
+      const uint64_t cd0 = ll.cd[p];
+      const uint64_t cd1 = ll.cd[q];
+      const uint32_t m0 = kCov ? ll.mask[p] : 0u;
+      const uint32_t m1 = kCov ? ll.mask[q] : 0u;
+      take(p, cd0, m0);
+      if (nx0 == q && q != 0u) {
+        take(q, cd1, m1);
+        p = nx1;
+      } else {
+        p = nx0;
+      }
     }
     uint32_t rem = depth < (uint32_t)kStack ? depth : (uint32_t)kStack;  // LDS entries left
     FrontKM<K> a;
