@@ -1499,28 +1499,8 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
     for (int s = 0; s < (kCov ? 8 : 1); ++s)
       cbs[s] = (kCov && c.opq_rgba && (uint32_t)s < c.cov) ? c.opq_rgba[(uint64_t)s * c.T + t] : c.background;
     bool seen = false;
-    p = prev;
-    for (;;) {  // one node per trip, oldest first
-      uint32_t node;
-      uint64_t cd;
-      uint32_t mm;
-      if (p != 0u) {  // the reversed part (only nodes below the capacity were linked)
-        node = p;
-        const uint32_t nx = ll.next[p];
-        ll.next[p] = kUnset;
-        cd = ll.cd[p];
-        mm = kCov ? ll.mask[p] : 0xFFFFFFFFu;
-        p = nx;
-      } else if (rem > 0u) {
-        --rem;
-        node = s_id[rem][tx];
-        cd = s_cd[rem][tx];
-        mm = kCov ? s_mk[rem][tx] : 0xFFFFFFFFu;
-      } else {
-        break;
-      }
-      const int w = (int)__builtin_amdgcn_readfirstlane(trip + 1u);
-      ++trip;
+    // one invocation's critical section (:62-108) at k-buffer bound w
+    auto replay = [&](uint32_t node, uint64_t cd, uint32_t mm, int w) {
       const uint32_t key = (uint32_t)(cd >> 32);
       const uint64_t me = ((uint64_t)key << 32) | (node - 1u);
       bool tail = true;
@@ -1549,6 +1529,50 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
         } else {
           cb = over(tc, cb);
         }
+      }
+    };
+    // K <= 8: a trip of the reversed part takes two chained nodes when it
+    // finds them (at K = 16 and 32 the second inlined insert costs more
+    // registers, and so occupancy, than the round trips it saves)
+    constexpr bool kTwo = K <= 8;
+    p = prev;
+    for (;;) {  // oldest first
+      // a trip replays at most two nodes (one without kTwo), which bounds
+      // the occupied slots on every active lane
+      const int w = (int)__builtin_amdgcn_readfirstlane(kTwo ? 2u * trip + 2u : trip + 1u);
+      ++trip;
+      if (p != 0u && !kTwo) {  // the reversed part (only nodes below the capacity were linked)
+        const uint32_t nx = ll.next[p];
+        ll.next[p] = kUnset;
+        const uint64_t cd0 = ll.cd[p];
+        const uint32_t m0 = kCov ? ll.mask[p] : 0xFFFFFFFFu;
+        replay(p, cd0, m0, w);
+        p = nx;
+      } else if (p != 0u) {
+        // a pair chained p -> p-1 by the append is now p-1 -> p: node p+1
+        // is loaded with p and taken along when it follows
+        const uint32_t r = p + 1u;
+        const bool rok = r < ll.cap;
+        const uint32_t nx0 = ll.next[p];
+        const uint32_t nx1 = rok ? ll.next[r] : 0u;
+        const uint64_t cd0 = ll.cd[p];
+        const uint64_t cd1 = rok ? ll.cd[r] : 0ull;
+        const uint32_t m0 = kCov ? ll.mask[p] : 0xFFFFFFFFu;
+        const uint32_t m1 = (kCov && rok) ? ll.mask[r] : 0xFFFFFFFFu;
+        ll.next[p] = kUnset;
+        replay(p, cd0, m0, w);
+        if (rok && nx0 == r) {
+          ll.next[r] = kUnset;
+          replay(r, cd1, m1, w);
+          p = nx1;
+        } else {
+          p = nx0;
+        }
+      } else if (rem > 0u) {
+        --rem;
+        replay(s_id[rem][tx], s_cd[rem][tx], kCov ? s_mk[rem][tx] : 0xFFFFFFFFu, w);
+      } else {
+        break;
       }
     }
     // composite (:176-230): the stored layers are already sorted
