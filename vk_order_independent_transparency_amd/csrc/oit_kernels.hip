@@ -1431,14 +1431,16 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
 // to the samples of its own mask, and the front K are blended per sample.
 //
 // The first walk (two consecutive nodes per round trip where the list has
-// them) keeps the newest kStack nodes of each list (index, depth and
+// them) keeps the newest kStack (6 or 8) nodes of each list (index, depth and
 // colour, mask) in LDS, one column per thread, and reverses only the older
 // ones in place; the replay takes the reversed part first, then the
 // LDS entries newest last. A list of up to kStack nodes is thus read from
 // memory once, the deeper part of a longer one twice.
 template <int K, bool kCov>
 __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll, ResolveOut o) {
-  constexpr int kStack = 8;
+  // K <= 8 keeps six nodes (21 KB of LDS per block: seven blocks per CU),
+  // K = 16 and 32 eight (27 KB), where registers allow fewer waves anyway
+  constexpr int kStack = K <= 8 ? 6 : 8;
   __shared__ float s_dec[256];
   __shared__ float s_alpha[256];
   __shared__ uint32_t s_id[kStack][kBlock];
