@@ -1702,8 +1702,14 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
   stage_thresholds(s_thr, c.tab);
   const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
   bool valid = i < f.count;
+  // K <= 8: the colour is loaded with the target and depth (one round
+  // trip), since most waves hold a fragment past the K-th slot and would
+  // otherwise load it a round trip later; at K = 16 and 32 few do, and
+  // the colour is read only after the slot test
+  constexpr bool kEarlyColour = K <= 8;
   uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
-  const float dv = valid ? ld_stream(&f.depth[i]) : 0.f;  // with the target: one round trip
+  const float dv = valid ? ld_stream(&f.depth[i]) : 0.f;
+  const float4 cv0 = (kEarlyColour && valid) ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
   if (t >= c.T) valid = false;
   bool tail = false;
   uint32_t flag = 0;
@@ -1712,11 +1718,11 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
   if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) valid = false;
   if (valid) {
     // the key's high half is the depth: a fragment nearer than the K-th
-    // slot's depth is kept without its colour being read (every fragment
-    // of a pixel with fewer than K, where that slot is empty)
+    // slot's depth is kept without its colour being encoded (every
+    // fragment of a pixel with fewer than K, where that slot is empty)
     const uint64_t s = slots[(uint64_t)(K - 1) * c.T + t];
     if (key >= (uint32_t)(s >> 32)) {
-      const uint32_t col = encode_color_lds(ld_stream(&f.rgba[i]), s_thr);
+      const uint32_t col = encode_color_lds(kEarlyColour ? cv0 : ld_stream(&f.rgba[i]), s_thr);
       const uint64_t z = ((uint64_t)key << 32) | col;
       if (z >= s) {
         tail = true;
