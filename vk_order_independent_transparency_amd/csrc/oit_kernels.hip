@@ -1437,7 +1437,7 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
 // LDS entries newest last. A list of up to kStack nodes is thus read from
 // memory once, the deeper part of a longer one twice.
 template <int K, bool kCov>
-__global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll, ResolveOut o) {
+__global__ __launch_bounds__(kBlock, (K == 16 && !kCov) ? 6 : 1) void k_interlock_resolve(Common c, LLBuf ll, ResolveOut o) {
   // K <= 8 keeps six nodes (21 KB of LDS per block: seven blocks per CU),
   // K = 16 and 32 eight (27 KB), where registers allow fewer waves anyway
   constexpr int kStack = K <= 8 ? 6 : 8;
@@ -1445,7 +1445,10 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
   __shared__ float s_alpha[256];
   __shared__ uint32_t s_id[kStack][kBlock];
   __shared__ uint64_t s_cd[kStack][kBlock];
-  __shared__ uint32_t s_mk[kCov ? kStack : 1][kBlock];
+  // masks only with coverage: without, K = 16 fits six blocks per CU
+  // (26 KB each) and the launch bound holds it to the 80 VGPRs of six
+  // waves per SIMD
+  __shared__ uint32_t s_mk[kCov ? kStack : 1][kCov ? kBlock : 1];
   stage_decode(s_dec, s_alpha, c.tab);
   if (blockIdx.x == 0 && threadIdx.x < 16u) {  // frame counters -> last-frame slot, zero for the next frame
     ll.counters[16 + threadIdx.x] = ll.counters[threadIdx.x];
@@ -1572,7 +1575,9 @@ __global__ __launch_bounds__(kBlock) void k_interlock_resolve(Common c, LLBuf ll
         }
       } else if (rem > 0u) {
         --rem;
-        replay(s_id[rem][tx], s_cd[rem][tx], kCov ? s_mk[rem][tx] : 0xFFFFFFFFu, w);
+        uint32_t mm = 0xFFFFFFFFu;
+        if constexpr (kCov) mm = s_mk[rem][tx];
+        replay(s_id[rem][tx], s_cd[rem][tx], mm, w);
       } else {
         break;
       }
