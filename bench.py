@@ -33,6 +33,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def host_cpu():
+    """The host's CPU model (first /proc/cpuinfo model name), the logical
+    CPUs the OS reports (nproc) and the CPUs this process may run on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity": len(os.sched_getaffinity(0))}
+
+
 def cpu_baseline(tg, d, c, K, cap, budget_s):
     """The oracle (single-threaded C restatement) on the same frame, repeated
     until about `budget_s` seconds of CPU work; median frame time."""
@@ -54,6 +69,7 @@ def cpu_baseline(tg, d, c, K, cap, budget_s):
         "value": len(tg) / med / 1e9,
         "unit": "G fragments/s",
         "cores": 1,
+        **host_cpu(),
         "kind": "port",
         "sample": f"{len(times)} full frames of the same workload (1920x1080, 1000 spheres, LL K={K}), "
                   f"median {med * 1e3:.1f} ms/frame, oracle/liboit_oracle.so single thread",
@@ -119,6 +135,7 @@ def cpu_baseline_all_cores(tg, d, c, K, budget_s, threads):
         "value": len(tg) / med / 1e9,
         "unit": "G fragments/s",
         "cores": threads,
+        **host_cpu(),
         "kind": "port",
         "sample": f"{len(times)} full frames of the same workload (1920x1080, 1000 spheres, LL K={K}), "
                   f"median {med * 1e3:.1f} ms/frame, fragments bucketed into {bands} pixel bands, "
@@ -126,20 +143,26 @@ def cpu_baseline_all_cores(tg, d, c, K, budget_s, threads):
     }
 
 
-def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream):
+ALGORITHM_ROWS = ((1, "linkedlist", 8), (1, "linkedlist", 16), (3, "loop64", 8), (3, "loop64", 16),
+                  (4, "spinlock", 8), (4, "spinlock", 16), (0, "simple", 8), (0, "simple", 16),
+                  (6, "weighted", 1), (2, "loop32", 8), (2, "loop32", 16), (5, "interlock", 8),
+                  (5, "interlock", 16))
+
+
+def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream, only=None):
     """Every algorithm on the same 1920x1080 stream (Linked List pool 1.5F,
     tail blending on): device time per frame, G fragments/s and the
-    fraction of the HBM roofline by the algorithm's compulsory bytes."""
+    fraction of the HBM roofline by the algorithm's compulsory bytes.
+    `only`: a set of "name:K" strings to restrict the rows to."""
     from vk_order_independent_transparency_amd import DeviceBuffer, Event, FrameConfig, Renderer
     from vk_order_independent_transparency_amd import roofline as rl
 
     P = p.targets
     out = DeviceBuffer((P, 4), "float32")
     rows = []
-    for alg, name, K in ((1, "linkedlist", 8), (1, "linkedlist", 16), (3, "loop64", 8), (3, "loop64", 16),
-                         (4, "spinlock", 8), (4, "spinlock", 16), (0, "simple", 8), (0, "simple", 16),
-                         (6, "weighted", 1), (2, "loop32", 8), (2, "loop32", 16), (5, "interlock", 8),
-                         (5, "interlock", 16)):
+    for alg, name, K in ALGORITHM_ROWS:
+        if only and f"{name}:{K}" not in only:
+            continue
         with Renderer(0) as r:
             r.configure(FrameConfig(algorithm=alg, layers=K, tail_blend=alg != 6, width=p.width, height=p.height,
                                     ll_pool_nodes=F + F // 2 if alg == 1 else 0, max_fragments=F))
@@ -347,6 +370,9 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--variant", type=int, default=0, help="OIT_VARIANT_* bits (kernel schedule; results identical)")
     ap.add_argument("--no-algorithms", action="store_true", help="skip the other algorithms on the same stream")
+    ap.add_argument("--only-algorithms", type=str, default="",
+                    help="comma-separated name:K rows of the algorithms pass (e.g. loop64:16,spinlock:16)")
+    ap.add_argument("--algorithm-steps", type=int, default=50, help="timed frames per algorithm row")
     ap.add_argument("--no-batch", action="store_true", help="skip the 64-frame batched workload")
     ap.add_argument("--batch-frames", type=int, default=64)
     ap.add_argument("--no-serve", action="store_true", help="skip the served workload")
@@ -442,7 +468,9 @@ def main():
 
     st = r.stats(stream)
     r.close()
-    algorithms = None if args.no_algorithms else bench_algorithms(tg_d, d_d, c_d, F, p, 50, stream)
+    only = {x.strip() for x in args.only_algorithms.split(",") if x.strip()}
+    algorithms = None if args.no_algorithms else bench_algorithms(tg_d, d_d, c_d, F, p, args.algorithm_steps, stream,
+                                                                  only)
 
     batch = None if args.no_batch else bench_batch(1, args.batch_frames)
     # the same generator with only 256x256 and 512x512 frames: per-frame launch

@@ -5,7 +5,9 @@ oracle to run the whole frame.
 Loop64 and Spinlock at K = 8 and 16; wherever a pixel holds at most K
 fragments the three front-K depth-key lists agree bit for bit (and the
 Linked List's and Spinlock's fragment ids too), and on sampled pixels the
-Linked List image equals the oracle run on those pixels' own fragments.
+Linked List, Loop64 and Spinlock images (colour-pass tail included, where
+a pixel holds more than K) equal the oracle run on those pixels' own
+fragments.
 
 3840x2160 with 4 samples per pixel (sample shading), 4000 spheres: WBOIT and
 the Linked List at K = 32 over the same stream; on 4096 sampled sample
@@ -79,6 +81,14 @@ def test_1080p_front_k_agree_across_variants(oracle):
         cfg = oracle.make_config(LL, K, 1, len(tgts), 1, ll_capacity=len(sub_t) + 1)
         want = oracle.run(cfg, sub_t, sub_d, sub_c)["out"]
         assert np.array_equal(res[LL][0][tgts].view(np.uint32), want.view(np.uint32)), K
+        # Loop64 and Spinlock on the same sampled pixels, tail included (canonical order), with the
+        # pixels over K counted so that the tail is known to be engaged
+        deep = int((n[tgts] > K).sum())
+        _say("K", K, "sampled pixels holding more than K fragments:", deep)
+        assert deep > 0
+        for alg in (LOOP64, SPIN):
+            want = oracle.run(oracle.make_config(alg, K, 1, len(tgts), 1), sub_t, sub_d, sub_c)["out"]
+            assert np.array_equal(res[alg][0][tgts].view(np.uint32), want.view(np.uint32)), (K, alg)
 
 
 def test_4k_4x_wboit_and_linked_list_k32(oracle):

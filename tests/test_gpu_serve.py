@@ -57,9 +57,19 @@ def test_mix_equals_alone():
         if r.algorithm == _abi.OIT_WEIGHTED:
             assert np.allclose(got, alone[i], rtol=1e-5, atol=1e-7), i
         elif r.algorithm in (_abi.OIT_SIMPLE, _abi.OIT_LOOP):
-            # Simple keeps the first K arrivals (README.md:23) and Loop32's shared depths keep the last
-            # colour stored: only where no pixel depends on arrival order do they agree exactly
-            pass
+            # Simple keeps the first K arrivals (README.md:23): pixels with at most K fragments do
+            # not depend on arrival order. Loop32's depth slots hold the K nearest distinct depths:
+            # pixels whose fragments all have distinct depths do not depend on it either.
+            tg, d, _ = scene.generate(r.scene)
+            T = r.scene.targets
+            n = np.bincount(tg, minlength=T)
+            if r.algorithm == _abi.OIT_SIMPLE:
+                keep = n <= r.layers
+            else:
+                pairs = np.unique(np.stack([tg.astype(np.uint64), d.view(np.uint32).astype(np.uint64)]), axis=1)
+                keep = np.bincount(pairs[0].astype(np.int64), minlength=T) == n
+            assert keep.sum() > T // 2, i
+            assert np.array_equal(got[keep].view(np.uint32), alone[i][keep].view(np.uint32)), i
         else:
             assert np.array_equal(got.view(np.uint32), alone[i].view(np.uint32)), i
     for c, r in zip(done, reqs):

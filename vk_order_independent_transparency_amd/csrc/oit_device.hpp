@@ -135,20 +135,6 @@ __device__ __forceinline__ uint32_t lanes_below(uint64_t mask) {
   return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
 }
 
-// Wave-aggregated allocation: every active lane with `want` gets a unique
-// index from *counter with ONE atomicAdd per wave (ballot + mbcnt).
-// Must be reached by all active lanes of the wave.
-__device__ __forceinline__ uint32_t wave_alloc(uint32_t* counter, bool want) {
-  const uint64_t mask = __ballot(want);
-  uint32_t base = 0;
-  if (mask) {
-    const uint32_t leader = (uint32_t)__builtin_ctzll(mask);
-    if (lane_id() == leader) base = atomicAdd(counter, (uint32_t)__builtin_popcountll(mask));
-    base = __shfl(base, (int)leader);
-  }
-  return base + lanes_below(mask);
-}
-
 // --------------------------------------------------------------------------
 // Relaxed agent-scope atomics (coherent across XCDs, no L2 writeback)
 // --------------------------------------------------------------------------
