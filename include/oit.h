@@ -120,6 +120,9 @@ typedef struct oit_stats {
   uint64_t ll_overflow;       /* LL fragments that found the pool full and were tail-blended */
   uint64_t tail_fragments;    /* fragments recorded in the colour-pass tail lists */
   uint64_t appends;           /* oit_append calls this frame */
+  uint64_t device_errors;     /* bits of device-side waits that gave up since the last oit_get_stats: 1 a list
+                                 insert, 2 a Spinlock lock, 4 a list walk that met a non-descending link; the
+                                 call then returns OIT_ERR_DEVICE and the affected frames' images are undefined */
 } oit_stats;
 
 typedef struct oit_ctx oit_ctx;
@@ -165,7 +168,24 @@ int oit_resolve(oit_ctx* ctx, float* out_rgba, void* stream);
  * pointers may be NULL. For parity tests; not a reference entry point. */
 int oit_resolve_debug(oit_ctx* ctx, float* out_rgba, uint32_t* frontk_key,
                       uint32_t* frontk_val, uint32_t* counts, void* stream);
-/* Copies the frame counters to the host (synchronises `stream`). */
+/* Failure-handling tests only (no reference counterpart): writes `value`
+ * into word `index` of one of the context's device structures, stream
+ * ordered, so that a test can plant a corrupted link or a lock that is never
+ * released and check that the bounded device waits end with OIT_ERR_DEVICE
+ * (oit_get_stats) instead of hanging. `index` is checked against the
+ * structure's allocated words (OIT_ERR_INVALID_ARG beyond it). */
+enum {
+  OIT_DEBUG_LL_HEAD = 0,    /* Linked List / interlock head per target */
+  OIT_DEBUG_LL_NEXT = 1,    /* Linked List / interlock link word per node */
+  OIT_DEBUG_SPIN_LOCK = 2   /* Spinlock lock word per target (count << 1 | held) */
+};
+int oit_debug_set_word(oit_ctx* ctx, uint32_t what, uint64_t index, uint32_t value, void* stream);
+/* Copies the frame counters to the host (synchronises `stream`). Device
+ * waits are bounded: a list insert or a Spinlock section that waits past
+ * its trip limit, or a list walk that meets a link that does not descend,
+ * leaves its loop and marks the context; the next oit_get_stats fills *out,
+ * clears the mark and returns OIT_ERR_DEVICE (a corrupted structure ends the
+ * kernels with an error instead of hanging the queue). */
 int oit_get_stats(oit_ctx* ctx, oit_stats* out, void* stream);
 
 /* Display image of a resolved frame: the MSAA resolve (average of `samples`

@@ -551,6 +551,9 @@ int orc_run_cov(const orc_config* cfg, uint64_t F, const uint32_t* target, const
       }
       if (have_tail && cfg->tail_blend) {
         float c4[4];
+        /* sequential: a fragment left out blends its own shaded colour (`color` = shading(IN),
+         * :51,126-128), an evicted element its stored sRGB8 colour (:112); canonical: the packed
+         * colour for both (which of the two a fragment is depends on the arrival order) */
         if (canonical || !tail_full) orc_decode_premult(tail_e.col, c4);
         else memcpy(c4, &rgba[4 * i], sizeof c4);
         uint32_t pos = canonical ? tail_e.id : (uint32_t)i;

@@ -28,7 +28,11 @@
  *    for Loop64/Spinlock an evicted fragment blends at its own id with its
  *    packed sRGB8 colour (ORC_TAIL_CANONICAL), or at the evicting fragment's
  *    position as a sequential reading of the shader gives
- *    (ORC_TAIL_SEQUENTIAL, for reporting only);
+ *    (ORC_TAIL_SEQUENTIAL, for reporting only, where a Spinlock fragment
+ *    that is never stored blends its full-precision colour as
+ *    oitSpinlock.frag.glsl:126-128 does; whether a fragment is rejected or
+ *    stored and evicted later depends on the arrival order, so the canonical
+ *    order uses the packed colour for both);
  *  - colour buffer: binary32 RGBA, no sRGB8 round trip between blends.
  */
 #ifndef OIT_ORACLE_H

@@ -99,7 +99,7 @@ def test_batch_equals_single_frame(mixed, concurrent, graph, frame_order):
         assert sz.frames == len(MIXED) and sz.groups == 9
         for rep in range(3):  # later runs reuse the pool the resolve left clean
             for o in fr.outs:
-                o.upload(np.full(o.shape, np.nan, np.float32))
+                o.upload(np.full(o.shape, np.nan, np.float32), st)  # ordered before the run on st
             b.run(fr.frames, st)
             _assert_images(fr.images(st), want)
 

@@ -89,6 +89,21 @@ def test_1080p_front_k_agree_across_variants(oracle):
         for alg in (LOOP64, SPIN):
             want = oracle.run(oracle.make_config(alg, K, 1, len(tgts), 1), sub_t, sub_d, sub_c)["out"]
             assert np.array_equal(res[alg][0][tgts].view(np.uint32), want.view(np.uint32)), (K, alg)
+        # The whole frame: the canonical tail order bit for bit, and the error of the canonical
+        # image against the sequential reading of the shaders (each evicted colour blended at the
+        # evicting fragment's position, oitLoop64.frag.glsl:124-140, oitSpinlock.frag.glsl:109-128)
+        tg_all, d_all, c_all = tg_h, d.numpy()[:F], c.numpy()[:F]
+        for alg in (LOOP64, SPIN):
+            want = oracle.run(oracle.make_config(alg, K, 1, p.width, p.height), tg_all, d_all, c_all)["out"]
+            assert np.array_equal(res[alg][0].view(np.uint32), want.view(np.uint32)), (K, alg)
+            seq = oracle.run(oracle.make_config(alg, K, 1, p.width, p.height, tail_order=oracle.TAIL_SEQUENTIAL),
+                             tg_all, d_all, c_all)["out"]
+            err = np.abs(res[alg][0].astype(np.float64) - seq.astype(np.float64))
+            diff = (err > 0).any(axis=1)
+            _say(f"{'loop64' if alg == LOOP64 else 'spinlock'} K={K}: canonical vs sequential tail order: "
+                 f"max abs error per channel {err.max(axis=0).tolist()}, {int(diff.sum())} of {T} pixels differ, "
+                 f"{int((err > 2.0 ** -12).any(axis=1).sum())} by more than 2^-12")
+            assert not diff[small].any()  # no tail where a pixel holds at most K fragments
 
 
 def test_4k_4x_wboit_and_linked_list_k32(oracle):

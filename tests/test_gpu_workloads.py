@@ -68,7 +68,7 @@ def test_batch_64_frames_equal_single_frames_and_oracle(oracle):
     with Batch(0, concurrent=True, graph=True) as b:
         for rep in range(3):  # the first run captures the graph, the later ones replay it
             for f in frames:
-                f.out.upload(np.full((f.width * f.height, 4), np.nan, np.float32))
+                f.out.upload(np.full((f.width * f.height, 4), np.nan, np.float32), st)
             b.run(frames, st)
             st.synchronize()
             for i, f in enumerate(frames):

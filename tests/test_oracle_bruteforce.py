@@ -142,3 +142,40 @@ def test_loop32_matches_loop64_front_for_distinct_depths(oracle):
             if comp is None:
                 comp = r32["composite"]
             assert np.array_equal(r32["composite"], comp)  # stable across arrival orders
+
+
+def test_spinlock_tail_colour_depends_on_arrival_order_only_in_the_sequential_reading(oracle):
+    """Four fragments on one pixel, K = 2. With the critical sections in draw order 0,1,2,3,
+    fragment 1 is stored and then evicted by 2 (its packed colour, oitSpinlock.frag.glsl:112)
+    and 3 is rejected (its full colour, :126-128); with the second pair first (2,3,0,1, an
+    order a GPU can take), 3 is stored and evicted and 1 rejected. So the reference's colour
+    rule makes the image depend on the arrival order, while the tail SET (1 and 3) does not;
+    the canonical order gives both their packed colours, which no order changes."""
+    d = np.float32([0.134, 0.353, 0.305, 0.390])
+    a = np.float32([0.3, 0.5, 0.4, 0.6])
+    rgb = np.float32([[0.9, 0.2, 0.1], [0.1, 0.8, 0.3], [0.2, 0.3, 0.9], [0.7, 0.7, 0.2]])
+    c = np.concatenate([rgb * a[:, None], a[:, None]], axis=1).astype(np.float32)
+    tg = np.zeros(4, np.uint32)
+    bg = np.float32([0.5, 0.5, 0.5, 1.0])
+
+    def over(top, base):
+        return (top + (np.float32(1) - top[3]) * base).astype(np.float32)
+
+    def q(x):
+        return np.asarray(oracle.decode_premult(oracle.encode(x)), np.float32)
+
+    def seq(order):
+        return oracle.run(oracle.make_config(oracle.SPINLOCK, 2, 1, 1, 1, tail_order=oracle.TAIL_SEQUENTIAL),
+                          tg, d[order], c[order])
+
+    # draw order: 2 evicts 1 (packed, blended at 2's position), 3 rejects itself (full colour)
+    s0 = seq([0, 1, 2, 3])
+    assert np.array_equal(s0["colorbuf"][0], over(c[3], over(q(c[1]), bg)))
+    # sections 2,3,0,1: 0 evicts 3 (packed), 1 rejects itself (full colour)
+    s1 = seq([2, 3, 0, 1])
+    assert np.array_equal(s1["colorbuf"][0], over(c[1], over(q(c[3]), bg)))
+    assert sorted(np.nonzero(s0["tail_flag"])[0].tolist()) == [1, 3]
+    assert sorted(np.nonzero(s1["tail_flag"])[0].tolist()) == [1, 3]  # positions of fragments 3 and 1
+    # canonical: fragments 1 and 3 at their own ids, packed colours, newest on top
+    r = oracle.run(oracle.make_config(oracle.SPINLOCK, 2, 1, 1, 1), tg, d, c)
+    assert np.array_equal(r["colorbuf"][0], over(over(q(c[3]), q(c[1])), bg))

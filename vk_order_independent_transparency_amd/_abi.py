@@ -50,6 +50,7 @@ BATCH_GRAPH = 0x2
 BATCH_FRAME_ORDER = 0x4
 
 OIT_PROFILE_STAGES = 6
+DEBUG_LL_HEAD, DEBUG_LL_NEXT, DEBUG_SPIN_LOCK = 0, 1, 2
 PROFILE_STAGE_NAMES = ("clear", "store", "link", "classify", "tail_link", "resolve")
 
 
@@ -103,6 +104,7 @@ class OitStats(ctypes.Structure):
         ("ll_overflow", ctypes.c_uint64),
         ("tail_fragments", ctypes.c_uint64),
         ("appends", ctypes.c_uint64),
+        ("device_errors", ctypes.c_uint64),
     ]
 
 
@@ -216,6 +218,7 @@ SIGNATURES = {
     "oit_draw_opaque": (ctypes.c_int, [_P, ctypes.POINTER(OitFragments), _P]),
     "oit_resolve": (ctypes.c_int, [_P, _P, _P]),
     "oit_resolve_debug": (ctypes.c_int, [_P, _P, _P, _P, _P, _P]),
+    "oit_debug_set_word": (ctypes.c_int, [_P, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_uint32, _P]),
     "oit_get_stats": (ctypes.c_int, [_P, ctypes.POINTER(OitStats), _P]),
     "oit_finalize_srgb8": (ctypes.c_int, [_P, _P, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint32,
                                           _P, _P]),

@@ -67,6 +67,7 @@ void build_codec_tables(CodecTables* t) {
 namespace {
 
 constexpr uint32_t kMaxFrameFragments = 0x7FFFFFFEu;  // ids and tail links fit in 31 bits
+constexpr uint32_t kCounterWords = 32;  // frame counters [0..15], last frame's [16..31]; then the error word
 constexpr uint64_t kMaxPoolNodes = 0xFFFFFFFEull;      // node indices stay below the kUnset sentinel
 
 enum Stage { kStClear = 0, kStStore, kStLink, kStClassify, kStTailLink, kStResolve, kStCount };
@@ -317,6 +318,10 @@ oit::Common common_of(const oit_ctx* c) {
 
 bool coverage(const oit_ctx* c) { return c->cfg.coverage_samples > 1; }
 
+// The context's device error word (oit::kErr* bits, sticky until
+// oit_get_stats reads it), after the 32 frame counters.
+uint32_t* dev_err(const oit_ctx* c) { return c->d_counters + kCounterWords; }
+
 oit::TailBuf tail_of(const oit_ctx* c) {
   oit::TailBuf t;
   const bool on = c->tl_head != nullptr && c->tl_head_valid;
@@ -327,6 +332,7 @@ oit::TailBuf tail_of(const oit_ctx* c) {
   t.counter = c->d_counters;
   t.cap = c->tl_cap;
   t.mask = c->cfg.coverage_samples > 1 ? c->tl_mask : nullptr;
+  t.err = dev_err(c);
   return t;
 }
 
@@ -491,9 +497,9 @@ int oit_create(oit_ctx** out, int device) {
   oit::CodecTables host;
   oit::build_codec_tables(&host);
   int rc = dev_alloc(&c->d_tab, 1);
-  if (!rc) rc = dev_alloc(&c->d_counters, 32);
+  if (!rc) rc = dev_alloc(&c->d_counters, kCounterWords + 1);
   if (!rc && hipMemcpy(c->d_tab, &host, sizeof host, hipMemcpyHostToDevice) != hipSuccess) rc = OIT_ERR_DEVICE;
-  if (!rc && hipMemset(c->d_counters, 0, 32 * sizeof(uint32_t)) != hipSuccess) rc = OIT_ERR_DEVICE;
+  if (!rc && hipMemset(c->d_counters, 0, (kCounterWords + 1) * sizeof(uint32_t)) != hipSuccess) rc = OIT_ERR_DEVICE;
   if (!rc && hipEventCreateWithFlags(&c->work_ev, hipEventDisableTiming) != hipSuccess) rc = OIT_ERR_DEVICE;
   if (rc) {
     dev_free(&c->d_tab);
@@ -884,6 +890,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
   switch (alg) {
     case OIT_LINKEDLIST: {
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      ll.err = dev_err(c);
       ll.mask = coverage(c) ? c->ll_mask : nullptr;
       c->ll_clean = false;
       c->counters_clean = false;
@@ -903,6 +910,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       int rc = ensure_pool_nodes(c, end + 1, s);
       if (rc) return rc;
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      ll.err = dev_err(c);
       ll.mask = coverage(c) ? c->ll_mask : nullptr;
       ll.full = c->il_full;
       c->ll_clean = false;
@@ -919,12 +927,14 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
     }
     case OIT_SIMPLE: {
       oit::KBuf kb{c->kb_count, nullptr, nullptr, c->kb_col, c->kb_key, c->kb_id};
+      kb.err = dev_err(c);
       kb.mask = coverage(c) ? c->kb_mask : nullptr;
       e = staged(c, kStStore, s, [&] { return oit::launch_simple_store(k, v, kb, tb, s); });
       break;
     }
     case OIT_SPINLOCK: {
       oit::KBuf kb{nullptr, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
+      kb.err = dev_err(c);
       kb.mask = coverage(c) ? c->kb_mask : nullptr;
       // a fragment's record is written by whichever fragment pushes it to the
       // tail, so this append's records start empty
@@ -998,6 +1008,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
   switch (alg) {
     case OIT_LINKEDLIST: {
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      ll.err = dev_err(c);
       const bool pair = !(c->cfg.variant & OIT_VARIANT_LL_SINGLE_WALK);
       const bool keys = (c->cfg.variant & OIT_VARIANT_LL_KEYS_ONLY) != 0;
       if (coverage(c)) {
@@ -1034,6 +1045,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
       break;
     case OIT_INTERLOCK: {
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
+      ll.err = dev_err(c);
       ll.mask = coverage(c) ? c->ll_mask : nullptr;
       ll.full = c->il_full;
       e = staged(c, kStResolve, s, [&] { return oit::launch_interlock_resolve(k, ll, o, s); });
@@ -1077,15 +1089,47 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
   DeviceGuard g(c->device);
   hipStream_t s = static_cast<hipStream_t>(stream);
   uint32_t h[16] = {0};
+  uint32_t err = 0;
   const uint32_t* src = c->d_counters + (c->stats_moved ? 16 : 0);
   if (hipMemcpyAsync(h, src, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess) return OIT_ERR_DEVICE;
+  if (hipMemcpyAsync(&err, dev_err(c), sizeof err, hipMemcpyDeviceToHost, s) != hipSuccess) return OIT_ERR_DEVICE;
   if (hipStreamSynchronize(s) != hipSuccess) return OIT_ERR_DEVICE;
   std::memset(out, 0, sizeof *out);
   out->fragments = c->frame_frags;
   out->ll_overflow = h[1];
   out->tail_fragments = h[0];
   out->appends = c->appends;
+  out->device_errors = err;
+  if (err != 0u) {
+    // reported once: the word starts over for later frames, and the next
+    // clear rebuilds the pool (walks that gave up left links behind)
+    if (oit::launch_fill_u32(dev_err(c), 1, 0u, s) != hipSuccess || hipStreamSynchronize(s) != hipSuccess)
+      return OIT_ERR_DEVICE;
+    c->ll_clean = false;
+    c->ll_dirty_upto = c->ll_alloc;
+    c->counters_clean = false;
+    return OIT_ERR_DEVICE;
+  }
   return OIT_OK;
+}
+
+int oit_debug_set_word(oit_ctx* c, uint32_t what, uint64_t index, uint32_t value, void* stream) {
+  if (!c) return OIT_ERR_INVALID_ARG;
+  if (!c->configured) return OIT_ERR_STATE;
+  uint32_t* base = nullptr;
+  uint64_t n = 0;
+  switch (what) {
+    case OIT_DEBUG_LL_HEAD: base = c->ll_head; n = c->ll_head ? c->cap_ll_head : 0; break;
+    case OIT_DEBUG_LL_NEXT: base = c->ll_next; n = c->ll_next ? c->ll_alloc : 0; break;
+    case OIT_DEBUG_SPIN_LOCK: base = c->kb_lock; n = c->kb_lock ? c->cap_kb_lock : 0; break;
+    default: return OIT_ERR_INVALID_ARG;
+  }
+  if (!base || index >= n) return OIT_ERR_INVALID_ARG;
+  DeviceGuard g(c->device);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  const hipError_t e = oit::launch_fill_u32(base + index, 1, value, s);
+  mark_work(c, s);
+  return status_of(e);
 }
 
 int oit_codec_tables(float* thr257, float* dec256, float* alpha256) {

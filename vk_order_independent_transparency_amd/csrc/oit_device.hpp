@@ -16,6 +16,32 @@ constexpr uint32_t kEmpty = 0xFFFFFFFFu;      // empty head / slot / list end
 constexpr uint32_t kUnset = 0xFFFFFFFEu;      // link word not yet written during a list build
 constexpr uint64_t kEmpty64 = 0xFFFFFFFFFFFFFFFFull;
 
+// Device error bits (oit_stats.device_errors). Every loop in which a lane
+// waits for other lanes (a list insert meeting a link word that is not yet
+// written, the Spinlock lock) gives up after kSpinLimit trips, and every list
+// walk requires strictly descending values; a lane that gives up sets a bit
+// in the context's error word and leaves the loop, so a corrupted structure
+// ends the kernel with an error instead of hanging the queue. A correct
+// frame waits at most a few trips per lane for a link word and, for the
+// lock, at most the trips of the critical sections queued before it.
+constexpr uint32_t kErrListWait = 1u;  // a list insert exceeded kSpinLimit trips
+constexpr uint32_t kErrLockWait = 2u;  // a Spinlock section waited kSpinLimit trips for its lock
+constexpr uint32_t kErrWalk = 4u;      // a list walk met a link that does not descend
+constexpr uint32_t kSpinLimit = 1u << 20;
+
+__device__ __forceinline__ void report_error(uint32_t* err, uint32_t bit) {
+  if (err) __hip_atomic_fetch_or(err, bit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Next value of a walk along a descending list: `nx` when it lies below the
+// current value `cur`, else 0 (the end of the walk) with kErrWalk reported,
+// so a corrupted list (a cycle, a stale link) cannot keep a walker looping.
+__device__ __forceinline__ uint32_t descend(uint32_t nx, uint32_t cur, uint32_t* err) {
+  if (nx < cur) return nx;
+  report_error(err, kErrWalk);
+  return 0u;
+}
+
 // Colour codec tables, filled on the host at context creation.
 //  thr[q]  (1 <= q <= 255): smallest binary32 c whose sRGB8 code is >= q,
 //          for the curve of unPremultLinearToSRGB (shaderCommon.glsl:60-69)
@@ -191,11 +217,16 @@ __device__ __forceinline__ uint32_t min_relaxed(uint32_t* p, uint32_t v) {
 // trip of its successful atomic.
 template <int kShift = 0, uint32_t kBias = 0>
 __device__ __forceinline__ void list_insert_desc(bool want, uint32_t* p0, uint32_t* next, uint32_t lo,
-                                                 uint32_t hi, uint32_t cap) {
+                                                 uint32_t hi, uint32_t cap, uint32_t* err) {
   bool active = want && hi < cap;
   bool first = true;
   uint32_t* p = p0;
+  uint32_t trips = 0;
   while (__ballot(active)) {
+    if (active && ++trips > kSpinLimit) {
+      report_error(err, kErrListWait);
+      active = false;
+    }
     if (active) {
       bool try_max = first;
       first = false;
@@ -272,10 +303,15 @@ __device__ __forceinline__ void lock_release_count(uint32_t* l, uint32_t count) 
 // reads it before its first atomic.
 template <int kShift = 0, uint32_t kBias = 0>
 __device__ __forceinline__ void list_insert_desc_from(bool want, uint32_t* p0, uint32_t* next, uint32_t lo,
-                                                      uint32_t hi, uint32_t cap) {
+                                                      uint32_t hi, uint32_t cap, uint32_t* err) {
   bool active = want && hi < cap;
   uint32_t* p = p0;
+  uint32_t trips = 0;
   while (__ballot(active)) {
+    if (active && ++trips > kSpinLimit) {
+      report_error(err, kErrListWait);
+      active = false;
+    }
     if (active) {
       bool try_max = false;
       const uint32_t v = ld_relaxed(p);

@@ -240,7 +240,7 @@ __device__ __forceinline__ float4 tail_fold_from(const TailBuf& tb, uint32_t hea
   uint32_t keep = 0xFFFFFFFFu;  // flagged entries to blend
   if (skip > 0u) {
     uint32_t flagged = 0;
-    for (uint32_t v = head; v != 0u && v < vcap; v = tb.next[tail_index(v)]) flagged += v & 1u;
+    for (uint32_t v = head; v != 0u && v < vcap; v = descend(tb.next[tail_index(v)], v, tb.err)) flagged += v & 1u;
     keep = flagged > skip ? flagged - skip : 0u;
   }
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -252,7 +252,7 @@ __device__ __forceinline__ float4 tail_fold_from(const TailBuf& tb, uint32_t hea
       acc = over(acc, tb.rgba[id]);
       if (v & 1u) --keep;
     }
-    v = nx;
+    v = descend(nx, v, tb.err);
   }
   return over(acc, bg);
 }
@@ -305,7 +305,7 @@ __device__ __forceinline__ float4 tail_fold_cov(const TailBuf& tb, uint32_t t, c
 #pragma unroll
     for (int s = 0; s < 8; ++s)
       if ((uint32_t)s < S && ((m >> s) & 1u)) acc[s] = over(acc[s], c);
-    v = nx;
+    v = descend(nx, v, tb.err);
   }
   float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -437,7 +437,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_link(Common c, FragView f, LLBuf 
   // chain inside the run; the run start lane's link is written by the insert
   if (valid && !r.start) st_relaxed(&ll.next[n], n - 1u);
   const bool ins = valid && r.end;
-  list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n, (uint32_t)ll.cap);
+  list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n, (uint32_t)ll.cap,
+                   ll.err);
 }
 
 // Composite pass, oitLinkedList.frag.glsl:106-172: walk the list (newest
@@ -462,7 +463,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
     a.insert(x, (uint32_t)cd, ev, evc);
     if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult(evc, c.tab));
     ++cnt;
-    p = nx;
+    p = descend(nx, p, ll.err);
   }
   const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
   float4 col = blend_front<K>(a, n, c.tab);
@@ -569,7 +570,8 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     const bool walk = ins[u] && old[u] < ll.cap;
-    list_insert_desc_from(walk, walk ? &ll.next[old[u]] : nullptr, ll.next, lo[u], hi[u], (uint32_t)ll.cap);
+    list_insert_desc_from(walk, walk ? &ll.next[old[u]] : nullptr, ll.next, lo[u], hi[u], (uint32_t)ll.cap,
+                          ll.err);
   }
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
@@ -647,9 +649,9 @@ __device__ __forceinline__ float4 ll_walk_core(uint32_t t, const LLBuf& ll, cons
       a.insert_bounded((cd1 & 0xFFFFFFFF00000000ull) | q, (uint32_t)cd1, ev, evc, w);
       if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
       ++cnt;
-      p = nx1;
+      p = descend(nx1, q, ll.err);
     } else {
-      p = nx0;
+      p = descend(nx0, p, ll.err);
     }
   }
   return tailc;
@@ -705,9 +707,9 @@ __device__ __forceinline__ float4 ll_walk_reset_keys(uint32_t t, const LLBuf& ll
       if (kTail && cnt >= (uint32_t)K)
         tailc = over(tailc, decode_premult_lds((uint32_t)ll.cd[(uint32_t)ev], s_dec, s_alpha));
       ++cnt;
-      p = nx1;
+      p = descend(nx1, q, ll.err);
     } else {
-      p = nx0;
+      p = descend(nx0, p, ll.err);
     }
   }
   const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
@@ -998,7 +1000,11 @@ __device__ __forceinline__ void spin_section(const Common& c, const KBuf& kb, ui
   }
   uint32_t fur = 0;
   uint64_t mx = 0;
-#pragma unroll
+  // the scan as a loop of eight loads per trip (K/2 below 16): unrolled in
+  // full it held 100-256 VGPRs (three or four waves per SIMD at K = 8, 16),
+  // as a loop 49-63 (eight waves); only sections of a full buffer scan
+  constexpr int kScanUnroll = K >= 16 ? 8 : (K >= 2 ? K / 2 : 1);
+#pragma unroll kScanUnroll
   for (int j = 0; j < K; ++j) {
     const uint64_t idx = (uint64_t)j * c.T + t;
     const uint64_t v = ((uint64_t)ld_relaxed(&kb.key[idx]) << 32) | ld_relaxed(&kb.id[idx]);
@@ -1080,11 +1086,16 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   bool p_tail = false;
   uint32_t p_tail_id = lid + 1u, p_tail_col = p_col, p_tail_mask = p_mask;
   bool tested = false, me_pass = false, p_pass = false;
+  uint32_t trips = 0;
   // The lock loop (:75-123) is driven by a wave-wide ballot: a lane that takes
   // the lock runs the critical section and releases it in the same trip, and
   // the compiler cannot hoist the exchange into an inner spin that a lock
   // holder of the same wave would wait behind.
   while (__ballot(pending)) {
+    if (pending && ++trips > kSpinLimit) {  // a lock that is never released: give up with an error
+      report_error(kb.err, kErrLockWait);
+      pending = false;
+    }
     if (pending) {
       uint32_t bound = 0u;
       if (!tested) bound = ld_relaxed(&kb.depth[t]);  // :66-69, issued with the first attempt
@@ -1120,6 +1131,11 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
     tail_mask = b_mask;
   }
   if constexpr (kTail) {
+    // every tail element blends its packed sRGB8 colour: whether a fragment
+    // is rejected on arrival (the reference then blends its shaded colour,
+    // oitSpinlock.frag.glsl:126-128) or stored and evicted later (its packed
+    // colour, :112) depends on the arrival order, so only one colour rule
+    // for both keeps the image independent of it
     float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
     if (tail) payload = decode_premult(tail_col, c.tab);
     tail_record_evict(tail, valid ? t : kEmpty, tail_id, 0u, payload, tb, tail_mask);
@@ -1487,9 +1503,9 @@ __global__ __launch_bounds__(kBlock, (K == 16 && !kCov) ? 6 : 1) void k_interloc
       take(p, cd0, m0);
       if (nx0 == q && q != 0u) {
         take(q, cd1, m1);
-        p = nx1;
+        p = descend(nx1, q, ll.err);
       } else {
-        p = nx0;
+        p = descend(nx0, p, ll.err);
       }
     }
     uint32_t rem = depth < (uint32_t)kStack ? depth : (uint32_t)kStack;  // LDS entries left
@@ -1927,7 +1943,8 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t first
 #pragma unroll
     for (int u = 0; u < kU; ++u) {  // a larger value was already at the head: continue behind it
       const bool walk = ins[u] && old[u] < vcap;
-      list_insert_desc_from<1, 1>(walk, walk ? &tb.next[tail_index(old[u])] : nullptr, tb.next, lo[u], v[u], vcap);
+      list_insert_desc_from<1, 1>(walk, walk ? &tb.next[tail_index(old[u])] : nullptr, tb.next, lo[u], v[u], vcap,
+                                  tb.err);
     }
   }
   // tail fragments of the frame: one atomic per block

@@ -27,6 +27,7 @@ struct TailBuf {
   uint32_t* counter;  // tail fragments of the frame (counted by the link kernel)
   uint32_t cap;       // Fcap
   uint32_t* mask = nullptr;  // [Fcap] coverage shading: samples the ROP blend writes
+  uint32_t* err = nullptr;   // device error word (kErr* bits), null: none
 };
 
 // Per-algorithm A-buffer views.
@@ -39,6 +40,7 @@ struct LLBuf {
   uint32_t* counters; // frame counters [16]; the resetting resolve moves them to counters[16..31]
   uint32_t* mask = nullptr;  // [cap] coverage shading: the node's sample mask
   float4* full = nullptr;    // [cap] ordered interlock: the shaded colour (its own tail blend uses it)
+  uint32_t* err = nullptr;   // device error word (kErr* bits), null: none
 };
 
 struct KBuf {          // Simple / Spinlock
@@ -49,6 +51,7 @@ struct KBuf {          // Simple / Spinlock
   uint32_t* key;      // [K][T]
   uint32_t* id;       // [K][T]
   uint32_t* mask = nullptr;  // [K][T] coverage shading
+  uint32_t* err = nullptr;   // device error word (kErr* bits), null: none
 };
 
 struct WBBuf {
