@@ -367,6 +367,26 @@ int oit_batch_tune(oit_batch* b, const oit_batch_frame* frames, uint32_t n, void
 int oit_batch_get_tuning(const oit_batch* b, oit_batch_tuning* out);
 int oit_batch_set_tuning(oit_batch* b, const oit_batch_tuning* t);
 
+/* ---- One Linked List frame without allocation -----------------------------
+ * No counterpart in the reference. For callers that cannot allocate device
+ * memory: code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc. Renders `frame` (its fragments in draw order, frame-local
+ * targets, no coverage masks) exactly as oit_clear + oit_append + oit_resolve
+ * of a context configured for OIT_LINKEDLIST with the frame's layers,
+ * tail_blend, background and ll_pool_nodes = count + 1 (so no fragment
+ * overflows): the same image bit for bit. The heads and node pool live in a
+ * library-owned device array of OIT_NOALLOC_SCRATCH_BYTES that each GPU the
+ * library runs on holds until the process ends; a frame needs about
+ * 4 * targets + 12 * (count + 1) bytes of it (plus 4 KiB) and a larger one
+ * returns OIT_ERR_CAPACITY. The call issues no hipMalloc/hipFree/hipMemcpy/
+ * hipMemset, only kernel launches on `stream` (the first call on a device
+ * looks up the array's address). Every call on one device shares its array:
+ * calls that overlap on one device race (serialise them, e.g. on one stream).
+ * `frame->out` must be 16-byte aligned; `frame->samples` planes of
+ * width * height targets as in oit_config. */
+#define OIT_NOALLOC_SCRATCH_BYTES (256ull << 20)
+int oit_ll_render_noalloc(int device, const oit_batch_frame* frame, void* stream);
+
 /* ---- Request loop ---------------------------------------------------------
  * No counterpart in the reference (one interactive frame loop,
  * Sample::onRender, oitRender.cpp:28-154). Requests (a seeded scene, an

@@ -22,5 +22,5 @@ from ._abi import (  # noqa: F401
 from .renderer import FrameConfig, Renderer, State  # noqa: F401
 from . import scene  # noqa: F401
 from .device import DeviceBuffer, Event, Stream, device_count  # noqa: F401
-from .batch import Batch, BatchFrame  # noqa: F401
+from .batch import Batch, BatchFrame, render_noalloc  # noqa: F401
 from .serve import Request, Server  # noqa: F401

@@ -51,6 +51,7 @@ BATCH_FRAME_ORDER = 0x4
 
 OIT_PROFILE_STAGES = 6
 DEBUG_LL_HEAD, DEBUG_LL_NEXT, DEBUG_SPIN_LOCK = 0, 1, 2
+NOALLOC_SCRATCH_BYTES = 256 << 20
 PROFILE_STAGE_NAMES = ("clear", "store", "link", "classify", "tail_link", "resolve")
 
 
@@ -258,6 +259,7 @@ SIGNATURES = {
     "oit_batch_tune": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchFrame), ctypes.c_uint32, _P]),
     "oit_batch_get_tuning": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchTuning)]),
     "oit_batch_set_tuning": (ctypes.c_int, [_P, ctypes.POINTER(OitBatchTuning)]),
+    "oit_ll_render_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(OitBatchFrame), _P]),
     "oit_server_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(OitServerConfig)]),
     "oit_server_destroy": (ctypes.c_int, [_P]),
     "oit_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(OitRequest)]),

@@ -114,3 +114,12 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def render_noalloc(frame: BatchFrame, stream=None, device: int = 0) -> None:
+    """One Linked List frame through oit_ll_render_noalloc: the image of
+    oit_clear + oit_append + oit_resolve with a pool of count + 1 nodes, with
+    the heads and nodes in the library's own device array (no allocation).
+    Raises OitError (OIT_ERR_CAPACITY) for a frame the array cannot hold."""
+    f = frame.to_c()
+    _abi.check("oit_ll_render_noalloc", _abi.lib().oit_ll_render_noalloc(device, ctypes.byref(f), stream_ptr(stream)))
