@@ -53,6 +53,8 @@ enum oit_algorithm {
 #define OIT_VARIANT_LL_SINGLE_WALK   0x2u /* Linked List: resolve walks one node per round trip */
 #define OIT_VARIANT_LL_APPEND_8      0x4u /* Linked List: fused append takes 8 chunks per wave (default 4) */
 #define OIT_VARIANT_LL_KEYS_ONLY     0x8u /* Linked List: resolve keeps sort keys only and reloads colours by node */
+#define OIT_VARIANT_STORE_CHUNKS_1   0x10u /* Loop64: store one fragment per lane (default: two 64-fragment chunks per wave) */
+#define OIT_VARIANT_STORE_CHUNKS_4   0x20u /* Loop64: store four 64-fragment chunks per wave */
 
 /* Status codes. */
 enum oit_status {
@@ -403,7 +405,7 @@ int oit_ll_render_noalloc(int device, const oit_batch_frame* frame, void* stream
 typedef struct oit_server_config {
   uint32_t slots;             /* requests in flight, 1..16 */
   uint32_t reserved;          /* 0 */
-  uint64_t budget_bytes;      /* device bytes for A-buffers, pools, aux and tail arrays */
+  uint64_t budget_bytes;      /* device bytes for A-buffers, pools, aux and tail arrays, and the slots' fragment-stream and output buffers */
 } oit_server_config;
 
 typedef struct oit_request {
@@ -426,6 +428,8 @@ typedef struct oit_completion {
   double   t_submit_us;       /* steady clock, microseconds since the server was created */
   double   t_start_us;        /* memory granted, kernels enqueued */
   double   t_done_us;         /* completion seen by the dispatcher */
+  double   device_us;         /* device time of the request's work (HIP events: the count, then generation
+                                 and the OIT passes; host gaps between them excluded); 0 on error */
 } oit_completion;
 
 typedef struct oit_server_stats {
@@ -442,13 +446,15 @@ int oit_server_create(oit_server** out, int device, const oit_server_config* cfg
 int oit_server_destroy(oit_server* s);
 /* Queues a request (stamps its arrival time); returns at once. */
 int oit_server_submit(oit_server* s, const oit_request* r);
-/* Moves up to `max` completions into out[], *n of them. */
+/* Moves up to `max` completions of oit_server_submit requests into out[],
+ * *n of them (oit_server_replay collects its own requests' completions). */
 int oit_server_poll(oit_server* s, oit_completion* out, uint32_t max, uint32_t* n);
 /* Waits until every submitted request has completed (OIT_ERR_NOT_READY on timeout). */
 int oit_server_drain(oit_server* s, uint32_t timeout_ms);
 /* Load driver: submits reqs[i] at arrival_us[i] (non-decreasing, relative to
  * the call) from the calling thread, waits for all, and writes out[i] for
- * reqs[i]. */
+ * reqs[i] (with the caller's tag). Its requests are marked with the call, so
+ * completions of other callers' requests are never taken for them. */
 int oit_server_replay(oit_server* s, const oit_request* reqs, const double* arrival_us, uint32_t n,
                       uint32_t timeout_ms, oit_completion* out);
 int oit_server_get_stats(oit_server* s, oit_server_stats* out);

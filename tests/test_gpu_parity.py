@@ -134,10 +134,11 @@ def test_linkedlist_pool_overflow_bitexact(oracle, cap_div):
 
 @pytest.mark.parametrize("K", [1, 4, 8, 16, 32])
 @pytest.mark.parametrize("tail", [0, 1])
-def test_loop64_bitexact(oracle, K, tail):
+@pytest.mark.parametrize("variant", [0, 0x10, 0x20])  # store: 2 chunks per wave, 1 fragment per lane, 4 chunks
+def test_loop64_bitexact(oracle, K, tail, variant):
     p, tg, d, c = deep_scene(seed=7, spheres=160 if K <= 16 else 400)
-    _say("Loop64", K, tail, len(tg))
-    g = run_gpu(LOOP64, K, tail, p.width, p.height, tg, d, c)
+    _say("Loop64", K, tail, variant, len(tg))
+    g = run_gpu(LOOP64, K, tail, p.width, p.height, tg, d, c, variant=variant)
     o = oracle.run(_oracle_cfg(oracle, oracle.LOOP64, K, tail, p), tg, d, c)
     _assert_same("count", g["count"], o["count"])
     _assert_same("frontk_key", g["frontk_key"], o["frontk_key"])

@@ -109,6 +109,7 @@ def test_budget_is_respected():
             need.append(4 * T + 28 * (F + 1))
         else:
             need.append(20 * T)
+        need[-1] += (24 * F + 16 * T) * 5 // 4  # the slot's stream and output buffers (grown by a quarter)
     budget = int(sorted(need)[-1] * 1.2)  # room for the largest request, not for all at once
     big = serve.Request(scene.SceneParams(seed=1, width=2048, height=2048, spheres=300, radius=(0.01, 0.05),
                                           alpha=(0.1, 0.5)), _abi.OIT_LOOP64, 32)
@@ -130,3 +131,21 @@ def test_bad_requests_are_rejected():
         with pytest.raises(_abi.OitError) as e:
             s.submit(serve.Request(r.scene, 9, 8))  # no such algorithm
         assert e.value.status == _abi.OIT_ERR_INVALID_ARG
+
+
+def test_replay_keeps_other_callers_completions_and_reports_device_time():
+    """A completion of a request submitted with oit_server_submit (tag 0, not yet polled) is not
+    taken by a replay call, whose own requests carry tags 0.. as well; every completion carries
+    its device time, which lies within its arrival-to-done interval."""
+    reqs = _requests(n=3, seed=11)
+    with serve.Server(0, slots=2) as s:
+        s.submit(reqs[0], tag=0)
+        s.drain()
+        done = s.replay(reqs[1:], np.zeros(2))
+        assert [c.tag for c in done] == [0, 1]
+        assert [c.fragments for c in done] == [scene.count(r.scene) for r in reqs[1:]]
+        (mine,) = s.poll()
+        assert mine.tag == 0 and mine.fragments == scene.count(reqs[0].scene)
+        for c in done + [mine]:
+            assert c.status == 0
+            assert 0 < c.device_us <= c.t_done_us - c.t_submit_us

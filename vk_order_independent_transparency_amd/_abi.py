@@ -44,6 +44,8 @@ VARIANT_LL_SPLIT_APPEND = 0x1
 VARIANT_LL_SINGLE_WALK = 0x2
 VARIANT_LL_APPEND_8 = 0x4
 VARIANT_LL_KEYS_ONLY = 0x8
+VARIANT_STORE_CHUNKS_1 = 0x10
+VARIANT_STORE_CHUNKS_4 = 0x20
 
 BATCH_CONCURRENT = 0x1
 BATCH_GRAPH = 0x2
@@ -191,6 +193,7 @@ class OitCompletion(ctypes.Structure):
         ("t_submit_us", ctypes.c_double),
         ("t_start_us", ctypes.c_double),
         ("t_done_us", ctypes.c_double),
+        ("device_us", ctypes.c_double),
     ]
 
 

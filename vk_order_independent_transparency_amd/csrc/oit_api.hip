@@ -944,7 +944,9 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       break;
     }
     case OIT_LOOP64: {
-      e = staged(c, kStStore, s, [&] { return oit::launch_loop64_store(k, v, c->l64, s); });
+      const int chunks = (c->cfg.variant & OIT_VARIANT_STORE_CHUNKS_1) ? 1
+                         : (c->cfg.variant & OIT_VARIANT_STORE_CHUNKS_4) ? 4 : 2;
+      e = staged(c, kStStore, s, [&] { return oit::launch_loop64_store(k, v, c->l64, chunks, s); });
       if (c->cfg.tail_blend) c->segments.push_back(v);
       break;
     }

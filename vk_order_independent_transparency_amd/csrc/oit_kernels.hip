@@ -1712,6 +1712,120 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, u
   }
 }
 
+// k_loop64_store with kU chunks of 64 consecutive fragments per wave: the
+// chunks' loads, probes and atomic chains are in flight together, so a
+// wave waits for one dependent round trip per chain step of all its chains
+// instead of one per step of each. Same pairs, probes and chain rule per
+// chunk as k_loop64_store, so the slots end the same.
+template <int K, int kU>
+__global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f, uint64_t* __restrict__ slots) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, c.tab);
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t first = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
+  uint32_t t[kU];
+  float dv[kU];
+  float4 cv[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {  // every chunk's fields in one round trip
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    const bool in = i < f.count;
+    t[u] = in ? ld_stream(&f.target[i]) : kEmpty;
+    dv[u] = in ? ld_stream(&f.depth[i]) : 0.f;
+    cv[u] = in ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+  uint64_t a[kU], b[kU];
+  bool hasb[kU], act[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    uint32_t tt = t[u] < c.T ? t[u] : kEmpty;
+    uint64_t z = kEmpty64;
+    if (tt != kEmpty) {
+      const uint32_t key = depth_key(dv[u]);
+      if (opaque_visible(c.opq_z, c.T, 1u, tt, key, 1u)) z = ((uint64_t)key << 32) | encode_color_lds(cv[u], s_thr);
+      else tt = kEmpty;
+    }
+    const Run r = same_target_run(tt);
+    const uint32_t zn_lo = __shfl_down((uint32_t)z, 1u);
+    const uint32_t zn_hi = __shfl_down((uint32_t)(z >> 32), 1u);
+    const uint64_t zn = ((uint64_t)zn_hi << 32) | zn_lo;
+    const bool lead = tt != kEmpty && ((r.lane - r.start_lane) & 1u) == 0u;
+    hasb[u] = lead && r.lane < r.end_lane;
+    a[u] = z;
+    b[u] = 0;
+    if (hasb[u]) {
+      a[u] = z < zn ? z : zn;
+      b[u] = z < zn ? zn : z;
+    }
+    act[u] = lead;
+    t[u] = tt;
+  }
+  // the pretests and the first slots of every chunk in one round trip
+  constexpr int kProbe = K / 2 < 4 ? K / 2 : 4;
+  int j[kU];
+  {
+    uint64_t probe[kU][kProbe > 0 ? kProbe : 1], s_last[kU], s_mid[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (act[u]) {
+#pragma unroll
+        for (int k = 0; k < kProbe; ++k) probe[u][k] = slots[(uint64_t)k * c.T + t[u]];
+        s_last[u] = slots[(uint64_t)(K - 1) * c.T + t[u]];
+        s_mid[u] = slots[(uint64_t)(K / 2) * c.T + t[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      j[u] = 0;
+      if (act[u]) {
+        if (a[u] > s_last[u]) {
+          act[u] = false;
+        } else if (a[u] > s_mid[u]) {
+          j[u] = K / 2;
+        } else {
+#pragma unroll
+          for (int k = 0; k < kProbe; ++k) j[u] += probe[u][k] < a[u] ? 1 : 0;
+        }
+      }
+    }
+  }
+  // the chains, one step of every live chain per round trip
+  for (;;) {
+    uint64_t old[kU];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (act[u]) {
+        old[u] = __hip_atomic_fetch_min(&slots[(uint64_t)j[u] * c.T + t[u]], a[u], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        any = true;
+      }
+    }
+    if (!any) break;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (!act[u]) continue;
+      if (old[u] == kEmpty64) {  // placed in an empty slot: b (if any) goes to the next one
+        if (hasb[u]) {
+          a[u] = b[u];
+          hasb[u] = false;
+        } else {
+          act[u] = false;
+        }
+      } else {
+        const uint64_t up = old[u] > a[u] ? old[u] : a[u];  // the key that leaves slot j
+        if (hasb[u] && b[u] < up) {
+          a[u] = b[u];
+          b[u] = up;
+        } else {
+          a[u] = up;
+        }
+      }
+      if (++j[u] >= K) act[u] = false;
+    }
+  }
+}
+
 // Tail classification: a fragment whose key is above the final K-th key was
 // evicted or rejected (the colours oitLoop64.frag.glsl:131-136 tail-blends);
 // one equal to it is "boundary": identical (depth, colour) keys are the same
@@ -2129,8 +2243,16 @@ struct SpinStore {
 
 template <int K>
 struct Loop64Store {
-  static hipError_t run(const Common& c, const FragView& f, uint64_t* slots, hipStream_t s) {
-    hipLaunchKernelGGL((k_loop64_store<K>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, slots);
+  static hipError_t run(const Common& c, const FragView& f, uint64_t* slots, int chunks, hipStream_t s) {
+    if (chunks <= 1) {
+      hipLaunchKernelGGL((k_loop64_store<K>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, slots);
+    } else {
+      const int u = chunks >= 4 ? 4 : 2;
+      const uint64_t per_block = (uint64_t)kBlock * u;
+      const dim3 g((uint32_t)((f.count + per_block - 1) / per_block)), b(kBlock);
+      if (u == 4) hipLaunchKernelGGL((k_loop64_store_m<K, 4>), g, b, 0, s, c, f, slots);
+      else hipLaunchKernelGGL((k_loop64_store_m<K, 2>), g, b, 0, s, c, f, slots);
+    }
     return hipGetLastError();
   }
 };
@@ -2274,9 +2396,9 @@ hipError_t launch_kbuf_resolve(const Common& c, const KBuf& kb, const TailBuf& t
   return dispatch_k<KBufResolve>(c.K, c, kb, tb, o, s);
 }
 
-hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, hipStream_t s) {
+hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, int chunks, hipStream_t s) {
   if (f.count == 0) return hipSuccess;
-  return dispatch_k<Loop64Store>(c.K, c, f, slots, s);
+  return dispatch_k<Loop64Store>(c.K, c, f, slots, chunks, s);
 }
 
 hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb,

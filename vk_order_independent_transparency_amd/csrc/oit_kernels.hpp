@@ -135,7 +135,9 @@ hipError_t launch_simple_store(const Common& c, const FragView& f, const KBuf& k
 hipError_t launch_spin_store(const Common& c, const FragView& f, const KBuf& kb, const TailBuf& tb, hipStream_t s);
 hipError_t launch_kbuf_resolve(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o, hipStream_t s);
 
-hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, hipStream_t s);
+// chunks: 64-fragment chunks per wave (1: one fragment per lane; 2 or 4:
+// every chunk's loads and chain steps in flight together)
+hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, int chunks, hipStream_t s);
 hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb,
                                   hipStream_t s);
 hipError_t launch_loop64_resolve(const Common& c, const uint64_t* slots, const TailBuf& tb, const ResolveOut& o,

@@ -38,7 +38,6 @@
 namespace {
 
 using Clock = std::chrono::steady_clock;
-constexpr int kAlgs = 7;
 constexpr uint32_t kMaxSlots = 16;
 
 int st(hipError_t e) {
@@ -74,23 +73,34 @@ double micros(Clock::time_point t, Clock::time_point t0) {
   return std::chrono::duration<double, std::micro>(t - t0).count();
 }
 
+// A slot's stream buffers grow in stream order on the slot's own stream
+// (hipFreeAsync / hipMallocAsync): only that stream's earlier work can still
+// read the old buffer, so nothing waits for the other slots (hipFree would
+// synchronise the whole device). Growth takes a quarter more than asked.
 template <typename T>
-int grow(T** p, uint64_t* cap, uint64_t need) {
+int grow(T** p, uint64_t* cap, uint64_t need, hipStream_t s) {
   if (need <= *cap && *p) return OIT_OK;
-  if (*p) {  // earlier requests' launches may still read it
-    (void)hipDeviceSynchronize();
-    (void)hipFree(*p);
-  }
+  if (*p) (void)hipFreeAsync(*p, s);
   *p = nullptr;
   *cap = 0;
-  const int rc = st(hipMalloc((void**)p, (need ? need : 1) * sizeof(T)));
-  if (!rc) *cap = need;
+  const uint64_t n = need ? need + need / 4 : 1;
+  const int rc = st(hipMallocAsync((void**)p, n * sizeof(T), s));
+  if (!rc) *cap = n;
   return rc;
 }
+
+// Device bytes of a request's fragment stream and own output in a slot.
+uint64_t stream_bytes(uint64_t F, uint64_t T, bool own_out) { return 24 * F + (own_out ? 16 * T : 0); }
 
 struct Pending {
   oit_request req;
   Clock::time_point t_submit;
+  uint64_t replay = 0;  // oit_server_replay call that submitted it (0: oit_server_submit)
+};
+
+struct Done {
+  oit_completion c;
+  uint64_t replay;
 };
 
 enum class SlotState { kIdle, kCounting, kSizing, kRunning };
@@ -100,28 +110,43 @@ struct Slot {
   uint32_t index = 0;
   hipStream_t stream = nullptr;
   hipEvent_t done = nullptr;
+  // device intervals of the request: count (ev[0] .. ev[1]) and the
+  // generation plus OIT passes (ev[2] .. done)
+  hipEvent_t ev[3] = {};
   oit::SceneGen gen;
-  oit_ctx* ctx[kAlgs] = {};
-  uint64_t resident[kAlgs] = {};
+  int ctx = -1;  // index of the server context the slot's request uses (sizing .. done)
   uint32_t* target = nullptr;
   float* depth = nullptr;
   float4* rgba = nullptr;
   uint64_t cap_target = 0, cap_depth = 0, cap_rgba = 0;
   float4* out_own = nullptr;
   uint64_t cap_out = 0;
+  uint64_t held_stream() const { return 4 * (cap_target + cap_depth) + 16 * (cap_rgba + cap_out); }
   SlotState state = SlotState::kIdle;
   Pending job{};
   uint64_t seq = 0;  // admission order (memory is granted in this order)
   uint64_t fragments = 0;
-  uint64_t budgeted = 0;
+  uint64_t budgeted = 0;     // A-buffer bytes of the request's context
+  uint64_t stream_need = 0;  // fragment-stream (and own output) bytes of the request
   const float* out = nullptr;
   Clock::time_point t_start;
 };
 
 }  // namespace
 
+// A context of the server's pool: any slot takes an idle one of the
+// request's algorithm (contexts keep their buffers between requests).
+struct PoolCtx {
+  oit_ctx* ctx = nullptr;
+  uint32_t alg = 0;
+  uint64_t resident = 0;  // device bytes it holds
+  bool busy = false;
+  uint64_t last_use = 0;  // admission number of its last request
+};
+
 struct oit_server {
   int device = 0;
+  std::vector<PoolCtx> ctxs;  // dispatcher only
   oit_server_config cfg{};
   Clock::time_point t0;
   std::vector<std::unique_ptr<Slot>> slots;
@@ -130,7 +155,8 @@ struct oit_server {
   std::condition_variable cv;       // work arrived / stop
   std::condition_variable idle_cv;  // everything finished
   std::deque<Pending> queue;
-  std::vector<oit_completion> done;
+  std::vector<Done> done;
+  uint64_t replay_calls = 0;
   bool stop = false;
   uint64_t submitted = 0, completed = 0, failed = 0;
   uint64_t held = 0, peak_held = 0;
@@ -143,11 +169,31 @@ struct oit_server {
 
 namespace {
 
+// Device bytes the slots hold: contexts and fragment-stream buffers.
 uint64_t held_bytes(const oit_server* s) {
   uint64_t h = 0;
-  for (const auto& sl : s->slots)
-    for (int a = 0; a < kAlgs; ++a) h += sl->resident[a];
+  for (const auto& sl : s->slots) h += sl->held_stream();
+  for (const auto& c : s->ctxs) h += c.resident;
   return h;
+}
+
+// The context a request of `alg` needing `need` bytes takes: the smallest
+// idle one that already holds enough, else the largest idle one (it grows),
+// else a new entry (created at launch). Returns its index.
+int pick_ctx(oit_server* s, uint32_t alg, uint64_t need) {
+  int fit = -1, big = -1;
+  for (int i = 0; i < (int)s->ctxs.size(); ++i) {
+    const PoolCtx& c = s->ctxs[i];
+    if (c.busy || c.alg != alg) continue;
+    if (c.resident >= need && (fit < 0 || c.resident < s->ctxs[fit].resident)) fit = i;
+    if (big < 0 || c.resident > s->ctxs[big].resident) big = i;
+  }
+  if (fit >= 0) return fit;
+  if (big >= 0) return big;
+  PoolCtx c;
+  c.alg = alg;
+  s->ctxs.push_back(c);
+  return (int)s->ctxs.size() - 1;
 }
 
 void publish_held(oit_server* s) {
@@ -170,39 +216,57 @@ void finish(oit_server* s, Slot& sl, int status) {
   c.t_submit_us = micros(sl.job.t_submit, s->t0);
   c.t_start_us = micros(sl.t_start, s->t0);
   c.t_done_us = micros(now, s->t0);
+  if (status == OIT_OK) {
+    float a = 0.f, b = 0.f;
+    if (hipEventElapsedTime(&a, sl.ev[0], sl.ev[1]) == hipSuccess &&
+        hipEventElapsedTime(&b, sl.ev[2], sl.done) == hipSuccess)
+      c.device_us = 1e3 * ((double)a + (double)b);
+    else
+      (void)hipGetLastError();
+  }
   sl.state = SlotState::kIdle;
+  if (sl.ctx >= 0) s->ctxs[sl.ctx].busy = false;  // finish runs on the dispatcher, as every ctxs access
+  sl.ctx = -1;
   std::lock_guard<std::mutex> lk(s->mu);
-  s->done.push_back(c);
+  s->done.push_back(Done{c, sl.job.replay});
   s->completed++;
   if (status != OIT_OK) s->failed++;
   s->in_flight--;
 }
 
-// Frees idle contexts (largest first) until `need` more bytes fit the budget.
-bool make_room(oit_server* s, Slot& self, int self_alg, uint64_t need) {
+// Frees a slot's fragment-stream and output buffers, in order on its stream.
+void release_stream(Slot& sl) {
+  for (void* p : {(void*)sl.target, (void*)sl.depth, (void*)sl.rgba, (void*)sl.out_own})
+    if (p) (void)hipFreeAsync(p, sl.stream);
+  sl.target = nullptr;
+  sl.depth = nullptr;
+  sl.rgba = nullptr;
+  sl.out_own = nullptr;
+  sl.cap_target = sl.cap_depth = sl.cap_rgba = sl.cap_out = 0;
+}
+
+// Frees idle contexts (least recently used first) and idle slots' stream
+// buffers until `need` more bytes fit the budget; `keep` is the context the
+// request itself will use.
+bool make_room(oit_server* s, Slot& self, int keep, uint64_t need) {
   uint64_t held = held_bytes(s);
   if (held + need <= s->cfg.budget_bytes) return true;
-  struct Cand {
-    Slot* sl;
-    int alg;
-    uint64_t bytes;
-  };
-  std::vector<Cand> c;
-  for (auto& sp : s->slots) {
-    Slot& sl = *sp;
-    for (int a = 0; a < kAlgs; ++a) {
-      if (!sl.ctx[a] || !sl.resident[a]) continue;
-      if (&sl == &self && a == self_alg) continue;
-      if (sl.state == SlotState::kRunning && a == (int)sl.job.req.algorithm) continue;
-      c.push_back({&sl, a, sl.resident[a]});
-    }
-  }
-  std::sort(c.begin(), c.end(), [](const Cand& x, const Cand& y) { return x.bytes > y.bytes; });
-  for (const Cand& x : c) {
+  std::vector<int> idle;
+  for (int i = 0; i < (int)s->ctxs.size(); ++i)
+    if (i != keep && !s->ctxs[i].busy && s->ctxs[i].resident) idle.push_back(i);
+  std::sort(idle.begin(), idle.end(), [&](int a, int b) { return s->ctxs[a].last_use < s->ctxs[b].last_use; });
+  for (int i : idle) {
     if (held + need <= s->cfg.budget_bytes) break;
-    if (oit_trim(x.sl->ctx[x.alg]) != OIT_OK) continue;
-    held -= x.sl->resident[x.alg];
-    x.sl->resident[x.alg] = 0;
+    if (oit_trim(s->ctxs[i].ctx) != OIT_OK) continue;
+    held -= s->ctxs[i].resident;
+    s->ctxs[i].resident = 0;
+  }
+  for (auto& sp : s->slots) {
+    if (held + need <= s->cfg.budget_bytes) break;
+    Slot& sl = *sp;
+    if (&sl == &self || sl.state == SlotState::kRunning || !sl.held_stream()) continue;
+    held -= sl.held_stream();
+    release_stream(sl);
   }
   return held + need <= s->cfg.budget_bytes;
 }
@@ -214,16 +278,18 @@ int launch(oit_server* s, Slot& sl) {
   const uint64_t F = sl.fragments;
   const uint64_t T = (uint64_t)r.scene.width * r.scene.height * r.scene.samples;
   int rc;
-  if ((rc = grow(&sl.target, &sl.cap_target, F))) return rc;
-  if ((rc = grow(&sl.depth, &sl.cap_depth, F))) return rc;
-  if ((rc = grow(&sl.rgba, &sl.cap_rgba, F))) return rc;
+  if ((rc = st(hipEventRecord(sl.ev[2], sl.stream)))) return rc;
+  if ((rc = grow(&sl.target, &sl.cap_target, F, sl.stream))) return rc;
+  if ((rc = grow(&sl.depth, &sl.cap_depth, F, sl.stream))) return rc;
+  if ((rc = grow(&sl.rgba, &sl.cap_rgba, F, sl.stream))) return rc;
   float* out = r.out;
   if (!out) {
-    if ((rc = grow(&sl.out_own, &sl.cap_out, T))) return rc;
+    if ((rc = grow(&sl.out_own, &sl.cap_out, T, sl.stream))) return rc;
     out = reinterpret_cast<float*>(sl.out_own);
   }
   sl.out = out;
-  if (!sl.ctx[alg] && (rc = oit_create(&sl.ctx[alg], s->device))) return rc;
+  PoolCtx& pc = s->ctxs[sl.ctx];
+  if (!pc.ctx && (rc = oit_create(&pc.ctx, s->device))) return rc;
   oit_config c;
   std::memset(&c, 0, sizeof c);
   c.algorithm = alg;
@@ -238,15 +304,15 @@ int launch(oit_server* s, Slot& sl) {
   c.background[3] = 1.0f;
   c.wboit_view_near = 2.0f;
   c.wboit_view_span = 8.0f;
-  if ((rc = oit_configure(sl.ctx[alg], &c))) return rc;
+  if ((rc = oit_configure(pc.ctx, &c))) return rc;
   oit_sizes z;
-  if (oit_query_sizes(sl.ctx[alg], &z) == OIT_OK) sl.resident[alg] = z.resident_bytes;
+  if (oit_query_sizes(pc.ctx, &z) == OIT_OK) pc.resident = z.resident_bytes;
   publish_held(s);
   if ((rc = sl.gen.emit(sl.target, sl.depth, reinterpret_cast<float*>(sl.rgba), sl.cap_target, sl.stream))) return rc;
   oit_fragments f{sl.target, sl.depth, reinterpret_cast<const float*>(sl.rgba), F, nullptr};
-  if ((rc = oit_clear(sl.ctx[alg], sl.stream))) return rc;
-  if ((rc = oit_append(sl.ctx[alg], &f, sl.stream))) return rc;
-  if ((rc = oit_resolve(sl.ctx[alg], out, sl.stream))) return rc;
+  if ((rc = oit_clear(pc.ctx, sl.stream))) return rc;
+  if ((rc = oit_append(pc.ctx, &f, sl.stream))) return rc;
+  if ((rc = oit_resolve(pc.ctx, out, sl.stream))) return rc;
   return st(hipEventRecord(sl.done, sl.stream));
 }
 
@@ -269,6 +335,8 @@ bool step(oit_server* s) {
     } else if (sl.state == SlotState::kCounting && sl.gen.count_ready()) {
       sl.fragments = sl.gen.fragments();
       sl.budgeted = request_bytes(sl.job.req, sl.fragments);
+      const auto& sc = sl.job.req.scene;
+      sl.stream_need = stream_bytes(sl.fragments, (uint64_t)sc.width * sc.height * sc.samples, !sl.job.req.out);
       sl.state = SlotState::kSizing;
       moved = true;
     }
@@ -282,13 +350,23 @@ bool step(oit_server* s) {
   for (Slot* slp : sizing) {
     Slot& sl = *slp;
     const int alg = (int)sl.job.req.algorithm;
-    if (sl.budgeted > s->cfg.budget_bytes) {
+    if (sl.budgeted + sl.stream_need > s->cfg.budget_bytes) {
       finish(s, sl, OIT_ERR_OVER_BUDGET);
       moved = true;
       continue;
     }
-    const uint64_t extra = sl.budgeted > sl.resident[alg] ? sl.budgeted - sl.resident[alg] : 0;
-    if (!make_room(s, sl, alg, extra)) break;
+    // the context's growth plus the slot's stream buffers' growth (with the
+    // quarter of slack grow() adds)
+    const int ci = pick_ctx(s, (uint32_t)alg, sl.budgeted);
+    const uint64_t held_c = s->ctxs[ci].resident;
+    const uint64_t held_s = sl.held_stream();
+    const uint64_t want_s = sl.stream_need + sl.stream_need / 4;
+    const uint64_t extra = (sl.budgeted > held_c ? sl.budgeted - held_c : 0) +
+                           (sl.stream_need > held_s ? want_s - held_s : 0);
+    if (!make_room(s, sl, ci, extra)) break;
+    sl.ctx = ci;
+    s->ctxs[ci].busy = true;
+    s->ctxs[ci].last_use = sl.seq;
     sl.t_start = Clock::now();
     sl.state = SlotState::kRunning;
     const int rc = launch(s, sl);
@@ -315,7 +393,9 @@ bool step(oit_server* s) {
     sl.out = nullptr;
     sl.t_start = Clock::now();
     sl.state = SlotState::kCounting;
-    const int rc = sl.gen.begin(&p.req.scene, sl.stream);
+    int rc = st(hipEventRecord(sl.ev[0], sl.stream));
+    if (!rc) rc = sl.gen.begin(&p.req.scene, sl.stream);
+    if (!rc) rc = st(hipEventRecord(sl.ev[1], sl.stream));
     if (rc) finish(s, sl, rc);
     moved = true;
   }
@@ -345,15 +425,18 @@ void dispatcher(oit_server* s) {
 }
 
 void release(oit_server* s) {
+  for (auto& c : s->ctxs)
+    if (c.ctx) (void)oit_destroy(c.ctx);
+  s->ctxs.clear();
   for (auto& sp : s->slots) {
     Slot& sl = *sp;
-    for (int a = 0; a < kAlgs; ++a)
-      if (sl.ctx[a]) (void)oit_destroy(sl.ctx[a]);
-    (void)hipFree(sl.target);
-    (void)hipFree(sl.depth);
-    (void)hipFree(sl.rgba);
-    (void)hipFree(sl.out_own);
+    if (sl.stream) {
+      release_stream(sl);
+      (void)hipStreamSynchronize(sl.stream);
+    }
     if (sl.done) (void)hipEventDestroy(sl.done);
+    for (hipEvent_t e : sl.ev)
+      if (e) (void)hipEventDestroy(e);
     if (sl.stream) (void)hipStreamDestroy(sl.stream);
   }
   s->slots.clear();
@@ -366,6 +449,18 @@ int check_request(const oit_request* r) {
   if (r->tail_blend > 1u || r->reserved != 0u) return OIT_ERR_INVALID_ARG;
   if (!oit::scene_params_valid(&r->scene)) return OIT_ERR_INVALID_ARG;
   if (r->out && ((uintptr_t)r->out & 15u)) return OIT_ERR_INVALID_ARG;
+  return OIT_OK;
+}
+
+int submit_tagged(oit_server* s, const oit_request& r, uint64_t replay) {
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (s->stop) return OIT_ERR_STATE;
+    s->queue.push_back(Pending{r, Clock::now(), replay});
+    s->submitted++;
+    s->busy = true;
+  }
+  s->cv.notify_all();
   return OIT_OK;
 }
 
@@ -393,7 +488,8 @@ int oit_server_create(oit_server** out, int device, const oit_server_config* cfg
     auto sl = std::make_unique<Slot>(device);
     sl->index = i;
     rc = st(hipStreamCreateWithFlags(&sl->stream, hipStreamNonBlocking));
-    if (!rc) rc = st(hipEventCreateWithFlags(&sl->done, hipEventDisableTiming));
+    if (!rc) rc = st(hipEventCreate(&sl->done));
+    for (int k = 0; k < 3 && !rc; ++k) rc = st(hipEventCreate(&sl->ev[k]));
     s->slots.push_back(std::move(sl));
   }
   if (rc) {
@@ -425,23 +521,21 @@ int oit_server_submit(oit_server* s, const oit_request* r) {
   if (!s) return OIT_ERR_INVALID_ARG;
   const int rc = check_request(r);
   if (rc) return rc;
-  {
-    std::lock_guard<std::mutex> lk(s->mu);
-    if (s->stop) return OIT_ERR_STATE;
-    s->queue.push_back(Pending{*r, Clock::now()});
-    s->submitted++;
-    s->busy = true;
-  }
-  s->cv.notify_all();
-  return OIT_OK;
+  return submit_tagged(s, *r, 0);
 }
 
 int oit_server_poll(oit_server* s, oit_completion* out, uint32_t max, uint32_t* n) {
   if (!s || !n || (max && !out)) return OIT_ERR_INVALID_ARG;
   std::lock_guard<std::mutex> lk(s->mu);
-  const uint32_t k = (uint32_t)std::min<size_t>(max, s->done.size());
-  for (uint32_t i = 0; i < k; ++i) out[i] = s->done[i];
-  s->done.erase(s->done.begin(), s->done.begin() + k);
+  uint32_t k = 0;  // completions of oit_server_submit only; replay calls collect their own
+  for (auto it = s->done.begin(); it != s->done.end() && k < max;) {
+    if (it->replay == 0) {
+      out[k++] = it->c;
+      it = s->done.erase(it);
+    } else {
+      ++it;
+    }
+  }
   *n = k;
   return OIT_OK;
 }
@@ -461,6 +555,11 @@ int oit_server_replay(oit_server* s, const oit_request* reqs, const double* arri
     if (rc) return rc;
     if (i && arrival_us[i] < arrival_us[i - 1]) return OIT_ERR_INVALID_ARG;
   }
+  uint64_t call;
+  {
+    std::lock_guard<std::mutex> lk(s->mu);
+    call = ++s->replay_calls;
+  }
   const auto start = Clock::now();
   for (uint32_t i = 0; i < n; ++i) {
     const auto due = start + std::chrono::duration_cast<Clock::duration>(
@@ -472,8 +571,8 @@ int oit_server_replay(oit_server* s, const oit_request* reqs, const double* arri
         std::this_thread::sleep_for(due - now - std::chrono::microseconds(200));
     }
     oit_request r = reqs[i];
-    r.tag = i;
-    const int rc = oit_server_submit(s, &r);
+    r.tag = i;  // index within this call; the caller's tag is restored below
+    const int rc = submit_tagged(s, r, call);
     if (rc) return rc;
   }
   int rc = oit_server_drain(s, timeout_ms);
@@ -481,10 +580,11 @@ int oit_server_replay(oit_server* s, const oit_request* reqs, const double* arri
   std::vector<bool> seen(n, false);
   std::lock_guard<std::mutex> lk(s->mu);
   for (auto it = s->done.begin(); it != s->done.end();) {
-    if (it->tag < n && !seen[it->tag]) {
-      out[it->tag] = *it;
-      out[it->tag].tag = reqs[it->tag].tag;
-      seen[it->tag] = true;
+    const uint64_t i = it->c.tag;
+    if (it->replay == call && i < n && !seen[i]) {
+      out[i] = it->c;
+      out[i].tag = reqs[i].tag;
+      seen[i] = true;
       it = s->done.erase(it);
     } else {
       ++it;
