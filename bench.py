@@ -327,7 +327,10 @@ def bench_batch(seed, frames_n, resolutions=(256, 512, 1024, 2048)):
             "sequential_ms": round(seq_ms, 4), "batched_ms": round(bat_ms, 4),
             "speedup": round(seq_ms / bat_ms, 3), "G_fragments_per_s": round(total / bat_ms / 1e6, 3),
             "frame_order_ms": round(order_ms, 4),
-            "bit_exact_vs_sequential": bool(exact), "append_chunks": int(t.append_chunks)}
+            "bit_exact_vs_sequential": bool(exact), "append_chunks": int(t.append_chunks),
+            "bin_density": float(t.bin_density),
+            "binned_frames": sum(1 for f in frames if f.count >= t.bin_density * f.width * f.height * f.samples
+                                 and f.count > 0)}
 
 
 def bench_serve(seed, n, rates, slots):

@@ -342,6 +342,10 @@ typedef struct oit_batch_tuning {
   uint32_t append_chunks;
   uint32_t resolve_variant[6][2];
   uint32_t resolve_blocks[6][2];
+  float    bin_density;       /* frames with at least this many fragments per target are binned: a
+                                 counting sort gathers each target's fragments into one bucket and the
+                                 composite replays it in descending id order (sorted in LDS), in place of
+                                 the list append and walk; the same image. 0 = every frame, default 8 */
 } oit_batch_tuning;
 
 typedef struct oit_batch_sizes {

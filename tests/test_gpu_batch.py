@@ -132,7 +132,8 @@ def test_batch_tuning_keeps_results(mixed):
         b.tune(fr.frames, st)
         t = b.tuning
         assert t.append_chunks in (4, 8)
-        _say("tuned", t.append_chunks, [list(r) for r in t.resolve_variant], [list(r) for r in t.resolve_blocks])
+        _say("tuned", t.append_chunks, [list(r) for r in t.resolve_variant], [list(r) for r in t.resolve_blocks],
+             t.bin_density)
         b.run(fr.frames, st)
         _assert_images(fr.images(st), want)
         for v in (0, _abi.VARIANT_LL_SINGLE_WALK, _abi.VARIANT_LL_KEYS_ONLY,
@@ -151,6 +152,62 @@ def test_batch_tuning_keeps_results(mixed):
         bad.append_chunks = 5
         with pytest.raises(_abi.OitError):
             b.tuning = bad
+        bad = b.tuning
+        bad.bin_density = float("nan")
+        with pytest.raises(_abi.OitError):
+            b.tuning = bad
+
+
+def _with_density(b, density):
+    t = b.tuning
+    t.bin_density = density
+    b.tuning = t
+
+
+@pytest.mark.parametrize("density", [0.0, 3.0e38])  # every frame binned / none
+@pytest.mark.parametrize("concurrent,graph", [(False, False), (True, True)])
+def test_binned_and_linked_frames_equal_single_frame(mixed, density, concurrent, graph):
+    """Frames binned (count, scan, scatter into buckets, buckets replayed in descending id order:
+    one lane for small buckets, one wave with an LDS sort for larger ones) or linked give the
+    single-frame Linked List image bit for bit, run after run."""
+    st, fr, want = mixed
+    with Batch(0, concurrent=concurrent, graph=graph) as b:
+        _with_density(b, density)
+        for rep in range(2):
+            for o in fr.outs:
+                o.upload(np.full(o.shape, np.nan, np.float32), st)
+            b.run(fr.frames, st)
+            _assert_images(fr.images(st), want)
+        _with_density(b, 3.0e38 if density == 0.0 else 0.0)  # the other way on the same pool
+        b.run(fr.frames, st)
+        _assert_images(fr.images(st), want)
+
+
+def test_binned_bucket_beyond_lds(oracle):
+    """One pixel with 1500 fragments and one with 700 (past and within the 1024 entries a
+    wave orders in LDS) plus sparse ones, binned: equal to the single-frame Linked List and
+    to the oracle."""
+    rng = np.random.default_rng(17)
+    W, H = 16, 8
+    tg = np.concatenate([np.full(1500, 5), np.full(700, 77), rng.integers(0, W * H, 600)]).astype(np.uint32)
+    rng.shuffle(tg)
+    F = len(tg)
+    d = rng.uniform(0.0, 1.0, F).astype(np.float32)
+    a = rng.uniform(0.05, 0.6, F).astype(np.float32)
+    c = np.concatenate([rng.uniform(0, 1, (F, 3)).astype(np.float32) * a[:, None], a[:, None]], axis=1)
+    c = np.ascontiguousarray(c, dtype=np.float32)
+    st = Stream()
+    t_d, d_d, c_d = DeviceBuffer.from_numpy(tg), DeviceBuffer.from_numpy(d), DeviceBuffer.from_numpy(c)
+    for K, tail in ((8, True), (32, True), (4, False)):
+        out = DeviceBuffer((W * H, 4), "float32")
+        with Batch(0) as b:
+            _with_density(b, 0.0)
+            b.run([BatchFrame(width=W, height=H, layers=K, tail_blend=tail, target=t_d, depth=d_d, rgba=c_d,
+                              count=F, out=out)], st)
+            st.synchronize()
+            got = out.numpy()
+        want = oracle.run(oracle.make_config(oracle.LINKEDLIST, K, int(tail), W, H, ll_capacity=F + 1), tg, d, c)
+        assert np.array_equal(got.view(np.uint32), want["out"].view(np.uint32)), (K, tail)
 
 
 def test_batch_rejects_bad_frames(mixed):

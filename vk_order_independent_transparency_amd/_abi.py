@@ -137,6 +137,7 @@ class OitBatchTuning(ctypes.Structure):
         ("append_chunks", ctypes.c_uint32),
         ("resolve_variant", (ctypes.c_uint32 * 2) * 6),
         ("resolve_blocks", (ctypes.c_uint32 * 2) * 6),
+        ("bin_density", ctypes.c_float),
     ]
 
 

@@ -37,6 +37,7 @@ constexpr uint64_t kMaxHeads = 0xFFFFFF00ull;
 constexpr uint64_t kMaxFrameFragments = 0x7FFFFFFEull;
 constexpr int kSide = 3;               // forked streams for concurrent groups
 constexpr uint32_t kDefaultBlocks = 8192;
+constexpr float kDefaultBinDensity = 8.f;  // fragments per target from which a frame is binned
 
 int st(hipError_t e) {
   if (e == hipSuccess) return OIT_OK;
@@ -73,6 +74,7 @@ uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 struct Group {
   uint32_t K, tail, f0, f1, tile0, tiles;
+  bool bin;  // holds binned frames (its deep buckets need k_bin_deep)
 };
 
 struct Plan {
@@ -82,13 +84,16 @@ struct Plan {
   std::vector<Group> groups;
   uint64_t pool_slots = 0;  // padded fragments (nodes without the terminator)
   uint64_t heads = 0;       // padded targets
+  bool bin = false;         // some frame is binned
 };
 
 bool same_table(const std::vector<BatchFrameDev>& a, const std::vector<BatchFrameDev>& b) {
   return a.size() == b.size() && (a.empty() || std::memcmp(a.data(), b.data(), a.size() * sizeof(BatchFrameDev)) == 0);
 }
 
-int make_plan(const oit_batch_frame* fr, uint32_t n, Plan& p) {
+// Frames with at least `bin_density` fragments per target are binned
+// (bucket per target) instead of linked (list per target).
+int make_plan(const oit_batch_frame* fr, uint32_t n, Plan& p, float bin_density) {
   if (n > 0 && !fr) return OIT_ERR_INVALID_ARG;
   std::vector<uint32_t> order(n);
   for (uint32_t i = 0; i < n; ++i) {
@@ -126,6 +131,8 @@ int make_plan(const oit_batch_frame* fr, uint32_t n, Plan& p) {
     d.T = f.width * f.height * f.samples;
     d.target_base = (uint32_t)tgt;
     d.frag_base = (uint32_t)frag;
+    d.mode = (f.frags.count > 0 && (double)f.frags.count >= (double)bin_density * (double)d.T) ? oit::kModeBin
+                                                                                                : oit::kModeList;
     frag += round_up(f.frags.count, kFragAlign);
     tgt += round_up(d.T, kTile);
     if (frag > kMaxPoolSlots || tgt > kMaxHeads) return OIT_ERR_CAPACITY;
@@ -138,7 +145,12 @@ int make_plan(const oit_batch_frame* fr, uint32_t n, Plan& p) {
       g.f1 = j + 1;
       g.tile0 = d.target_base / kTile;
       g.tiles = 0;
+      g.bin = false;
       p.groups.push_back(g);
+    }
+    if (d.mode == oit::kModeBin) {
+      p.groups.back().bin = true;
+      p.bin = true;
     }
     p.table.push_back(d);
   }
@@ -183,11 +195,13 @@ oit_batch_tuning default_tuning() {
   t.append_chunks = 4;
   for (int k = 0; k < 6; ++k)
     for (int tl = 0; tl < 2; ++tl) t.resolve_blocks[k][tl] = kDefaultBlocks;
+  t.bin_density = kDefaultBinDensity;
   return t;
 }
 
 bool tuning_valid(const oit_batch_tuning& t) {
   if (t.append_chunks != 4 && t.append_chunks != 8) return false;
+  if (!(t.bin_density >= 0.f)) return false;  // NaN too
   for (int k = 0; k < 6; ++k)
     for (int tl = 0; tl < 2; ++tl)
       if (t.resolve_variant[k][tl] & ~(OIT_VARIANT_LL_SINGLE_WALK | OIT_VARIANT_LL_KEYS_ONLY)) return false;
@@ -207,6 +221,12 @@ struct oit_batch {
   uint32_t* counters = nullptr;  // [0] overflow (stays 0: pools are exact)
   uint64_t cap_nodes = 0;        // allocated nodes, terminator included
   uint64_t cap_heads = 0;
+  // binned frames: bucket cursor and deep-bucket queue per head, scan base
+  // per tile, queue lengths per group
+  uint32_t* cursor = nullptr;
+  uint32_t* deep_q = nullptr;
+  uint32_t* tile_sum = nullptr;
+  uint32_t* deep_n = nullptr;
   bool dirty = false;            // a run did not complete: re-initialise before the next
   // frame table
   BatchFrameDev* d_table = nullptr;
@@ -241,6 +261,10 @@ void release(oit_batch* b) {
   (void)hipFree(b->cd);
   (void)hipFree(b->next);
   (void)hipFree(b->counters);
+  (void)hipFree(b->cursor);
+  (void)hipFree(b->deep_q);
+  (void)hipFree(b->tile_sum);
+  (void)hipFree(b->deep_n);
   (void)hipFree(b->d_table);
   if (b->h_table) (void)hipHostFree(b->h_table);
   (void)hipFree(b->d_sched);
@@ -291,10 +315,16 @@ int ensure_pool(oit_batch* b, uint64_t nodes, uint64_t heads, hipStream_t s) {
   }
   if (heads > b->cap_heads) {
     (void)hipFree(b->head);
-    b->head = nullptr;
+    (void)hipFree(b->cursor);
+    (void)hipFree(b->deep_q);
+    (void)hipFree(b->tile_sum);
+    b->head = b->cursor = b->deep_q = b->tile_sum = nullptr;
     b->cap_heads = 0;
     const uint64_t cap = round_up(heads, 1ull << 16);
     if ((rc = st(hipMalloc((void**)&b->head, cap * sizeof(uint32_t))))) return rc;
+    if ((rc = st(hipMalloc((void**)&b->cursor, cap * sizeof(uint32_t))))) return rc;
+    if ((rc = st(hipMalloc((void**)&b->deep_q, cap * sizeof(uint32_t))))) return rc;
+    if ((rc = st(hipMalloc((void**)&b->tile_sum, cap / kTile * sizeof(uint32_t))))) return rc;
     b->cap_heads = cap;
     if ((rc = st(oit::launch_fill_u32(b->head, cap, 0u, s)))) return rc;
   }
@@ -379,6 +409,10 @@ BatchArgs args_for(const oit_batch* b, const Plan& p) {
   a.tile0 = 0;
   a.tiles = 0;
   a.tab = b->d_tab;
+  a.cursor = b->cursor;
+  a.tile_sum = b->tile_sum;
+  a.deep_q = b->deep_q;
+  a.deep_n = b->deep_n;
   return a;
 }
 
@@ -392,21 +426,34 @@ int launch_append(const oit_batch* b, const Plan& p, uint32_t chunks, hipStream_
   return st(oit::launch_batch_append(a, pool_view(b), chunks, s));
 }
 
-int launch_group(const oit_batch* b, const Plan& p, const Group& g, uint32_t variant, uint32_t blocks,
+// Binned frames' scan and scatter, after the append launch counted them.
+int launch_bin(const oit_batch* b, const Plan& p, uint32_t chunks, hipStream_t s) {
+  if (!p.bin) return OIT_OK;
+  BatchArgs a = args_for(b, p);
+  a.units = (uint32_t)(p.pool_slots / (64u * chunks));
+  return st(oit::launch_bin_prepare(a, pool_view(b), (uint32_t)(p.heads / kTile), chunks, s));
+}
+
+int launch_group(const oit_batch* b, const Plan& p, const Group& g, uint32_t gi, uint32_t variant, uint32_t blocks,
                  hipStream_t s) {
   BatchArgs a = args_for(b, p);
   a.f0 = g.f0;
   a.f1 = g.f1;
   a.tile0 = g.tile0;
   a.tiles = g.tiles;
-  return st(oit::launch_batch_resolve(a, pool_view(b), g.K, g.tail != 0, !(variant & OIT_VARIANT_LL_SINGLE_WALK),
-                                      (variant & OIT_VARIANT_LL_KEYS_ONLY) != 0, blocks, s));
+  a.deep_q = b->deep_q + (uint64_t)g.tile0 * kTile;  // a group queues at most its own targets
+  a.deep_n = b->deep_n + gi;
+  int rc = st(oit::launch_batch_resolve(a, pool_view(b), g.K, g.tail != 0, !(variant & OIT_VARIANT_LL_SINGLE_WALK),
+                                        (variant & OIT_VARIANT_LL_KEYS_ONLY) != 0, blocks, s));
+  if (!rc && g.bin) rc = st(oit::launch_bin_deep(a, pool_view(b), g.K, g.tail != 0, s));
+  return rc;
 }
 
 // Append + every group's resolve on s (resolve groups forked when asked).
 int enqueue(oit_batch* b, const Plan& p, hipStream_t s) {
   int rc;
   if ((rc = launch_append(b, p, b->tuning.append_chunks, s))) return rc;
+  if ((rc = launch_bin(b, p, b->tuning.append_chunks, s))) return rc;
   const bool fork = (b->flags & OIT_BATCH_CONCURRENT) && p.groups.size() > 1;
   if (fork && (rc = st(hipEventRecord(b->fork_ev, s)))) return rc;
   bool used[kSide] = {};
@@ -422,8 +469,8 @@ int enqueue(oit_batch* b, const Plan& p, hipStream_t s) {
         used[j] = true;
       }
     }
-    if ((rc = launch_group(b, p, gr, b->tuning.resolve_variant[ki][gr.tail], b->tuning.resolve_blocks[ki][gr.tail],
-                           gs)))
+    if ((rc = launch_group(b, p, gr, (uint32_t)g, b->tuning.resolve_variant[ki][gr.tail],
+                           b->tuning.resolve_blocks[ki][gr.tail], gs)))
       return rc;
   }
   for (int j = 0; j < kSide; ++j) {
@@ -468,7 +515,7 @@ int run_graph(oit_batch* b, const Plan& p, hipStream_t s) {
 
 int prepare(oit_batch* b, const oit_batch_frame* frames, uint32_t n, Plan& p, hipStream_t s) {
   int rc;
-  if ((rc = make_plan(frames, n, p))) return rc;
+  if ((rc = make_plan(frames, n, p, b->tuning.bin_density))) return rc;
   return ensure_pool(b, p.pool_slots + 1, p.heads, s);
 }
 
@@ -496,6 +543,8 @@ int oit_batch_create(oit_batch** out, int device, uint32_t flags) {
   if (!rc) rc = st(hipMemcpy(b->d_tab, &host, sizeof host, hipMemcpyHostToDevice));
   if (!rc) rc = st(hipMalloc((void**)&b->counters, 32 * sizeof(uint32_t)));
   if (!rc) rc = st(hipMemset(b->counters, 0, 32 * sizeof(uint32_t)));
+  if (!rc) rc = st(hipMalloc((void**)&b->deep_n, oit::kBatchMaxGroups * sizeof(uint32_t)));
+  if (!rc) rc = st(hipMemset(b->deep_n, 0, oit::kBatchMaxGroups * sizeof(uint32_t)));
   if (!rc) rc = st(hipEventCreateWithFlags(&b->table_ev, hipEventDisableTiming));
   if (!rc) rc = st(hipEventCreateWithFlags(&b->fork_ev, hipEventDisableTiming));
   for (int i = 0; !rc && i < kSide; ++i) {
@@ -522,11 +571,12 @@ int oit_batch_destroy(oit_batch* b) {
 int oit_batch_query(const oit_batch* b, const oit_batch_frame* frames, uint32_t n, oit_batch_sizes* out) {
   if (!b || !out) return OIT_ERR_INVALID_ARG;
   Plan p;
-  const int rc = make_plan(frames, n, p);
+  const int rc = make_plan(frames, n, p, b->tuning.bin_density);
   if (rc) return rc;
   out->pool_nodes = p.pool_slots + 1;
   out->head_entries = p.heads;
-  out->bytes = out->pool_nodes * 12u + out->head_entries * 4u;
+  // nodes 12 B; per head: the head, a bucket cursor and a deep-queue entry; a scan base per tile
+  out->bytes = out->pool_nodes * 12u + out->head_entries * 12u + out->head_entries / kTile * 4u;
   out->groups = (uint32_t)p.groups.size();
   out->frames = n;
   return OIT_OK;
@@ -577,7 +627,7 @@ int oit_batch_tune(oit_batch* b, const oit_batch_frame* frames, uint32_t n, void
     for (size_t gi = 0; gi < G; ++gi) {
       int r;
       if (timed && (r = st(hipEventRecord(ev[2 + 2 * gi], s)))) return r;
-      if ((r = launch_group(b, p, p.groups[gi], variant[gi], blocks[gi], s))) return r;
+      if ((r = launch_group(b, p, p.groups[gi], (uint32_t)gi, variant[gi], blocks[gi], s))) return r;
       if (timed && (r = st(hipEventRecord(ev[3 + 2 * gi], s)))) return r;
     }
     return OIT_OK;
@@ -588,6 +638,38 @@ int oit_batch_tune(oit_batch* b, const oit_batch_frame* frames, uint32_t n, void
     cur_v[gi] = b->tuning.resolve_variant[ki][p.groups[gi].tail];
     cur_b[gi] = b->tuning.resolve_blocks[ki][p.groups[gi].tail];
   }
+  // frames binned from which density (fragments per target): the whole run
+  // timed for each candidate, from never to always
+  if (!rc) {
+    const float dens[7] = {3.0e38f, 32.f, 16.f, 8.f, 4.f, 2.f, 0.f};
+    float best_ms = 0.f;
+    for (int di = 0; di < 7 && !rc; ++di) {
+      Plan pd;
+      if ((rc = make_plan(frames, n, pd, dens[di]))) break;
+      if ((rc = upload_table(b, pd, s))) break;
+      float ms_min = 0.f;
+      for (int rep = 0; rep < 2 && !rc; ++rep) {
+        if ((rc = st(hipEventRecord(ev[0], s)))) break;
+        if ((rc = launch_append(b, pd, b->tuning.append_chunks, s))) break;
+        if ((rc = launch_bin(b, pd, b->tuning.append_chunks, s))) break;
+        for (size_t gi = 0; gi < G && !rc; ++gi)
+          rc = launch_group(b, pd, pd.groups[gi], (uint32_t)gi, cur_v[gi], cur_b[gi], s);
+        if (rc) break;
+        if ((rc = st(hipEventRecord(ev[1], s)))) break;
+        if ((rc = st(hipEventSynchronize(ev[1])))) break;
+        float ms = 0.f;
+        if ((rc = st(hipEventElapsedTime(&ms, ev[0], ev[1])))) break;
+        if (rep == 0 || ms < ms_min) ms_min = ms;
+      }
+      if (rc) break;
+      if (di == 0 || ms_min < best_ms) {
+        best_ms = ms_min;
+        best.bin_density = dens[di];
+      }
+    }
+    if (!rc) rc = make_plan(frames, n, p, best.bin_density);
+    if (!rc) rc = upload_table(b, p, s);
+  }
   // append: chunks per wave
   if (!rc) {
     float best_ms = 0.f;
@@ -596,6 +678,7 @@ int oit_batch_tune(oit_batch* b, const oit_batch_frame* frames, uint32_t n, void
       for (int rep = 0; rep < 3 && !rc; ++rep) {
         if ((rc = st(hipEventRecord(ev[0], s)))) break;
         if ((rc = launch_append(b, p, chunks, s))) break;
+        if ((rc = launch_bin(b, p, chunks, s))) break;
         if ((rc = st(hipEventRecord(ev[1], s)))) break;
         if ((rc = resolve_all(cur_v.data(), cur_b.data(), false))) break;
         if ((rc = st(hipEventSynchronize(ev[1])))) break;
@@ -622,6 +705,7 @@ int oit_batch_tune(oit_batch* b, const oit_batch_frame* frames, uint32_t n, void
       std::vector<float> tmin(G, 0.f);
       for (int rep = 0; rep < 2 && !rc; ++rep) {
         if ((rc = launch_append(b, p, best.append_chunks, s))) break;
+        if ((rc = launch_bin(b, p, best.append_chunks, s))) break;
         if ((rc = resolve_all(v.data(), c.data(), true))) break;
         if ((rc = st(hipStreamSynchronize(s)))) break;
         for (size_t gi = 0; gi < G && !rc; ++gi) {

@@ -847,6 +847,345 @@ __device__ __forceinline__ uint32_t find_frame(const BatchFrameDev* fr, uint32_t
   return lo;
 }
 
+// ---------------------------------------------------------------------------
+// Binned batch frames (BatchFrameDev::mode == kModeBin): the fragments of
+// each target are gathered into one bucket of the pool by a counting sort
+// (count per target, exclusive scan, scatter) instead of being linked into a
+// list, and the composite orders a bucket by fragment id, newest first (the
+// walk order of the Linked List's descending lists), and replays the Linked
+// List composite (oitLinkedList.frag.glsl:116-172) over it: the image is the
+// single-frame Linked List's bit for bit. There is no ordered insert to walk
+// past concurrent inserts of the same pixel and no chain of dependent loads
+// per list, which is what deep pixels (hundreds of layers) cost the lists.
+// Frame f's buckets fill its own node range: during a run head[t] holds the
+// count of target t, cursor[t] its bucket's end slot after the scatter, and
+// bucket entry s is node s + 1 (node 0 stays the terminator) with
+// cd = (depth key << 32) | colour and next = fragment id + 1. The composite
+// returns head[t] to 0 and the bucket's link words to kUnset, the pool's
+// clean state, so a later run may take the frame either way.
+// ---------------------------------------------------------------------------
+
+constexpr uint32_t kBinShallow = 8;      // buckets of at most this many entries: one lane replays them
+constexpr uint32_t kBinDeepMax = 1024;   // larger ones: one wave, ordered in LDS up to this size
+
+// Count pass of one wave's share (the binned frame's form of the append).
+template <int kU>
+__device__ __forceinline__ void bin_count_wave(const BatchFrameDev& fr, uint64_t first, const LLBuf& ll) {
+  const uint32_t lane = lane_id();
+  uint32_t t[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    t[u] = i < fr.count ? ld_stream(&fr.target[i]) : kEmpty;
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint32_t tt = t[u] < fr.T ? t[u] : kEmpty;
+    const Run r = same_target_run(tt);
+    if (tt != kEmpty && r.end)  // one atomic per run of one target
+      __hip_atomic_fetch_add(&ll.head[fr.target_base + tt], r.lane - r.start_lane + 1u, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Block-wide exclusive scan of one u32 per thread (kThreads a multiple of
+// 64, at most 1024); returns the block total.
+template <int kThreads>
+__device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t& excl, uint32_t* s_part) {
+  const uint32_t lane = lane_id(), w = threadIdx.x >> 6;
+  uint32_t inc = v;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = __shfl_up(inc, (unsigned)o);
+    if (lane >= (uint32_t)o) inc += y;
+  }
+  if (lane == 63u) s_part[w] = inc;
+  __syncthreads();
+  constexpr int kW = kThreads / 64;
+  uint32_t before = 0, total = 0;
+#pragma unroll
+  for (int k = 0; k < kW; ++k) {
+    const uint32_t p = s_part[k];
+    if ((uint32_t)k < w) before += p;
+    total += p;
+  }
+  __syncthreads();  // s_part may be rewritten by the next call
+  excl = before + inc - v;
+  return total;
+}
+
+// Scan step 1: the sum of each 256-target tile's counts (0 for a Linked
+// List frame's tiles). Block 0 also empties the deep-bucket queues.
+__global__ __launch_bounds__(kBlock) void k_bin_tile_sums(BatchArgs b, const uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_part[kBlock / 64];
+  const uint32_t tile = b.tile0 + blockIdx.x;
+  const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, (uint64_t)tile * kBlock);
+  const bool bin = b.frames[f].mode == kModeBin;
+  const uint32_t v = bin ? count[(uint64_t)tile * kBlock + threadIdx.x] : 0u;
+  uint32_t excl;
+  const uint32_t total = block_scan_u32<kBlock>(v, excl, s_part);
+  if (threadIdx.x == 0) b.tile_sum[tile] = total;
+  if (blockIdx.x == 0 && threadIdx.x < kBatchMaxGroups) b.deep_n[threadIdx.x] = 0u;
+}
+
+// Scan step 2, one block per frame: each tile's first bucket slot, the
+// frame's node range start plus the counts of its earlier tiles.
+__global__ __launch_bounds__(1024) void k_bin_frame_scan(BatchArgs b) {
+  __shared__ uint32_t s_part[16];
+  const uint32_t f = b.f0 + blockIdx.x;
+  if (f >= b.f1) return;
+  const BatchFrameDev& fr = b.frames[f];
+  if (fr.mode != kModeBin) return;
+  const uint32_t t0 = fr.target_base / kBlock;
+  const uint32_t nt = (fr.T + kBlock - 1u) / kBlock;
+  uint32_t carry = fr.frag_base;
+  for (uint32_t base = 0; base < nt; base += 1024u) {  // the same trip count on every thread
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nt ? b.tile_sum[t0 + i] : 0u;
+    uint32_t excl;
+    const uint32_t total = block_scan_u32<1024>(v, excl, s_part);
+    if (i < nt) b.tile_sum[t0 + i] = carry + excl;
+    carry += total;
+  }
+}
+
+// Scan step 3: cursor[t] = the first slot of target t's bucket.
+__global__ __launch_bounds__(kBlock) void k_bin_tile_apply(BatchArgs b, const uint32_t* __restrict__ count) {
+  __shared__ uint32_t s_part[kBlock / 64];
+  const uint32_t tile = b.tile0 + blockIdx.x;
+  const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, (uint64_t)tile * kBlock);
+  if (b.frames[f].mode != kModeBin) return;  // the whole block takes the same branch
+  const uint64_t t = (uint64_t)tile * kBlock + threadIdx.x;
+  uint32_t excl;
+  (void)block_scan_u32<kBlock>(count[t], excl, s_part);
+  b.cursor[t] = b.tile_sum[tile] + excl;
+}
+
+// Scatter pass: each fragment of a binned frame to the next entry of its
+// target's bucket (one atomic per run of one target), as a node: encoded
+// (depth key, colour) and its id + 1.
+template <int kU>
+__global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs b, LLBuf ll) {
+  __shared__ float s_thr[257];
+  stage_thresholds(s_thr, b.tab);
+  const uint32_t unit = __builtin_amdgcn_readfirstlane(blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6));
+  if (unit >= b.units) return;
+  const uint64_t g = (uint64_t)unit * (64u * kU);
+  const uint32_t f = find_frame<false>(b.frames, b.f0, b.f1, g);
+  const BatchFrameDev fr = b.frames[f];
+  if (fr.mode != kModeBin) return;
+  const uint32_t lane = lane_id();
+  const uint64_t first = g - fr.frag_base;
+  uint32_t t[kU];
+  float dep[kU];
+  float4 col[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {  // every field of every chunk in one round trip
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    const bool in = i < fr.count;
+    t[u] = in ? ld_stream(&fr.target[i]) : kEmpty;
+    dep[u] = in ? ld_stream(&fr.depth[i]) : 0.f;
+    col[u] = in ? ld_stream(&fr.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    const uint32_t tt = t[u] < fr.T ? t[u] : kEmpty;
+    const Run r = same_target_run(tt);
+    uint32_t base = 0;
+    if (tt != kEmpty && r.end)
+      base = __hip_atomic_fetch_add(&b.cursor[fr.target_base + tt], r.lane - r.start_lane + 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    base = __shfl(base, (int)r.end_lane);  // the run's first slot, from its last lane
+    if (tt != kEmpty) {
+      const uint32_t s = base + (r.lane - r.start_lane) + 1u;  // node index
+      ll.cd[s] = ((uint64_t)depth_key(dep[u]) << 32) | encode_color_lds(col[u], s_thr);
+      ll.next[s] = (uint32_t)i + 1u;
+    }
+  }
+}
+
+// Composite of a bucket of n <= kBinShallow entries (nodes start + 1 ..
+// start + n) by one lane: the entries in descending id order through the
+// Linked List's front-K insertion and composite tail (ll_walk_core), then
+// the front K blended over the tail; resets the bucket. Lane-local only.
+template <int K, bool kTail>
+__device__ __forceinline__ float4 bin_replay_shallow(uint32_t t, uint32_t n, uint32_t start, const LLBuf& ll,
+                                                     const float* s_dec, const float* s_alpha) {
+  uint32_t ids[kBinShallow];
+  uint64_t kcs[kBinShallow];
+#pragma unroll
+  for (int j = 0; j < (int)kBinShallow; ++j) {
+    ids[j] = 0u;
+    kcs[j] = 0ull;
+    if ((uint32_t)j < n) {
+      ids[j] = ll.next[start + 1u + j];
+      kcs[j] = ll.cd[start + 1u + j];
+    }
+  }
+  FrontK<K> a;
+  a.init();
+  float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
+  uint32_t prev = 0xFFFFFFFFu;
+#pragma unroll
+  for (int s = 0; s < (int)kBinShallow; ++s) {
+    if ((uint32_t)s < n) {
+      uint32_t best = 0u;
+      uint64_t bkc = 0ull;
+#pragma unroll
+      for (int j = 0; j < (int)kBinShallow; ++j) {  // the newest entry not replayed yet
+        const bool take = ids[j] < prev && ids[j] > best;
+        best = take ? ids[j] : best;
+        bkc = take ? kcs[j] : bkc;
+      }
+      prev = best;
+      uint64_t ev;
+      uint32_t evc;
+      a.insert_bounded((bkc & 0xFFFFFFFF00000000ull) | best, (uint32_t)bkc, ev, evc, s + 1);
+      if (kTail && (uint32_t)s >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
+    }
+  }
+#pragma unroll
+  for (int j = 0; j < (int)kBinShallow; ++j)
+    if ((uint32_t)j < n) ll.next[start + 1u + j] = kUnset;
+  if (n) ll.head[t] = 0u;
+  const uint32_t m = n < (uint32_t)K ? n : (uint32_t)K;
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((uint32_t)k < m) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
+  return over(col, tailc);
+}
+
+// Lane-wise 64-bit shuffles built from 32-bit ones.
+__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
+  const uint32_t lo = __shfl((uint32_t)v, src), hi = __shfl((uint32_t)(v >> 32), src);
+  return ((uint64_t)hi << 32) | lo;
+}
+__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, unsigned d) {
+  const uint32_t lo = __shfl_up((uint32_t)v, d), hi = __shfl_up((uint32_t)(v >> 32), d);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Orders LDS ops of one wave: every lane's earlier LDS access has completed
+// and none is moved across (the wave's lanes exchange data through LDS).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+}
+
+// Buckets of more than kBinShallow entries, one wave each, from the group's
+// queue: ids and entry positions into LDS, a bitonic sort by id (newest
+// first), the entries' (depth key, colour) gathered in that order, then the
+// front-K insertion with lane k holding front slot k (each entry: one ballot
+// for its position, one shift by lane shuffles), the composite tail folded
+// in the same order, and the composite. A bucket past kBinDeepMax entries
+// is replayed by repeated selection of the newest entry not replayed yet,
+// straight from the pool. Two waves per block: 32 KB of LDS each.
+constexpr uint32_t kBinDeepWaves = 2;
+
+template <int K, bool kTail>
+__global__ __launch_bounds__(64 * kBinDeepWaves) void k_bin_deep(BatchArgs b, LLBuf ll) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  __shared__ uint64_t s_key[kBinDeepWaves][kBinDeepMax];  // (id + 1) << 32 | entry
+  __shared__ uint64_t s_kc[kBinDeepWaves][kBinDeepMax];   // (depth key << 32) | colour, in sorted order
+  for (uint32_t k = threadIdx.x; k < 256u; k += 64u * kBinDeepWaves) {
+    s_dec[k] = b.tab->dec[k];
+    s_alpha[k] = b.tab->alpha[k];
+  }
+  __syncthreads();
+  const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+  uint64_t* key = s_key[w];
+  uint64_t* kcs = s_kc[w];
+  const uint32_t total = __builtin_amdgcn_readfirstlane(*b.deep_n);
+  for (uint32_t e = blockIdx.x * kBinDeepWaves + w; e < total; e += gridDim.x * kBinDeepWaves) {
+    const uint32_t t = __builtin_amdgcn_readfirstlane(b.deep_q[e]);
+    const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, t);
+    const BatchFrameDev& fr = b.frames[f];
+    const uint32_t n = __builtin_amdgcn_readfirstlane(ll.head[t]);
+    const uint32_t start = __builtin_amdgcn_readfirstlane(b.cursor[t]) - n;
+    uint64_t fx = kEmpty64;  // front slot `lane` (lanes below K), ascending
+    uint32_t fc = 0u;
+    float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
+    uint32_t cnt = 0;
+    auto step = [&](uint64_t x, uint32_t c) {  // FrontK::insert with the composite tail, wave-wide
+      const bool below = lane < (uint32_t)K && fx < x;
+      const uint32_t p = (uint32_t)__builtin_popcountll(__ballot(below));
+      const uint64_t upx = shfl_up_u64(fx, 1u);
+      const uint32_t upc = __shfl_up(fc, 1u);
+      const uint32_t lastc = __shfl(fc, K - 1);
+      const uint32_t evc = p < (uint32_t)K ? lastc : c;
+      if (lane < (uint32_t)K && lane > p) {
+        fx = upx;
+        fc = upc;
+      } else if (lane == p) {
+        fx = x;
+        fc = c;
+      }
+      if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
+      ++cnt;
+    };
+    if (n <= kBinDeepMax) {
+      uint32_t N = 64u;
+      while (N < n) N <<= 1;
+      for (uint32_t i = lane; i < N; i += 64u)
+        key[i] = i < n ? ((uint64_t)ll.next[start + 1u + i] << 32) | i : 0ull;
+      wave_lds_sync();
+      for (uint32_t k = 2u; k <= N; k <<= 1) {  // bitonic sort, descending
+        for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
+          for (uint32_t i = lane; i < N; i += 64u) {
+            const uint32_t l = i ^ j;
+            if (l > i) {
+              const uint64_t x = key[i], y = key[l];
+              const bool swap = (i & k) == 0u ? x < y : x > y;
+              if (swap) {
+                key[i] = y;
+                key[l] = x;
+              }
+            }
+          }
+          wave_lds_sync();
+        }
+      }
+      for (uint32_t j = lane; j < n; j += 64u) kcs[j] = ll.cd[start + 1u + (uint32_t)key[j]];
+      wave_lds_sync();
+      for (uint32_t j = 0; j < n; ++j) {
+        const uint64_t kc = kcs[j];
+        step((kc & 0xFFFFFFFF00000000ull) | (uint32_t)(key[j] >> 32), (uint32_t)kc);
+      }
+      wave_lds_sync();  // the next bucket rewrites key[] and kcs[]
+    } else {
+      uint32_t prev = 0xFFFFFFFFu;
+      for (uint32_t j = 0; j < n; ++j) {
+        uint32_t best = 0u, bi = 0u;
+        for (uint32_t i = lane; i < n; i += 64u) {
+          const uint32_t id1 = ll.next[start + 1u + i];
+          if (id1 < prev && id1 > best) {
+            best = id1;
+            bi = i;
+          }
+        }
+        const uint32_t top = wave_max_u32(best);
+        const uint64_t own = __ballot(best == top && top != 0u);
+        bi = __shfl(bi, (int)__builtin_ctzll(own));
+        prev = top;
+        const uint64_t kc = ll.cd[start + 1u + bi];
+        step((kc & 0xFFFFFFFF00000000ull) | top, (uint32_t)kc);
+      }
+    }
+    const uint32_t m = cnt < (uint32_t)K ? cnt : (uint32_t)K;
+    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t k = 0; k < m; ++k) col = over(col, decode_premult_lds(__shfl(fc, (int)k), s_dec, s_alpha));
+    col = over(col, tailc);
+    for (uint32_t i = lane; i < n; i += 64u) ll.next[start + 1u + i] = kUnset;
+    if (lane == 0u) {
+      ll.head[t] = 0u;
+      fr.out[t - fr.target_base] = over(col, fr.background);
+    }
+  }
+}
+
 template <int kU>
 __global__ __launch_bounds__(kBlock) void k_batch_append(BatchArgs b, LLBuf ll) {
   __shared__ float s_thr[257];
@@ -882,6 +1221,10 @@ __global__ __launch_bounds__(kBlock) void k_batch_append(BatchArgs b, LLBuf ll) 
     f = find_frame<false>(b.frames, b.f0, b.f1, g);
   }
   const BatchFrameDev fr = b.frames[f];
+  if (fr.mode == kModeBin) {  // binned frame: count its targets' fragments
+    bin_count_wave<kU>(fr, g - fr.frag_base, ll);
+    return;
+  }
   AppendSrc a;
   a.target = fr.target;
   a.depth = fr.depth;
@@ -905,6 +1248,17 @@ __global__ __launch_bounds__(kBlock) void k_batch_resolve(BatchArgs b, LLBuf ll)
     const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, (uint64_t)tile * kBlock);
     const BatchFrameDev& fr = b.frames[f];
     const uint32_t t = tile * kBlock + threadIdx.x;
+    if (fr.mode == kModeBin) {  // the whole tile is one frame's: a uniform branch
+      const uint32_t n = ll.head[t];
+      if (n > kBinShallow) {  // for k_bin_deep
+        b.deep_q[atomicAdd(b.deep_n, 1u)] = t;
+        continue;
+      }
+      const float4 bc = bin_replay_shallow<K, kTail>(t, n, b.cursor[t] - n, ll, s_dec, s_alpha);
+      const uint32_t local = t - fr.target_base;
+      if (local < fr.T) fr.out[local] = over(bc, fr.background);
+      continue;
+    }
     uint32_t cnt;
     float4 col;
     if constexpr (kKeys) {
@@ -2470,6 +2824,33 @@ hipError_t launch_batch_append(const BatchArgs& a, const LLBuf& ll, uint32_t chu
   else if (chunks == 4) hipLaunchKernelGGL((k_batch_append<4>), g, b, 0, s, a, ll);
   else return hipErrorInvalidValue;
   return hipGetLastError();
+}
+
+hipError_t launch_bin_prepare(const BatchArgs& a, const LLBuf& ll, uint32_t tiles, uint32_t chunks, hipStream_t s) {
+  if (tiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_bin_tile_sums, dim3(tiles), dim3(kBlock), 0, s, a, ll.head);
+  hipLaunchKernelGGL(k_bin_frame_scan, dim3(a.f1 - a.f0), dim3(1024), 0, s, a);
+  hipLaunchKernelGGL(k_bin_tile_apply, dim3(tiles), dim3(kBlock), 0, s, a, ll.head);
+  if (a.units) {
+    const dim3 g((a.units + kBlock / 64 - 1) / (kBlock / 64)), b(kBlock);
+    if (chunks == 8) hipLaunchKernelGGL((k_bin_scatter<8>), g, b, 0, s, a, ll);
+    else hipLaunchKernelGGL((k_bin_scatter<4>), g, b, 0, s, a, ll);
+  }
+  return hipGetLastError();
+}
+
+template <int K>
+struct BinDeep {
+  static hipError_t run(const BatchArgs& a, const LLBuf& ll, bool tail, hipStream_t s) {
+    const dim3 g(2048), b(64 * kBinDeepWaves);
+    if (tail) hipLaunchKernelGGL((k_bin_deep<K, true>), g, b, 0, s, a, ll);
+    else hipLaunchKernelGGL((k_bin_deep<K, false>), g, b, 0, s, a, ll);
+    return hipGetLastError();
+  }
+};
+
+hipError_t launch_bin_deep(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, hipStream_t s) {
+  return dispatch_k<BinDeep>(K, a, ll, tail, s);
 }
 
 hipError_t launch_batch_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, bool pair, bool keys,

@@ -91,7 +91,12 @@ struct BatchFrameDev {
   uint32_t T;              // targets
   uint32_t target_base;    // head index of target 0 (multiple of 256)
   uint32_t frag_base;      // fragment 0 is pool node frag_base + 1 (multiple of 512)
+  uint32_t mode;           // kModeList or kModeBin
+  uint32_t pad[3];
 };
+constexpr uint32_t kModeList = 0;  // a Linked List per target (append, list walk)
+constexpr uint32_t kModeBin = 1;   // a bucket per target (count, scan, scatter, ordered replay)
+constexpr uint32_t kBatchMaxGroups = 16;  // (K, tail) groups: 6 layer counts x 2
 
 struct BatchArgs {
   const BatchFrameDev* frames;  // device table, bases ascending
@@ -106,6 +111,12 @@ struct BatchArgs {
   // round the (f1 + 1) prefix offsets of the frames within it
   const uint32_t* sched = nullptr;
   uint32_t rounds = 0;
+  // binned frames: per target the bucket cursor, per tile the scan base,
+  // and the queue of buckets too large for one lane (with its length)
+  uint32_t* cursor = nullptr;
+  uint32_t* tile_sum = nullptr;
+  uint32_t* deep_q = nullptr;
+  uint32_t* deep_n = nullptr;
 };
 constexpr uint32_t kBatchSlice = 512;  // fragments of a schedule slice (the frames' alignment)
 
@@ -116,6 +127,12 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_batch_append(const BatchArgs& a, const LLBuf& ll, uint32_t chunks, hipStream_t s);
 hipError_t launch_batch_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, bool pair, bool keys,
                                 uint32_t grid_cap, hipStream_t s);
+// Binned frames after the append launch (which counted them): the scan of
+// the counts into bucket cursors over `tiles` tiles from a.tile0 and the
+// frame table's frames, then the scatter pass over a.units units.
+hipError_t launch_bin_prepare(const BatchArgs& a, const LLBuf& ll, uint32_t tiles, uint32_t chunks, hipStream_t s);
+// The buckets of one group too large for one lane (after its resolve).
+hipError_t launch_bin_deep(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, hipStream_t s);
 
 hipError_t launch_ll_store(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s);
 hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, hipStream_t s);
