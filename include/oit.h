@@ -345,7 +345,8 @@ typedef struct oit_batch_tuning {
   float    bin_density;       /* frames with at least this many fragments per target are binned: a
                                  counting sort gathers each target's fragments into one bucket and the
                                  composite replays it in descending id order (sorted in LDS), in place of
-                                 the list append and walk; the same image. 0 = every frame, default 8 */
+                                 the list append and walk; the same image. 0 = every frame; default 3e38
+                                 (none), which oit_batch_tune may lower */
 } oit_batch_tuning;
 
 typedef struct oit_batch_sizes {

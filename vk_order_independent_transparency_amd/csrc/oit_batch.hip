@@ -33,11 +33,14 @@ constexpr uint32_t kTile = 256;        // resolve tile (one block of targets)
 constexpr uint32_t kFragAlign = 512;   // append unit at 8 chunks per wave (one schedule slice)
 constexpr uint32_t kRounds = 1024;     // rounds of the interleaved append schedule
 constexpr uint64_t kMaxPoolSlots = 0xFFFFF000ull;  // node indices stay below kUnset
-constexpr uint64_t kMaxHeads = 0xFFFFFF00ull;
+constexpr uint64_t kMaxHeads = 0xFFFF0000ull;  // the head array, rounded up to 2^16, stays below 2^32
 constexpr uint64_t kMaxFrameFragments = 0x7FFFFFFEull;
 constexpr int kSide = 3;               // forked streams for concurrent groups
 constexpr uint32_t kDefaultBlocks = 8192;
-constexpr float kDefaultBinDensity = 8.f;  // fragments per target from which a frame is binned
+// fragments per target from which a frame is binned: none by default (on
+// the 64-frame mix every binned frame ran slower than its lists; the tuner
+// sweeps the threshold)
+constexpr float kDefaultBinDensity = 3.0e38f;
 
 int st(hipError_t e) {
   if (e == hipSuccess) return OIT_OK;
@@ -74,7 +77,7 @@ uint64_t round_up(uint64_t v, uint64_t a) { return (v + a - 1) / a * a; }
 
 struct Group {
   uint32_t K, tail, f0, f1, tile0, tiles;
-  bool bin;  // holds binned frames (its deep buckets need k_bin_deep)
+  bool bin;  // holds binned frames
 };
 
 struct Plan {
@@ -413,6 +416,7 @@ BatchArgs args_for(const oit_batch* b, const Plan& p) {
   a.tile_sum = b->tile_sum;
   a.deep_q = b->deep_q;
   a.deep_n = b->deep_n;
+  a.cursor_cap = (uint32_t)b->cap_heads;
   return a;
 }
 
@@ -441,11 +445,10 @@ int launch_group(const oit_batch* b, const Plan& p, const Group& g, uint32_t gi,
   a.f1 = g.f1;
   a.tile0 = g.tile0;
   a.tiles = g.tiles;
-  a.deep_q = b->deep_q + (uint64_t)g.tile0 * kTile;  // a group queues at most its own targets
-  a.deep_n = b->deep_n + gi;
+  (void)gi;
   int rc = st(oit::launch_batch_resolve(a, pool_view(b), g.K, g.tail != 0, !(variant & OIT_VARIANT_LL_SINGLE_WALK),
                                         (variant & OIT_VARIANT_LL_KEYS_ONLY) != 0, blocks, s));
-  if (!rc && g.bin) rc = st(oit::launch_bin_deep(a, pool_view(b), g.K, g.tail != 0, s));
+  if (!rc && g.bin) rc = st(oit::launch_bin_resolve(a, pool_view(b), g.K, g.tail != 0, blocks, s));
   return rc;
 }
 

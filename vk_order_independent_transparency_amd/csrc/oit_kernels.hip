@@ -851,24 +851,35 @@ __device__ __forceinline__ uint32_t find_frame(const BatchFrameDev* fr, uint32_t
 // Binned batch frames (BatchFrameDev::mode == kModeBin): the fragments of
 // each target are gathered into one bucket of the pool by a counting sort
 // (count per target, exclusive scan, scatter) instead of being linked into a
-// list, and the composite orders a bucket by fragment id, newest first (the
-// walk order of the Linked List's descending lists), and replays the Linked
-// List composite (oitLinkedList.frag.glsl:116-172) over it: the image is the
-// single-frame Linked List's bit for bit. There is no ordered insert to walk
-// past concurrent inserts of the same pixel and no chain of dependent loads
-// per list, which is what deep pixels (hundreds of layers) cost the lists.
-// Frame f's buckets fill its own node range: during a run head[t] holds the
-// count of target t, cursor[t] its bucket's end slot after the scatter, and
-// bucket entry s is node s + 1 (node 0 stays the terminator) with
-// cd = (depth key << 32) | colour and next = fragment id + 1. The composite
-// returns head[t] to 0 and the bucket's link words to kUnset, the pool's
-// clean state, so a later run may take the frame either way.
+// list; buckets of more than kBinShallow entries are sorted in place by
+// fragment id, newest first (the walk order of the Linked List's descending
+// lists), by one wave each (a bitonic sort in registers up to 64 entries,
+// in LDS up to kBinDeepMax); the composite replays each bucket newest first
+// through the Linked List composite (oitLinkedList.frag.glsl:116-172), one
+// lane per target, so the image is the single-frame Linked List's bit for
+// bit. A sorted bucket is read with independent loads, where a list walk is
+// a chain of dependent ones. Frame f's buckets fill its own node range:
+// during a run head[t] holds the count of target t, the cursor of t (kept
+// at bin_cursor(t): consecutive targets' cursors in different cache lines,
+// so a wave's atomics on them do not queue on one line) its bucket's first
+// slot before the scatter and its end after, and bucket entry s is node
+// s + 1 (node 0 stays the terminator) with cd = (depth key << 32) | colour
+// and next = fragment id + 1. The composite returns head[t] to 0 and the
+// bucket's link words to kUnset, the pool's clean state, so a later run may
+// take the frame either way.
 // ---------------------------------------------------------------------------
 
-constexpr uint32_t kBinShallow = 8;      // buckets of at most this many entries: one lane replays them
-constexpr uint32_t kBinDeepMax = 1024;   // larger ones: one wave, ordered in LDS up to this size
+constexpr uint32_t kBinShallow = 8;      // buckets a lane orders itself, in registers
+constexpr uint32_t kBinDeepMax = 1024;   // larger ones sorted by a wave; beyond, the lane selects by id
 
-// Count pass of one wave's share (the binned frame's form of the append).
+// Position of target t's cursor: the 64 targets of a block of 64 spread over
+// 64 regions of cap / 64 entries (cap a multiple of 64).
+__device__ __forceinline__ uint32_t bin_cursor(uint32_t t, uint32_t cap) {
+  return (t & 63u) * (cap >> 6) + (t >> 6);
+}
+
+// Count pass of one wave's share (the binned frame's form of the append):
+// returnless atomics, one per run of one target.
 template <int kU>
 __device__ __forceinline__ void bin_count_wave(const BatchFrameDev& fr, uint64_t first, const LLBuf& ll) {
   const uint32_t lane = lane_id();
@@ -882,7 +893,7 @@ __device__ __forceinline__ void bin_count_wave(const BatchFrameDev& fr, uint64_t
   for (int u = 0; u < kU; ++u) {
     const uint32_t tt = t[u] < fr.T ? t[u] : kEmpty;
     const Run r = same_target_run(tt);
-    if (tt != kEmpty && r.end)  // one atomic per run of one target
+    if (tt != kEmpty && r.end)
       __hip_atomic_fetch_add(&ll.head[fr.target_base + tt], r.lane - r.start_lane + 1u, __ATOMIC_RELAXED,
                              __HIP_MEMORY_SCOPE_AGENT);
   }
@@ -915,7 +926,7 @@ __device__ __forceinline__ uint32_t block_scan_u32(uint32_t v, uint32_t& excl, u
 }
 
 // Scan step 1: the sum of each 256-target tile's counts (0 for a Linked
-// List frame's tiles). Block 0 also empties the deep-bucket queues.
+// List frame's tiles). Block 0 also empties the queue of large buckets.
 __global__ __launch_bounds__(kBlock) void k_bin_tile_sums(BatchArgs b, const uint32_t* __restrict__ count) {
   __shared__ uint32_t s_part[kBlock / 64];
   const uint32_t tile = b.tile0 + blockIdx.x;
@@ -925,7 +936,7 @@ __global__ __launch_bounds__(kBlock) void k_bin_tile_sums(BatchArgs b, const uin
   uint32_t excl;
   const uint32_t total = block_scan_u32<kBlock>(v, excl, s_part);
   if (threadIdx.x == 0) b.tile_sum[tile] = total;
-  if (blockIdx.x == 0 && threadIdx.x < kBatchMaxGroups) b.deep_n[threadIdx.x] = 0u;
+  if (blockIdx.x == 0 && threadIdx.x == 0) *b.deep_n = 0u;
 }
 
 // Scan step 2, one block per frame: each tile's first bucket slot, the
@@ -949,16 +960,25 @@ __global__ __launch_bounds__(1024) void k_bin_frame_scan(BatchArgs b) {
   }
 }
 
-// Scan step 3: cursor[t] = the first slot of target t's bucket.
+// Scan step 3: each target's cursor = the first slot of its bucket; targets
+// with more than kBinShallow entries are queued for k_bin_sort.
 __global__ __launch_bounds__(kBlock) void k_bin_tile_apply(BatchArgs b, const uint32_t* __restrict__ count) {
   __shared__ uint32_t s_part[kBlock / 64];
   const uint32_t tile = b.tile0 + blockIdx.x;
   const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, (uint64_t)tile * kBlock);
   if (b.frames[f].mode != kModeBin) return;  // the whole block takes the same branch
-  const uint64_t t = (uint64_t)tile * kBlock + threadIdx.x;
+  const uint32_t t = tile * kBlock + threadIdx.x;
+  const uint32_t n = count[t];
   uint32_t excl;
-  (void)block_scan_u32<kBlock>(count[t], excl, s_part);
-  b.cursor[t] = b.tile_sum[tile] + excl;
+  (void)block_scan_u32<kBlock>(n, excl, s_part);
+  b.cursor[bin_cursor(t, b.cursor_cap)] = b.tile_sum[tile] + excl;
+  // queue entries of the wave from one atomic
+  const bool deep = n > kBinShallow;
+  const uint64_t m = __ballot(deep);
+  uint32_t q = 0;
+  if (lane_id() == 0u && m) q = atomicAdd(b.deep_n, (uint32_t)__builtin_popcountll(m));
+  q = __shfl(q, 0);
+  if (deep) b.deep_q[q + lanes_below(m)] = t;
 }
 
 // Scatter pass: each fragment of a binned frame to the next entry of its
@@ -987,84 +1007,28 @@ __global__ __launch_bounds__(kBlock) void k_bin_scatter(BatchArgs b, LLBuf ll) {
     dep[u] = in ? ld_stream(&fr.depth[i]) : 0.f;
     col[u] = in ? ld_stream(&fr.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
   }
+  uint32_t base[kU];
+  Run r[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {  // every chunk's cursor atomics in flight together
+    const uint32_t tt = t[u] < fr.T ? t[u] : kEmpty;
+    t[u] = tt;
+    r[u] = same_target_run(tt);
+    base[u] = 0;
+    if (tt != kEmpty && r[u].end)
+      base[u] = __hip_atomic_fetch_add(&b.cursor[bin_cursor(fr.target_base + tt, b.cursor_cap)],
+                                       r[u].lane - r[u].start_lane + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     const uint64_t i = first + (uint64_t)u * 64u + lane;
-    const uint32_t tt = t[u] < fr.T ? t[u] : kEmpty;
-    const Run r = same_target_run(tt);
-    uint32_t base = 0;
-    if (tt != kEmpty && r.end)
-      base = __hip_atomic_fetch_add(&b.cursor[fr.target_base + tt], r.lane - r.start_lane + 1u, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-    base = __shfl(base, (int)r.end_lane);  // the run's first slot, from its last lane
-    if (tt != kEmpty) {
-      const uint32_t s = base + (r.lane - r.start_lane) + 1u;  // node index
+    const uint32_t bs = __shfl(base[u], (int)r[u].end_lane);  // the run's first slot, from its last lane
+    if (t[u] != kEmpty) {
+      const uint32_t s = bs + (r[u].lane - r[u].start_lane) + 1u;  // node index
       ll.cd[s] = ((uint64_t)depth_key(dep[u]) << 32) | encode_color_lds(col[u], s_thr);
       ll.next[s] = (uint32_t)i + 1u;
     }
   }
-}
-
-// Composite of a bucket of n <= kBinShallow entries (nodes start + 1 ..
-// start + n) by one lane: the entries in descending id order through the
-// Linked List's front-K insertion and composite tail (ll_walk_core), then
-// the front K blended over the tail; resets the bucket. Lane-local only.
-template <int K, bool kTail>
-__device__ __forceinline__ float4 bin_replay_shallow(uint32_t t, uint32_t n, uint32_t start, const LLBuf& ll,
-                                                     const float* s_dec, const float* s_alpha) {
-  uint32_t ids[kBinShallow];
-  uint64_t kcs[kBinShallow];
-#pragma unroll
-  for (int j = 0; j < (int)kBinShallow; ++j) {
-    ids[j] = 0u;
-    kcs[j] = 0ull;
-    if ((uint32_t)j < n) {
-      ids[j] = ll.next[start + 1u + j];
-      kcs[j] = ll.cd[start + 1u + j];
-    }
-  }
-  FrontK<K> a;
-  a.init();
-  float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
-  uint32_t prev = 0xFFFFFFFFu;
-#pragma unroll
-  for (int s = 0; s < (int)kBinShallow; ++s) {
-    if ((uint32_t)s < n) {
-      uint32_t best = 0u;
-      uint64_t bkc = 0ull;
-#pragma unroll
-      for (int j = 0; j < (int)kBinShallow; ++j) {  // the newest entry not replayed yet
-        const bool take = ids[j] < prev && ids[j] > best;
-        best = take ? ids[j] : best;
-        bkc = take ? kcs[j] : bkc;
-      }
-      prev = best;
-      uint64_t ev;
-      uint32_t evc;
-      a.insert_bounded((bkc & 0xFFFFFFFF00000000ull) | best, (uint32_t)bkc, ev, evc, s + 1);
-      if (kTail && (uint32_t)s >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < (int)kBinShallow; ++j)
-    if ((uint32_t)j < n) ll.next[start + 1u + j] = kUnset;
-  if (n) ll.head[t] = 0u;
-  const uint32_t m = n < (uint32_t)K ? n : (uint32_t)K;
-  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-#pragma unroll
-  for (int k = 0; k < K; ++k)
-    if ((uint32_t)k < m) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
-  return over(col, tailc);
-}
-
-// Lane-wise 64-bit shuffles built from 32-bit ones.
-__device__ __forceinline__ uint64_t shfl_u64(uint64_t v, int src) {
-  const uint32_t lo = __shfl((uint32_t)v, src), hi = __shfl((uint32_t)(v >> 32), src);
-  return ((uint64_t)hi << 32) | lo;
-}
-__device__ __forceinline__ uint64_t shfl_up_u64(uint64_t v, unsigned d) {
-  const uint32_t lo = __shfl_up((uint32_t)v, d), hi = __shfl_up((uint32_t)(v >> 32), d);
-  return ((uint64_t)hi << 32) | lo;
 }
 
 // Orders LDS ops of one wave: every lane's earlier LDS access has completed
@@ -1074,71 +1038,72 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_wave_barrier();
 }
 
-// Buckets of more than kBinShallow entries, one wave each, from the group's
-// queue: ids and entry positions into LDS, a bitonic sort by id (newest
-// first), the entries' (depth key, colour) gathered in that order, then the
-// front-K insertion with lane k holding front slot k (each entry: one ballot
-// for its position, one shift by lane shuffles), the composite tail folded
-// in the same order, and the composite. A bucket past kBinDeepMax entries
-// is replayed by repeated selection of the newest entry not replayed yet,
-// straight from the pool. Two waves per block: 32 KB of LDS each.
-constexpr uint32_t kBinDeepWaves = 2;
+// Every global load the lane issued has returned, and no memory access is
+// moved across (the lanes of a wave then rewrite what the others read).
+__device__ __forceinline__ void loads_done() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
 
-template <int K, bool kTail>
-__global__ __launch_bounds__(64 * kBinDeepWaves) void k_bin_deep(BatchArgs b, LLBuf ll) {
-  __shared__ float s_dec[256];
-  __shared__ float s_alpha[256];
-  __shared__ uint64_t s_key[kBinDeepWaves][kBinDeepMax];  // (id + 1) << 32 | entry
-  __shared__ uint64_t s_kc[kBinDeepWaves][kBinDeepMax];   // (depth key << 32) | colour, in sorted order
-  for (uint32_t k = threadIdx.x; k < 256u; k += 64u * kBinDeepWaves) {
-    s_dec[k] = b.tab->dec[k];
-    s_alpha[k] = b.tab->alpha[k];
-  }
-  __syncthreads();
+// Lane-wise 64-bit shuffle built from 32-bit ones.
+__device__ __forceinline__ uint64_t shfl_xor_u64(uint64_t v, int m) {
+  const uint32_t lo = __shfl_xor((uint32_t)v, m), hi = __shfl_xor((uint32_t)(v >> 32), m);
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// The queued buckets (more than kBinShallow entries), one wave each, sorted
+// in place by id, newest first: up to 64 entries by a bitonic network across
+// the lanes (one entry per lane, the colour word carried along), up to
+// kBinDeepMax through LDS (id and entry index sorted, then the colour words
+// gathered in that order); every entry is read before any is rewritten.
+// Larger buckets stay as they are (the composite selects by id).
+constexpr uint32_t kBinSortWaves = kBlock / 64;
+
+__global__ __launch_bounds__(kBlock) void k_bin_sort(BatchArgs b, LLBuf ll) {
+  __shared__ uint64_t s_key[kBinSortWaves][kBinDeepMax];  // (id + 1) << 32 | entry
   const uint32_t w = threadIdx.x >> 6, lane = lane_id();
   uint64_t* key = s_key[w];
-  uint64_t* kcs = s_kc[w];
   const uint32_t total = __builtin_amdgcn_readfirstlane(*b.deep_n);
-  for (uint32_t e = blockIdx.x * kBinDeepWaves + w; e < total; e += gridDim.x * kBinDeepWaves) {
+  for (uint32_t e = blockIdx.x * kBinSortWaves + w; e < total; e += gridDim.x * kBinSortWaves) {
     const uint32_t t = __builtin_amdgcn_readfirstlane(b.deep_q[e]);
-    const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, t);
-    const BatchFrameDev& fr = b.frames[f];
     const uint32_t n = __builtin_amdgcn_readfirstlane(ll.head[t]);
-    const uint32_t start = __builtin_amdgcn_readfirstlane(b.cursor[t]) - n;
-    uint64_t fx = kEmpty64;  // front slot `lane` (lanes below K), ascending
-    uint32_t fc = 0u;
-    float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
-    uint32_t cnt = 0;
-    auto step = [&](uint64_t x, uint32_t c) {  // FrontK::insert with the composite tail, wave-wide
-      const bool below = lane < (uint32_t)K && fx < x;
-      const uint32_t p = (uint32_t)__builtin_popcountll(__ballot(below));
-      const uint64_t upx = shfl_up_u64(fx, 1u);
-      const uint32_t upc = __shfl_up(fc, 1u);
-      const uint32_t lastc = __shfl(fc, K - 1);
-      const uint32_t evc = p < (uint32_t)K ? lastc : c;
-      if (lane < (uint32_t)K && lane > p) {
-        fx = upx;
-        fc = upc;
-      } else if (lane == p) {
-        fx = x;
-        fc = c;
+    const uint32_t start = __builtin_amdgcn_readfirstlane(b.cursor[bin_cursor(t, b.cursor_cap)]) - n;
+    uint32_t* nx = ll.next + start + 1u;
+    uint64_t* cd = ll.cd + start + 1u;
+    if (n <= 64u) {
+      uint64_t k = lane < n ? ((uint64_t)nx[lane] << 32) | lane : 0ull;
+      uint64_t c = lane < n ? cd[lane] : 0ull;
+      for (uint32_t sz = 2u; sz <= 64u; sz <<= 1) {  // bitonic across the lanes, descending
+        for (uint32_t j = sz >> 1; j > 0u; j >>= 1) {
+          const uint64_t ok = shfl_xor_u64(k, (int)j);
+          const uint64_t oc = shfl_xor_u64(c, (int)j);
+          const bool lower = (lane & j) == 0u;           // this lane holds the pair's first position
+          const bool desc = (lane & sz) == 0u;           // this pair's block runs descending
+          const bool take = lower == desc ? ok > k : ok < k;
+          if (take) {
+            k = ok;
+            c = oc;
+          }
+        }
       }
-      if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
-      ++cnt;
-    };
-    if (n <= kBinDeepMax) {
-      uint32_t N = 64u;
+      loads_done();
+      if (lane < n) {
+        nx[lane] = (uint32_t)(k >> 32);
+        cd[lane] = c;
+      }
+    } else if (n <= kBinDeepMax) {
+      uint32_t N = 128u;
       while (N < n) N <<= 1;
-      for (uint32_t i = lane; i < N; i += 64u)
-        key[i] = i < n ? ((uint64_t)ll.next[start + 1u + i] << 32) | i : 0ull;
+      for (uint32_t i = lane; i < N; i += 64u) key[i] = i < n ? ((uint64_t)nx[i] << 32) | i : 0ull;
       wave_lds_sync();
-      for (uint32_t k = 2u; k <= N; k <<= 1) {  // bitonic sort, descending
-        for (uint32_t j = k >> 1; j > 0u; j >>= 1) {
+      for (uint32_t sz = 2u; sz <= N; sz <<= 1) {  // bitonic sort, descending
+        for (uint32_t j = sz >> 1; j > 0u; j >>= 1) {
           for (uint32_t i = lane; i < N; i += 64u) {
             const uint32_t l = i ^ j;
             if (l > i) {
               const uint64_t x = key[i], y = key[l];
-              const bool swap = (i & k) == 0u ? x < y : x > y;
+              const bool swap = (i & sz) == 0u ? x < y : x > y;
               if (swap) {
                 key[i] = y;
                 key[l] = x;
@@ -1148,42 +1113,112 @@ __global__ __launch_bounds__(64 * kBinDeepWaves) void k_bin_deep(BatchArgs b, LL
           wave_lds_sync();
         }
       }
-      for (uint32_t j = lane; j < n; j += 64u) kcs[j] = ll.cd[start + 1u + (uint32_t)key[j]];
-      wave_lds_sync();
-      for (uint32_t j = 0; j < n; ++j) {
-        const uint64_t kc = kcs[j];
-        step((kc & 0xFFFFFFFF00000000ull) | (uint32_t)(key[j] >> 32), (uint32_t)kc);
+      constexpr uint32_t kPer = kBinDeepMax / 64u;  // sorted entries per lane
+      uint64_t c[kPer];
+#pragma unroll
+      for (uint32_t m = 0; m < kPer; ++m) {
+        const uint32_t j = lane + 64u * m;
+        c[m] = j < n ? cd[(uint32_t)key[j]] : 0ull;
       }
-      wave_lds_sync();  // the next bucket rewrites key[] and kcs[]
-    } else {
-      uint32_t prev = 0xFFFFFFFFu;
-      for (uint32_t j = 0; j < n; ++j) {
-        uint32_t best = 0u, bi = 0u;
-        for (uint32_t i = lane; i < n; i += 64u) {
-          const uint32_t id1 = ll.next[start + 1u + i];
-          if (id1 < prev && id1 > best) {
-            best = id1;
-            bi = i;
-          }
+      loads_done();
+#pragma unroll
+      for (uint32_t m = 0; m < kPer; ++m) {
+        const uint32_t j = lane + 64u * m;
+        if (j < n) {
+          nx[j] = (uint32_t)(key[j] >> 32);
+          cd[j] = c[m];
         }
-        const uint32_t top = wave_max_u32(best);
-        const uint64_t own = __ballot(best == top && top != 0u);
-        bi = __shfl(bi, (int)__builtin_ctzll(own));
-        prev = top;
-        const uint64_t kc = ll.cd[start + 1u + bi];
-        step((kc & 0xFFFFFFFF00000000ull) | top, (uint32_t)kc);
       }
-    }
-    const uint32_t m = cnt < (uint32_t)K ? cnt : (uint32_t)K;
-    float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-    for (uint32_t k = 0; k < m; ++k) col = over(col, decode_premult_lds(__shfl(fc, (int)k), s_dec, s_alpha));
-    col = over(col, tailc);
-    for (uint32_t i = lane; i < n; i += 64u) ll.next[start + 1u + i] = kUnset;
-    if (lane == 0u) {
-      ll.head[t] = 0u;
-      fr.out[t - fr.target_base] = over(col, fr.background);
+      wave_lds_sync();  // the next bucket rewrites key[]
     }
   }
+}
+
+// Composite of target t's bucket (n entries, nodes start + 1 .. start + n)
+// by one lane: the entries newest first through the Linked List's front-K
+// insertion and composite tail (ll_walk_core), then the front K blended
+// over the tail; resets the bucket. Up to kBinShallow entries are ordered
+// here in registers; up to kBinDeepMax k_bin_sort has ordered them; beyond,
+// each step selects the newest entry not replayed yet. Lane-local only.
+template <int K, bool kTail>
+__device__ __forceinline__ float4 bin_replay(uint32_t t, uint32_t n, uint32_t start, const LLBuf& ll,
+                                             const float* s_dec, const float* s_alpha) {
+  const uint32_t* nx = ll.next + start + 1u;
+  const uint64_t* cd = ll.cd + start + 1u;
+  FrontK<K> a;
+  a.init();
+  float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
+  auto take = [&](uint32_t s, uint32_t id1, uint64_t kc) {
+    uint64_t ev;
+    uint32_t evc;
+    a.insert((kc & 0xFFFFFFFF00000000ull) | id1, (uint32_t)kc, ev, evc);
+    if (kTail && s >= (uint32_t)K) tailc = over(tailc, decode_premult_lds(evc, s_dec, s_alpha));
+  };
+  if (n <= kBinShallow) {
+    uint32_t ids[kBinShallow];
+    uint64_t kcs[kBinShallow];
+#pragma unroll
+    for (int j = 0; j < (int)kBinShallow; ++j) {
+      ids[j] = 0u;
+      kcs[j] = 0ull;
+      if ((uint32_t)j < n) {
+        ids[j] = nx[j];
+        kcs[j] = cd[j];
+      }
+    }
+    uint32_t prev = 0xFFFFFFFFu;
+#pragma unroll
+    for (int s = 0; s < (int)kBinShallow; ++s) {
+      if ((uint32_t)s < n) {
+        uint32_t best = 0u;
+        uint64_t bkc = 0ull;
+#pragma unroll
+        for (int j = 0; j < (int)kBinShallow; ++j) {  // the newest entry not replayed yet
+          const bool sel = ids[j] < prev && ids[j] > best;
+          best = sel ? ids[j] : best;
+          bkc = sel ? kcs[j] : bkc;
+        }
+        prev = best;
+        take((uint32_t)s, best, bkc);
+      }
+    }
+  } else if (n <= kBinDeepMax) {
+    uint32_t s = 0;
+    for (; s + 4u <= n; s += 4u) {  // sorted: four independent entries per trip
+      uint32_t id1[4];
+      uint64_t kc[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        id1[q] = nx[s + q];
+        kc[q] = cd[s + q];
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) take(s + q, id1[q], kc[q]);
+    }
+    for (; s < n; ++s) take(s, nx[s], cd[s]);
+  } else {
+    uint32_t prev = 0xFFFFFFFFu;
+    for (uint32_t s = 0; s < n; ++s) {
+      uint32_t best = 0u, bi = 0u;
+      for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t id1 = nx[j];
+        if (id1 < prev && id1 > best) {
+          best = id1;
+          bi = j;
+        }
+      }
+      prev = best;
+      take(s, best, cd[bi]);
+    }
+  }
+  for (uint32_t j = 0; j < n; ++j) ll.next[start + 1u + j] = kUnset;
+  if (n) ll.head[t] = 0u;
+  const uint32_t m = n < (uint32_t)K ? n : (uint32_t)K;
+  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((uint32_t)k < m) col = over(col, decode_premult_lds(a.col[k], s_dec, s_alpha));
+  return over(col, tailc);
 }
 
 template <int kU>
@@ -1238,6 +1273,27 @@ __global__ __launch_bounds__(kBlock) void k_batch_append(BatchArgs b, LLBuf ll) 
   ll_append_wave<false, kU>(a, s_thr, ll, TailBuf{});
 }
 
+// Composite of a group's binned frames (k_batch_resolve takes its linked
+// ones): one lane per target, bin_replay.
+template <int K, bool kTail>
+__global__ __launch_bounds__(kBlock) void k_bin_resolve(BatchArgs b, LLBuf ll) {
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, b.tab);
+  __syncthreads();
+  for (uint32_t tile = b.tile0 + blockIdx.x; tile < b.tile0 + b.tiles; tile += gridDim.x) {
+    const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, (uint64_t)tile * kBlock);
+    const BatchFrameDev& fr = b.frames[f];
+    if (fr.mode != kModeBin) continue;  // the whole tile is one frame's
+    const uint32_t t = tile * kBlock + threadIdx.x;
+    const uint32_t n = ll.head[t];
+    const uint32_t start = n ? b.cursor[bin_cursor(t, b.cursor_cap)] - n : 0u;
+    const float4 bc = bin_replay<K, kTail>(t, n, start, ll, s_dec, s_alpha);
+    const uint32_t local = t - fr.target_base;
+    if (local < fr.T) fr.out[local] = over(bc, fr.background);
+  }
+}
+
 template <int K, bool kTail, bool kPair, bool kKeys>
 __global__ __launch_bounds__(kBlock) void k_batch_resolve(BatchArgs b, LLBuf ll) {
   __shared__ float s_dec[256];
@@ -1248,17 +1304,7 @@ __global__ __launch_bounds__(kBlock) void k_batch_resolve(BatchArgs b, LLBuf ll)
     const uint32_t f = find_frame<true>(b.frames, b.f0, b.f1, (uint64_t)tile * kBlock);
     const BatchFrameDev& fr = b.frames[f];
     const uint32_t t = tile * kBlock + threadIdx.x;
-    if (fr.mode == kModeBin) {  // the whole tile is one frame's: a uniform branch
-      const uint32_t n = ll.head[t];
-      if (n > kBinShallow) {  // for k_bin_deep
-        b.deep_q[atomicAdd(b.deep_n, 1u)] = t;
-        continue;
-      }
-      const float4 bc = bin_replay_shallow<K, kTail>(t, n, b.cursor[t] - n, ll, s_dec, s_alpha);
-      const uint32_t local = t - fr.target_base;
-      if (local < fr.T) fr.out[local] = over(bc, fr.background);
-      continue;
-    }
+    if (fr.mode == kModeBin) continue;  // k_bin_resolve's (the whole tile is one frame's)
     uint32_t cnt;
     float4 col;
     if constexpr (kKeys) {
@@ -2836,21 +2882,25 @@ hipError_t launch_bin_prepare(const BatchArgs& a, const LLBuf& ll, uint32_t tile
     if (chunks == 8) hipLaunchKernelGGL((k_bin_scatter<8>), g, b, 0, s, a, ll);
     else hipLaunchKernelGGL((k_bin_scatter<4>), g, b, 0, s, a, ll);
   }
+  hipLaunchKernelGGL(k_bin_sort, dim3(2048), dim3(kBlock), 0, s, a, ll);
   return hipGetLastError();
 }
 
 template <int K>
-struct BinDeep {
-  static hipError_t run(const BatchArgs& a, const LLBuf& ll, bool tail, hipStream_t s) {
-    const dim3 g(2048), b(64 * kBinDeepWaves);
-    if (tail) hipLaunchKernelGGL((k_bin_deep<K, true>), g, b, 0, s, a, ll);
-    else hipLaunchKernelGGL((k_bin_deep<K, false>), g, b, 0, s, a, ll);
+struct BinResolve {
+  static hipError_t run(const BatchArgs& a, const LLBuf& ll, bool tail, uint32_t grid_cap, hipStream_t s) {
+    if (a.tiles == 0) return hipSuccess;
+    uint32_t blocks = a.tiles;
+    if (grid_cap && blocks > grid_cap) blocks = grid_cap;
+    if (tail) hipLaunchKernelGGL((k_bin_resolve<K, true>), dim3(blocks), dim3(kBlock), 0, s, a, ll);
+    else hipLaunchKernelGGL((k_bin_resolve<K, false>), dim3(blocks), dim3(kBlock), 0, s, a, ll);
     return hipGetLastError();
   }
 };
 
-hipError_t launch_bin_deep(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, hipStream_t s) {
-  return dispatch_k<BinDeep>(K, a, ll, tail, s);
+hipError_t launch_bin_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, uint32_t grid_cap,
+                              hipStream_t s) {
+  return dispatch_k<BinResolve>(K, a, ll, tail, grid_cap, s);
 }
 
 hipError_t launch_batch_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, bool pair, bool keys,

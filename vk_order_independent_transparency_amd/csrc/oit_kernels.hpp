@@ -113,10 +113,11 @@ struct BatchArgs {
   uint32_t rounds = 0;
   // binned frames: per target the bucket cursor, per tile the scan base,
   // and the queue of buckets too large for one lane (with its length)
-  uint32_t* cursor = nullptr;
+  uint32_t* cursor = nullptr;    // [cursor_cap], target t's at (t % 64) * (cursor_cap / 64) + t / 64
   uint32_t* tile_sum = nullptr;
   uint32_t* deep_q = nullptr;
   uint32_t* deep_n = nullptr;
+  uint32_t cursor_cap = 0;       // a multiple of 64
 };
 constexpr uint32_t kBatchSlice = 512;  // fragments of a schedule slice (the frames' alignment)
 
@@ -129,10 +130,12 @@ hipError_t launch_batch_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K,
                                 uint32_t grid_cap, hipStream_t s);
 // Binned frames after the append launch (which counted them): the scan of
 // the counts into bucket cursors over `tiles` tiles from a.tile0 and the
-// frame table's frames, then the scatter pass over a.units units.
+// frame table's frames, the scatter pass over a.units units, and the sort
+// of the buckets too large for one lane to order.
 hipError_t launch_bin_prepare(const BatchArgs& a, const LLBuf& ll, uint32_t tiles, uint32_t chunks, hipStream_t s);
-// The buckets of one group too large for one lane (after its resolve).
-hipError_t launch_bin_deep(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, hipStream_t s);
+// Composite of the binned frames among a group's tiles.
+hipError_t launch_bin_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, uint32_t grid_cap,
+                              hipStream_t s);
 
 hipError_t launch_ll_store(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s);
 hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, hipStream_t s);
