@@ -1,8 +1,9 @@
 """Reconfiguration reuses device buffers: a context that serves frames of
 changing size, K and algorithm keeps its allocations while they fit, stays
-bit-exact with the oracle on every frame (the Linked List pool and the
-colour-pass tail records are kept clean across sizes, algorithms and
-frames cleared before their resolve), and oit_trim releases everything."""
+bit-exact with the oracle on every frame (the Linked List pool, the Loop64
+slots and the colour-pass tail queue are kept clean across sizes,
+algorithms and frames cleared before their resolve), and oit_trim releases
+everything."""
 import numpy as np
 import pytest
 
@@ -47,14 +48,25 @@ def test_reconfigure_sequence_stays_exact(oracle):
             (LL, 16, 1, 17, 64, 48, 160, 1.5, False),   # LL again: fresh pool filled clean
             (SPIN, 4, 1, 18, 48, 40, 120, 0, False),
             (WB, 1, 0, 19, 48, 40, 120, 0, False),
-            # tail records: the link empties what it links; an unresolved
-            # frame leaves records that the next Spinlock append must clear
+            # tail queue: every clear empties it with the tail lists; an
+            # unresolved frame leaves records the next frame must not link
             (SPIN, 4, 1, 20, 64, 48, 200, 0, True),
             (SPIN, 8, 1, 21, 48, 40, 160, 0, False),
             (LL, 8, 1, 22, 48, 40, 160, 0.5, True),
             (SPIN, 4, 1, 23, 64, 48, 200, 0, False),
             (LOOP64, 4, 1, 24, 64, 48, 200, 0, True),
             (SPIN, 2, 1, 25, 64, 48, 200, 0, False),
+            # Loop64 slots: the composite empties them for the next frame of
+            # the same shape; a reconfiguration or an unresolved frame refills
+            (LOOP64, 8, 1, 26, 64, 48, 200, 0, False),
+            (LOOP64, 8, 1, 27, 64, 48, 160, 0, False),
+            (LOOP64, 8, 0, 28, 64, 48, 240, 0, False),
+            (LOOP64, 16, 1, 29, 48, 40, 160, 0, False),
+            (LOOP64, 16, 1, 30, 48, 40, 200, 0, True),
+            (LOOP64, 4, 1, 31, 63, 37, 150, 0, False),
+            # targets not a multiple of four: the clear's fills end in single words
+            (SPIN, 4, 1, 32, 63, 37, 150, 0, False),
+            (SPIN, 4, 1, 33, 63, 37, 150, 0, True),
         ]
         resident = []
         for alg, K, tail, seed, w, h, sp, pf, dirty in seq:

@@ -109,6 +109,7 @@ struct oit_ctx {
   uint32_t* kb_id = nullptr;
   // Loop64
   uint64_t* l64 = nullptr;
+  bool l64_clean = false;  // every slot empty (a fill, or the last frame's composite)
   // WBOIT
   float4* wb_accum = nullptr;
   float* wb_reveal = nullptr;
@@ -117,6 +118,7 @@ struct oit_ctx {
   uint32_t* tl_next = nullptr;
   float4* tl_rgba = nullptr;
   uint64_t* tl_rec = nullptr;
+  uint32_t* tl_qn = nullptr;  // entries per tail-queue region [oit::tail_regions(tl_cap)]
   uint32_t tl_cap = 0;
   bool tl_head_valid = false;  // cleared for the current frame
   // frame state: oit_clear opens a frame; the first resolve links the tail
@@ -257,12 +259,14 @@ void free_frame_buffers(oit_ctx* c) {
   dev_free(&c->kb_key);
   dev_free(&c->kb_id);
   dev_free(&c->l64);
+  c->l64_clean = false;
   dev_free(&c->wb_accum);
   dev_free(&c->wb_reveal);
   dev_free(&c->tl_head);
   dev_free(&c->tl_next);
   dev_free(&c->tl_rgba);
   dev_free(&c->tl_rec);
+  dev_free(&c->tl_qn);
   c->tl_cap = 0;
   c->tl_head_valid = false;
   c->ll_cap = 0;
@@ -330,6 +334,7 @@ oit::TailBuf tail_of(const oit_ctx* c) {
   t.rgba = c->tl_rgba;
   t.rec = c->tl_rec;
   t.counter = c->d_counters;
+  t.qn = c->tl_qn;
   t.cap = c->tl_cap;
   t.mask = c->cfg.coverage_samples > 1 ? c->tl_mask : nullptr;
   t.err = dev_err(c);
@@ -355,6 +360,7 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
   }
   if (!c->tl_head_valid) {
     hipError_t e = oit::launch_fill_u32(c->tl_head, c->T, 0u, s);  // empty tail lists
+    if (e == hipSuccess && c->tl_qn) e = oit::launch_fill_u32(c->tl_qn, oit::tail_regions(c->tl_cap), 0u, s);
     if (e != hipSuccess) return status_of(e);
     c->tl_head_valid = true;
   }
@@ -368,17 +374,25 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
   float4* nr = nullptr;
   uint64_t* nrec = nullptr;
   uint32_t* nm = nullptr;
+  uint32_t* nq = nullptr;
+  const uint64_t regions = oit::tail_regions(cap);
   int rc = dev_alloc(&nn, cap);
   if (!rc) rc = dev_alloc(&nr, cap);
   if (!rc) rc = dev_alloc(&nrec, cap);
+  if (!rc) rc = dev_alloc(&nq, regions);
   if (!rc && coverage(c)) rc = dev_alloc(&nm, cap);
   if (rc) {
     dev_free(&nn);
     dev_free(&nr);
     dev_free(&nrec);
+    dev_free(&nq);
     dev_free(&nm);
     return rc;
   }
+  // region counters: this frame's so far, zero beyond
+  hipError_t fe = oit::launch_fill_u32(nq, regions, 0u, s);
+  if (fe == hipSuccess && c->tl_cap && c->tl_qn)
+    fe = hipMemcpyAsync(nq, c->tl_qn, oit::tail_regions(c->tl_cap) * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
   if (c->tl_cap) {
     (void)hipMemcpyAsync(nn, c->tl_next, (uint64_t)c->tl_cap * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     (void)hipMemcpyAsync(nr, c->tl_rgba, (uint64_t)c->tl_cap * sizeof(float4), hipMemcpyDeviceToDevice, s);
@@ -387,16 +401,15 @@ int ensure_tail_capacity(oit_ctx* c, uint64_t need, hipStream_t s) {
       (void)hipMemcpyAsync(nm, c->tl_mask, (uint64_t)c->tl_cap * sizeof(uint32_t), hipMemcpyDeviceToDevice, s);
     (void)hipStreamSynchronize(s);
   }
-  // records this frame has not written read as "not in the tail"
-  const hipError_t fe = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(nrec + c->tl_cap),
-                                             2 * (cap - c->tl_cap), kEmpty, s);
   dev_free(&c->tl_next);
   dev_free(&c->tl_rgba);
   dev_free(&c->tl_rec);
+  dev_free(&c->tl_qn);
   dev_free(&c->tl_mask);
   c->tl_next = nn;
   c->tl_rgba = nr;
   c->tl_rec = nrec;
+  c->tl_qn = nq;
   c->tl_mask = nm;
   c->cap_tl_mask = nm ? cap : 0;
   c->tl_cap = (uint32_t)cap;
@@ -662,20 +675,22 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
     rc = reserve(c, &c->tl_head, &c->cap_tl_head, T);
     if (!rc && cfg->max_fragments && cfg->max_fragments > c->tl_cap) {
       const uint64_t cap = cfg->max_fragments < kMaxFrameFragments ? cfg->max_fragments : kMaxFrameFragments;
-      uint64_t d1 = c->tl_cap, d2 = c->tl_cap, d3 = c->tl_cap;
+      uint64_t d1 = c->tl_cap, d2 = c->tl_cap, d3 = c->tl_cap, d4 = oit::tail_regions(c->tl_cap);
       rc = reserve(c, &c->tl_next, &d1, cap);
       if (!rc) rc = reserve(c, &c->tl_rgba, &d2, cap);
       if (!rc) rc = reserve(c, &c->tl_rec, &d3, cap);
+      if (!rc) rc = reserve(c, &c->tl_qn, &d4, oit::tail_regions(cap));
       if (!rc) c->tl_cap = (uint32_t)cap;
     }
     if (!rc && cov > 1 && c->tl_cap) rc = reserve(c, &c->tl_mask, &c->cap_tl_mask, c->tl_cap);
     if (!rc && cov <= 1) release(c, &c->tl_mask, &c->cap_tl_mask);
   } else if (!rc) {
     release(c, &c->tl_head, &c->cap_tl_head);
-    uint64_t d1 = c->tl_cap, d2 = c->tl_cap, d3 = c->tl_cap;
+    uint64_t d1 = c->tl_cap, d2 = c->tl_cap, d3 = c->tl_cap, d4 = 0;
     release(c, &c->tl_next, &d1);
     release(c, &c->tl_rgba, &d2);
     release(c, &c->tl_rec, &d3);
+    release(c, &c->tl_qn, &d4);
     release(c, &c->tl_mask, &c->cap_tl_mask);
     c->tl_cap = 0;
   }
@@ -684,6 +699,7 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
     return rc;
   }
   c->tl_head_valid = false;
+  c->l64_clean = false;  // slots beyond the last frame's T x K may hold an older frame's keys
   if (pooled(alg) && ll_grown) {
     // a new allocation is filled clean (every head 0, every link kUnset) by
     // the next clear, on its stream
@@ -769,39 +785,47 @@ int oit_clear(oit_ctx* c, void* stream) {
     c->stats_moved = false;
     return OIT_OK;
   }
-  hipError_t e = staged(c, kStClear, s, [&]() -> hipError_t {
-    hipError_t r = hipSuccess;
-    if (need_counters) r = oit::launch_fill_u32(c->d_counters, 16, 0u, s);
-    switch (c->cfg.algorithm) {
-      case OIT_LOOP:  // depths to all ones (oitRender.cpp:242-265), colour slots to 0
-        if (r == hipSuccess) r = oit::launch_fill_u32(c->l32_z, T * K, kEmpty, s);
-        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->l32_c), 2 * T * K, 0u, s);
-        break;
-      case OIT_INTERLOCK:
-      case OIT_LINKEDLIST:  // head and counter to 0, oitRender.cpp:206-212; links of a dirty range to kUnset
-        // every allocated head: a larger earlier frame may have left some
-        if (need_ll_fill && r == hipSuccess) r = oit::launch_fill_u32(c->ll_head, c->cap_ll_head, 0u, s);
-        if (need_ll_fill && r == hipSuccess)
-          r = oit::launch_fill_u32(c->ll_next, c->ll_dirty_upto ? c->ll_dirty_upto : c->ll_alloc, oit::kUnset, s);
-        break;
-      case OIT_SIMPLE:  // count to 0, oitRender.cpp:161-169
-        if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_count, T, 0u, s);
-        break;
-      case OIT_SPINLOCK:  // count, spin to 0, depth to ~0, oitRender.cpp:343-354 (the count is in the lock word)
-        if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_lock, T, 0u, s);
-        if (r == hipSuccess) r = oit::launch_fill_u32(c->kb_depth, T, kEmpty, s);
-        break;
-      case OIT_LOOP64:  // every byte to 0xFF, oitRender.cpp:307
-        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->l64), 2 * T * K, kEmpty, s);
-        break;
-      case OIT_WEIGHTED:  // accum 0, reveal 1 (log2 0), oitRender.cpp:394-397
-        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_accum), 4 * T, 0u, s);
-        if (r == hipSuccess) r = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(c->wb_reveal), T, 0u, s);
-        break;
-    }
-    if (r == hipSuccess && need_tail) r = oit::launch_fill_u32(c->tl_head, T, 0u, s);  // empty tail lists
-    return r;
-  });
+  // every fill of the clear in one launch
+  oit::FillRange fr[8];
+  int nf = 0;
+  auto fill = [&](void* p, uint64_t n, uint32_t v) { fr[nf++] = oit::FillRange{static_cast<uint32_t*>(p), n, v}; };
+  if (need_counters) fill(c->d_counters, 16, 0u);
+  bool l64_fill = false;
+  switch (c->cfg.algorithm) {
+    case OIT_LOOP:  // depths to all ones (oitRender.cpp:242-265), colour slots to 0
+      fill(c->l32_z, T * K, kEmpty);
+      fill(c->l32_c, 2 * T * K, 0u);
+      break;
+    case OIT_INTERLOCK:
+    case OIT_LINKEDLIST:  // head and counter to 0, oitRender.cpp:206-212; links of a dirty range to kUnset
+      // every allocated head: a larger earlier frame may have left some
+      if (need_ll_fill) {
+        fill(c->ll_head, c->cap_ll_head, 0u);
+        fill(c->ll_next, c->ll_dirty_upto ? c->ll_dirty_upto : c->ll_alloc, oit::kUnset);
+      }
+      break;
+    case OIT_SIMPLE:  // count to 0, oitRender.cpp:161-169
+      fill(c->kb_count, T, 0u);
+      break;
+    case OIT_SPINLOCK:  // count, spin to 0, depth to ~0, oitRender.cpp:343-354 (the count is in the lock word)
+      fill(c->kb_lock, T, 0u);
+      fill(c->kb_depth, T, kEmpty);
+      break;
+    case OIT_LOOP64:  // every byte to 0xFF, oitRender.cpp:307; the composite leaves its slots empty
+      l64_fill = !c->l64_clean;
+      if (l64_fill) fill(c->l64, 2 * T * K, kEmpty);
+      break;
+    case OIT_WEIGHTED:  // accum 0, reveal 1 (log2 0), oitRender.cpp:394-397
+      fill(c->wb_accum, 4 * T, 0u);
+      fill(c->wb_reveal, T, 0u);
+      break;
+  }
+  if (need_tail) {
+    fill(c->tl_head, T, 0u);  // empty tail lists
+    if (c->tl_qn) fill(c->tl_qn, oit::tail_regions(c->tl_cap), 0u);  // and an empty tail queue
+  }
+  hipError_t e = staged(c, kStClear, s, [&]() -> hipError_t { return oit::launch_fill_ranges(fr, nf, s); });
+  if (e == hipSuccess && l64_fill) c->l64_clean = true;
   if (e == hipSuccess && ll) {
     c->ll_clean = true;
     c->ll_dirty_upto = 0;
@@ -936,16 +960,13 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       oit::KBuf kb{nullptr, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
       kb.err = dev_err(c);
       kb.mask = coverage(c) ? c->kb_mask : nullptr;
-      // a fragment's record is written by whichever fragment pushes it to the
-      // tail, so this append's records start empty
-      if (tb.head && tb.rec)
-        e = oit::launch_fill_u32(reinterpret_cast<uint32_t*>(tb.rec + c->frame_frags), 2 * f->count, kEmpty, s);
-      if (e == hipSuccess) e = staged(c, kStStore, s, [&] { return oit::launch_spin_store(k, v, kb, tb, s); });
+      e = staged(c, kStStore, s, [&] { return oit::launch_spin_store(k, v, kb, tb, s); });
       break;
     }
     case OIT_LOOP64: {
       const int chunks = (c->cfg.variant & OIT_VARIANT_STORE_CHUNKS_1) ? 1
                          : (c->cfg.variant & OIT_VARIANT_STORE_CHUNKS_4) ? 4 : 2;
+      c->l64_clean = false;
       e = staged(c, kStStore, s, [&] { return oit::launch_loop64_store(k, v, c->l64, chunks, s); });
       if (c->cfg.tail_blend) c->segments.push_back(v);
       break;
@@ -995,7 +1016,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     }
   }
   // Linked List: only fragments whose node would be at or past the pool's
-  // end (id >= cap - 1) can be tails, and only those have records
+  // end (id >= cap - 1) can be tails
   const uint64_t first_tail = alg == OIT_LINKEDLIST ? (c->ll_cap ? c->ll_cap - 1 : 0) : 0;
   const bool no_tails = !tb.next || first_tail >= c->frame_frags;
   if (link_tails && tb.head && !no_tails) {
@@ -1041,6 +1062,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     }
     case OIT_LOOP64:
       e = staged(c, kStResolve, s, [&] { return oit::launch_loop64_resolve(k, c->l64, tbr, o, s); });
+      c->l64_clean = e == hipSuccess;  // the composite emptied every slot it read
       break;
     case OIT_LOOP:
       e = staged(c, kStResolve, s, [&] { return oit::launch_loop32_resolve(k, c->l32_z, c->l32_c, tbr, o, s); });
@@ -1110,6 +1132,7 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
     c->ll_clean = false;
     c->ll_dirty_upto = c->ll_alloc;
     c->counters_clean = false;
+    c->l64_clean = false;
     return OIT_ERR_DEVICE;
   }
   return OIT_OK;

@@ -51,34 +51,60 @@ __device__ __forceinline__ uint32_t tail_index(uint32_t v) { return (v >> 1) - 1
 // values below this have link words in arrays of `cap` fragments
 __device__ __forceinline__ uint32_t tail_value_cap(uint32_t cap) { return (cap + 1u) << 1; }
 
-// Colour-pass tail lists, phase 1: the record of frame-local fragment `lid`
-// is rec[lid] = (target << 32) | tail value, or kEmpty64 when the fragment
-// is not in the tail; a tail fragment also stores its premultiplied colour
-// and a kUnset link word. Records are indexed by fragment id, so no counter
-// is shared between waves.
-// tail_record_own: the lane's own fragment, tail or not (writes the record
-// either way).
-__device__ __forceinline__ void tail_record_own(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
-                                                const TailBuf& tb, uint32_t mask = 0xFFFFFFFFu) {
-  if (lid >= tb.cap) return;
-  tb.rec[lid] = tail ? ((uint64_t)t << 32) | tail_value(lid, flag) : kEmpty64;
+// Colour-pass tail lists, phase 1: a tail fragment `lid` stores its
+// premultiplied colour and a kUnset link word at index lid, and its record
+// (target << 32) | tail value goes to the frame's tail queue. The queue is
+// cut into regions of 1024 fragment ids, each with its own counter, and a
+// record goes to the region of its fragment's id: a fragment enters the
+// tail once, so region r's entries fit in rec[r * 1024 ..], and the link
+// pass, which takes the regions in order, sees the records roughly in id
+// order. The lanes of one region share one atomic. The
+// queue holds only tail fragments, so the link pass reads nothing of the
+// others. Called by every active lane of the wave that may hold a tail
+// fragment.
+__device__ __forceinline__ void tail_queue_push(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
+                                                const TailBuf& tb, uint32_t mask) {
+  tail = tail && lid < tb.cap && tb.qn != nullptr;
   if (tail) {
     tb.rgba[lid] = payload;
     tb.next[lid] = kUnset;
     if (tb.mask) tb.mask[lid] = mask;
   }
+  const uint32_t r = lid >> kTailRegionShift;
+  const uint32_t lane = lane_id();
+  // group the lanes by region (cross-lane work only), then one atomic per
+  // group, all of them in flight together
+  uint64_t rem = __ballot(tail);
+  uint32_t my_leader = 0;
+  uint64_t my_group = 0;
+  while (rem) {
+    const uint32_t leader = (uint32_t)__builtin_ctzll(rem);
+    const uint32_t rl = __builtin_amdgcn_readlane(r, leader);
+    const uint64_t m = __ballot(tail && r == rl);
+    if ((m >> lane) & 1ull) {
+      my_leader = leader;
+      my_group = m;
+    }
+    rem &= ~m;
+  }
+  uint32_t q = 0;
+  if (tail && lane == my_leader) q = atomicAdd(&tb.qn[r], (uint32_t)__builtin_popcountll(my_group));
+  q = __shfl(q, (int)my_leader);
+  if (tail)
+    tb.rec[((uint64_t)r << kTailRegionShift) + q + lanes_below(my_group)] = ((uint64_t)t << 32) | tail_value(lid, flag);
+}
+
+// tail_record_own: the lane's own fragment goes to the tail.
+__device__ __forceinline__ void tail_record_own(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
+                                                const TailBuf& tb, uint32_t mask = 0xFFFFFFFFu) {
+  tail_queue_push(tail, t, lid, flag, payload, tb, mask);
 }
 
 // tail_record_evict: fragment `lid` (the lane's own or one it pushed out of
-// the A-buffer) goes to the tail; records of the range were filled with
-// kEmpty64 before the kernel.
+// the A-buffer) goes to the tail.
 __device__ __forceinline__ void tail_record_evict(bool tail, uint32_t t, uint32_t lid, uint32_t flag,
                                                   float4 payload, const TailBuf& tb, uint32_t mask = 0xFFFFFFFFu) {
-  if (!tail || lid >= tb.cap) return;
-  tb.rec[lid] = ((uint64_t)t << 32) | tail_value(lid, flag);
-  tb.rgba[lid] = payload;
-  tb.next[lid] = kUnset;
-  if (tb.mask) tb.mask[lid] = mask;
+  tail_queue_push(tail, t, lid, flag, payload, tb, mask);
 }
 
 // Sorted register array of K (sort key, colour) pairs, ascending; inserting x
@@ -370,6 +396,34 @@ __global__ __launch_bounds__(kBlock) void k_fill_u32(uint32_t* __restrict__ p, u
   const uint4 v4 = make_uint4(v, v, v, v);
   for (uint64_t i = (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n4; i += stride) p4[i] = v4;
   for (uint64_t i = n4 * 4 + (uint64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) p[i] = v;
+}
+
+// Up to kMaxFillRanges fills in one launch (a frame's clear): 16-byte stores
+// over the ranges' concatenated 16-byte units, the last n mod 4 words of
+// each range by the grid's first threads.
+struct FillArgs {
+  uint32_t* p[kMaxFillRanges];
+  uint64_t n[kMaxFillRanges];
+  uint64_t start[kMaxFillRanges + 1];  // prefix of the ranges' 16-byte units
+  uint32_t v[kMaxFillRanges];
+  int count;
+};
+
+__global__ __launch_bounds__(kBlock) void k_fill_ranges(FillArgs a) {
+  const uint64_t gid = (uint64_t)blockIdx.x * kBlock + threadIdx.x;
+  const uint64_t stride = (uint64_t)gridDim.x * kBlock;
+  if (gid < 4u * (uint64_t)a.count) {
+    const int j = (int)(gid >> 2);
+    const uint64_t e = gid & 3u;
+    if (e < (a.n[j] & 3u)) a.p[j][(a.n[j] & ~3ull) + e] = a.v[j];
+  }
+  const uint64_t total = a.start[a.count];
+  int j = 0;
+  for (uint64_t i = gid; i < total; i += stride) {
+    while (i >= a.start[j + 1]) ++j;  // a thread's units increase, so its range index only grows
+    const uint32_t v = a.v[j];
+    reinterpret_cast<uint4*>(a.p[j])[i - a.start[j]] = make_uint4(v, v, v, v);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -2270,9 +2324,11 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
 }
 
 // Composite (oitLoop64.frag.glsl:162-183): blend slots front to back until
-// the first empty one.
+// the first empty one. The slots it read are set back to empty for the next
+// frame (the occupied ones are a prefix: slots stay sorted, and no key is
+// the empty key), so the next clear need not write the whole A-buffer.
 template <int K, bool kColorTail, bool kDebug>
-__global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint64_t* __restrict__ slots, TailBuf tb,
+__global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, uint64_t* __restrict__ slots, TailBuf tb,
                                                            ResolveOut o) {
   __shared__ float s_dec[256];
   __shared__ float s_alpha[256];
@@ -2301,6 +2357,9 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, const uint6
       ++n;
     }
   }
+#pragma unroll
+  for (int k = 0; k < K; ++k)
+    if ((uint32_t)k < n) slots[(uint64_t)k * c.T + t] = kEmpty64;
   float4 d = dest_of(c, t);
   if (kColorTail) {
     uint32_t skip = 0;
@@ -2407,42 +2466,56 @@ __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, Re
 // arrive roughly in id order, so most inserts succeed at the head.
 // ---------------------------------------------------------------------------
 
-// Each wave takes kU chunks of 64 ids per trip: their records are loaded
-// together and their first head attempts are in flight together; only lanes
-// that find a larger value at the head walk.
+// Work item w of the grid is chunk (w mod 2) of 512 entries of queue
+// region r0 + w / 2, taken in that order, so the frame's records are linked
+// roughly in id order, as they were produced, and most inserts succeed at
+// the head. A wave loads its 8 x 64 records together and has their first
+// head attempts in flight together; only lanes that find a larger value at
+// the head walk. Neighbouring entries of one target with consecutive ids (a
+// sphere's front and back fragment) are chained here, newest to oldest, and
+// published with one insert: no other fragment's id lies between theirs.
 template <int kU>
-__global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t first, uint32_t frags) {
+__global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t r0, uint32_t r1) {
   __shared__ uint32_t s_count[kBlock / 64];
-  const uint32_t n = frags < tb.cap ? frags : tb.cap;
+  constexpr uint32_t kChunk = 64u * kU;
+  constexpr uint32_t kPerRegion = (1u << kTailRegionShift) / kChunk;
   const uint32_t vcap = tail_value_cap(tb.cap);
   const uint32_t lane = lane_id();
-  const uint64_t wave = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
-  const uint64_t step = (uint64_t)gridDim.x * (kBlock / 64) * 64u * kU;
+  const uint64_t items = (uint64_t)(r1 - r0 + 1u) * kPerRegion;
+  const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
   uint32_t mine = 0;
-  // every lane of a wave runs the same number of trips: runs and inserts are wave-wide
-  for (uint64_t base = first + wave * 64u * kU; base < n; base += step) {
+  for (uint64_t w = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < items; w += waves) {
+    const uint32_t reg = r0 + (uint32_t)(w / kPerRegion);
+    const uint32_t base = (uint32_t)(w % kPerRegion) * kChunk;
+    const uint32_t q = __builtin_amdgcn_readfirstlane(tb.qn[reg]);
+    const uint32_t n = q < (1u << kTailRegionShift) ? q : (1u << kTailRegionShift);
+    if (base >= n) continue;  // wave-uniform
+    const uint64_t reg0 = (uint64_t)reg << kTailRegionShift;
     uint64_t rec[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-      const uint64_t lid = base + (uint64_t)u * 64u + lane;
-      rec[u] = lid < n ? ld_stream(&tb.rec[lid]) : kEmpty64;
+      const uint32_t pos = base + (uint32_t)u * 64u + lane;
+      rec[u] = pos < n ? ld_stream(&tb.rec[reg0 + pos]) : kEmpty64;
     }
     uint32_t t[kU], v[kU], lo[kU];
     bool ins[kU];
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
-      const uint32_t lid = (uint32_t)(base + (uint64_t)u * 64u + lane);
       const bool tail = rec[u] != kEmpty64;
       t[u] = tail ? (uint32_t)(rec[u] >> 32) : kEmpty;
       v[u] = (uint32_t)rec[u];
       mine += tail ? 1u : 0u;
-      // consecutive ids of one target: chained newest to oldest here, then
-      // published with one insert (no other value of the list lies between)
-      const Run r = same_target_run(t[u]);
+      const uint32_t id1 = v[u] >> 1;  // id + 1
+      const uint32_t tp = __shfl_up(t[u], 1u), tn = __shfl_down(t[u], 1u);
+      const uint32_t ip = __shfl_up(id1, 1u), in = __shfl_down(id1, 1u);
       const uint32_t v_prev = __shfl_up(v[u], 1u);
-      lo[u] = __shfl(v[u], (int)r.start_lane);
-      if (tail && !r.start) st_relaxed(&tb.next[lid], v_prev);
-      ins[u] = tail && r.end;
+      const bool start = lane == 0u || tp != t[u] || ip + 1u != id1;
+      const bool end = lane == 63u || tn != t[u] || id1 + 1u != in;
+      const uint64_t starts = __ballot(start);
+      const uint32_t start_lane = 63u - (uint32_t)__builtin_clzll(starts & ((2ull << lane) - 1ull));
+      lo[u] = __shfl(v[u], (int)start_lane);
+      if (tail && !start) st_relaxed(&tb.next[tail_index(v[u])], v_prev);
+      ins[u] = tail && end;
     }
     uint32_t old[kU];
 #pragma unroll
@@ -2667,8 +2740,7 @@ struct Loop64Classify {
 
 template <int K>
 struct Loop64Resolve {
-  static hipError_t run(const Common& c, const uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
-                        hipStream_t s) {
+  static hipError_t run(const Common& c, uint64_t* slots, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
     const bool dbg = o.fk_key || o.fk_val || o.counts;
     const bool ctail = c.tail && tb.head != nullptr;
     const dim3 g(blocks_for(c.T)), b(kBlock);
@@ -2736,6 +2808,33 @@ hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
   uint64_t want = (n / 4 + kBlock - 1) / kBlock;
   const uint32_t g = (uint32_t)(want < 4096 ? (want ? want : 1) : 4096);
   hipLaunchKernelGGL(k_fill_u32, dim3(g), dim3(kBlock), 0, s, p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_ranges(const FillRange* r, int count, hipStream_t s) {
+  FillArgs a{};
+  int m = 0;
+  a.start[0] = 0;
+  for (int i = 0; i < count; ++i) {
+    if (r[i].n == 0) continue;
+    if (m == kMaxFillRanges || (reinterpret_cast<uintptr_t>(r[i].p) & 15u)) {  // not packable: a launch of its own
+      const hipError_t e = launch_fill_u32(r[i].p, r[i].n, r[i].v, s);
+      if (e != hipSuccess) return e;
+      continue;
+    }
+    a.p[m] = r[i].p;
+    a.n[m] = r[i].n;
+    a.v[m] = r[i].v;
+    a.start[m + 1] = a.start[m] + r[i].n / 4;
+    ++m;
+  }
+  if (m == 0) return hipSuccess;
+  a.count = m;
+  for (int i = m + 1; i <= kMaxFillRanges; ++i) a.start[i] = a.start[m];
+  uint64_t want = (a.start[m] + kBlock - 1) / kBlock;
+  if (want < 1) want = 1;  // the remainders
+  const uint32_t g = (uint32_t)(want < 4096 ? want : 4096);
+  hipLaunchKernelGGL(k_fill_ranges, dim3(g), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 
@@ -2807,7 +2906,7 @@ hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint
   return dispatch_k<Loop64Classify>(c.K, c, f, slots, tb, s);
 }
 
-hipError_t launch_loop64_resolve(const Common& c, const uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
+hipError_t launch_loop64_resolve(const Common& c, uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
                                  hipStream_t s) {
   return dispatch_k<Loop64Resolve>(c.K, c, slots, tb, o, s);
 }
@@ -2831,11 +2930,14 @@ hipError_t launch_codec_selftest(const CodecTables* tab, uint32_t lo, uint32_t h
 }
 
 hipError_t launch_tail_link(const TailBuf& tb, uint32_t first, uint32_t frags, hipStream_t s) {
-  const uint64_t n = frags < tb.cap ? frags : tb.cap;
-  if (first >= n) return hipSuccess;
-  uint64_t blocks = (n - first + 8u * kBlock - 1) / (8u * kBlock);  // eight 64-id chunks per wave and trip
-  if (blocks > 4096u) blocks = 4096u;
-  hipLaunchKernelGGL(k_tail_link<8>, dim3((uint32_t)blocks), dim3(kBlock), 0, s, tb, first, frags);
+  const uint32_t n = frags < tb.cap ? frags : tb.cap;
+  if (first >= n || !tb.qn) return hipSuccess;
+  // a wave per 512 entries of a region of 1024 ids
+  const uint32_t r0 = first >> kTailRegionShift, r1 = (n - 1u) >> kTailRegionShift;
+  constexpr uint32_t kPerRegion = (1u << kTailRegionShift) / 512u;
+  uint64_t blocks = ((uint64_t)(r1 - r0 + 1u) * kPerRegion + kBlock / 64 - 1) / (kBlock / 64);
+  if (blocks > 8192u) blocks = 8192u;
+  hipLaunchKernelGGL(k_tail_link<8>, dim3((uint32_t)blocks), dim3(kBlock), 0, s, tb, r0, r1);
   return hipGetLastError();
 }
 

@@ -19,12 +19,17 @@ struct FragView {
 };
 
 // Colour-pass tail lists (per target, descending fragment id: newest first).
+// Tail queue regions: 1024 frame-local fragment ids each.
+constexpr uint32_t kTailRegionShift = 10;
+inline uint64_t tail_regions(uint64_t cap) { return (cap + (1u << kTailRegionShift) - 1) >> kTailRegionShift; }
+
 struct TailBuf {
   uint32_t* head;     // [T] newest tail value, 0 = empty
   uint32_t* next;     // [Fcap] link words, indexed by frame-local id
   float4* rgba;       // [Fcap] premultiplied colour the tail blend uses
-  uint64_t* rec;      // [Fcap] (target << 32) | ((id + 1) << 1 | boundary), kEmpty64 = not in the tail
+  uint64_t* rec;      // [Fcap] tail queue: (target << 32) | ((id + 1) << 1 | boundary), by region of 1024 ids
   uint32_t* counter;  // tail fragments of the frame (counted by the link kernel)
+  uint32_t* qn;       // [tail_regions(Fcap)] entries of each region, zero at the frame's start
   uint32_t cap;       // Fcap
   uint32_t* mask = nullptr;  // [Fcap] coverage shading: samples the ROP blend writes
   uint32_t* err = nullptr;   // device error word (kErr* bits), null: none
@@ -125,6 +130,15 @@ constexpr uint32_t kBatchSlice = 512;  // fragments of a schedule slice (the fra
 void build_codec_tables(CodecTables* t);
 
 hipError_t launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
+// Several fills in one launch (ranges at 16-byte aligned addresses; others
+// are filled by launches of their own).
+struct FillRange {
+  uint32_t* p;
+  uint64_t n;
+  uint32_t v;
+};
+constexpr int kMaxFillRanges = 6;
+hipError_t launch_fill_ranges(const FillRange* r, int count, hipStream_t s);
 hipError_t launch_batch_append(const BatchArgs& a, const LLBuf& ll, uint32_t chunks, hipStream_t s);
 hipError_t launch_batch_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, bool tail, bool pair, bool keys,
                                 uint32_t grid_cap, hipStream_t s);
@@ -160,7 +174,8 @@ hipError_t launch_kbuf_resolve(const Common& c, const KBuf& kb, const TailBuf& t
 hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, int chunks, hipStream_t s);
 hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb,
                                   hipStream_t s);
-hipError_t launch_loop64_resolve(const Common& c, const uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
+// Also sets the slots it reads back to empty.
+hipError_t launch_loop64_resolve(const Common& c, uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
                                  hipStream_t s);
 
 hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& wb, float view_near, float view_span,
@@ -168,6 +183,8 @@ hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& w
 hipError_t launch_wboit_resolve(const Common& c, const WBBuf& wb, const ResolveOut& o, hipStream_t s);
 
 // Links the records of ids first .. frags - 1 (the ones that can be tails).
+// Links the tail queue's records of frame-local ids [first, frags) into the
+// per-target lists.
 hipError_t launch_tail_link(const TailBuf& tb, uint32_t first, uint32_t frags, hipStream_t s);
 // coverage shading (Common::cov samples per pixel, masks in the buffers)
 hipError_t launch_ll_resolve_cov(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o,
