@@ -149,13 +149,15 @@ ALGORITHM_ROWS = ((1, "linkedlist", 8), (1, "linkedlist", 16), (3, "loop64", 8),
                   (5, "interlock", 16))
 
 
-def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream, only=None, variant=0):
+def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream, only=None, variant=0, counts=None):
     """Every algorithm on the same 1920x1080 stream (Linked List pool 1.5F,
     tail blending on): device time per frame, G fragments/s and the
     fraction of the HBM roofline by the algorithm's compulsory bytes.
-    `only`: a set of "name:K" strings to restrict the rows to."""
+    `only`: a set of "name:K" strings to restrict the rows to. `counts`:
+    fragments per target (host), for the slots the composites touch."""
     from vk_order_independent_transparency_amd import DeviceBuffer, Event, FrameConfig, Renderer
     from vk_order_independent_transparency_amd import roofline as rl
+    import numpy as np
 
     P = p.targets
     out = DeviceBuffer((P, 4), "float32")
@@ -180,11 +182,13 @@ def bench_algorithms(tg_d, d_d, c_d, F, p, steps, stream, only=None, variant=0):
         if alg == 1:
             nbytes = rl.ll_frame_bytes_reference(F, P)
         elif alg == 3:
-            nbytes = sum(rl.loop64_stage_bytes(F, P, K).values())
+            occ = None if counts is None else int(np.minimum(counts, K).sum())
+            nbytes = sum(rl.loop64_stage_bytes(F, P, K, occupied=occ).values())
         elif alg in (0, 4):
             nbytes = sum(rl.kbuffer_stage_bytes(F, P, K, alg == 4).values())
         elif alg == 2:
-            nbytes = sum(rl.loop32_stage_bytes(F, P, K).values())
+            occ = None if counts is None else int(np.minimum(counts, K).sum())
+            nbytes = sum(rl.loop32_stage_bytes(F, P, K, occupied=occ).values())
         elif alg == 5:
             nbytes = sum(rl.interlock_stage_bytes(F, P).values())
         else:
@@ -451,6 +455,7 @@ def main():
     from vk_order_independent_transparency_amd import (DeviceBuffer, Event, FrameConfig, OIT_LINKEDLIST, Renderer,
                                                        Stream, scene)
     from vk_order_independent_transparency_amd import roofline as rl
+    import numpy as np
 
     K = args.layers
     p = scene.HD_1080P
@@ -529,8 +534,9 @@ def main():
     st = r.stats(stream)
     r.close()
     only = {x.strip() for x in args.only_algorithms.split(",") if x.strip()}
+    counts = np.bincount(tg[tg < P].astype(np.int64), minlength=P)  # fragments per target
     algorithms = None if args.no_algorithms else bench_algorithms(tg_d, d_d, c_d, F, p, args.algorithm_steps, stream,
-                                                                  only, args.variant)
+                                                                  only, args.variant, counts)
 
     frames_4k = None if args.no_4k else bench_4k_4x(10, args.variant)
     batch = None if args.no_batch else bench_batch(1, args.batch_frames)

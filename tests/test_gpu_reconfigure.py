@@ -1,7 +1,7 @@
 """Reconfiguration reuses device buffers: a context that serves frames of
 changing size, K and algorithm keeps its allocations while they fit, stays
 bit-exact with the oracle on every frame (the Linked List pool, the Loop64
-slots and the colour-pass tail queue are kept clean across sizes,
+and Loop32 slots and the colour-pass tail queue are kept clean across sizes,
 algorithms and frames cleared before their resolve), and oit_trim releases
 everything."""
 import numpy as np
@@ -12,7 +12,7 @@ from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, Ren
 
 pytestmark = pytest.mark.gpu
 
-LL, LOOP64, SPIN, WB = 1, 3, 4, 6
+LL, LOOP32, LOOP64, SPIN, WB = 1, 2, 3, 4, 6
 
 
 def _frame(r, oracle, alg, K, tail, seed, w, h, spheres, pool_factor=1.5, unresolved_first=False):
@@ -67,6 +67,12 @@ def test_reconfigure_sequence_stays_exact(oracle):
             # targets not a multiple of four: the clear's fills end in single words
             (SPIN, 4, 1, 32, 63, 37, 150, 0, False),
             (SPIN, 4, 1, 33, 63, 37, 150, 0, True),
+            # Loop32 depth and colour slots, as the Loop64 slots
+            (LOOP32, 8, 1, 34, 64, 48, 200, 0, False),
+            (LOOP32, 8, 1, 35, 64, 48, 160, 0, False),
+            (LOOP32, 16, 1, 36, 63, 37, 150, 0, True),
+            (LOOP32, 4, 0, 37, 48, 40, 120, 0, False),
+            (LOOP32, 4, 1, 38, 48, 40, 200, 0, False),
         ]
         resident = []
         for alg, K, tail, seed, w, h, sp, pf, dirty in seq:

@@ -110,6 +110,7 @@ struct oit_ctx {
   // Loop64
   uint64_t* l64 = nullptr;
   bool l64_clean = false;  // every slot empty (a fill, or the last frame's composite)
+  bool l32_clean = false;  // Loop32: likewise for the depth and colour slots
   // WBOIT
   float4* wb_accum = nullptr;
   float* wb_reveal = nullptr;
@@ -242,6 +243,7 @@ void free_frame_buffers(oit_ctx* c) {
   dev_free(&c->il_full);
   dev_free(&c->l32_z);
   dev_free(&c->l32_c);
+  c->l32_clean = false;
   c->cap_il_full = c->cap_l32 = 0;
   dev_free(&c->kb_mask);
   dev_free(&c->tl_mask);
@@ -700,6 +702,7 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   }
   c->tl_head_valid = false;
   c->l64_clean = false;  // slots beyond the last frame's T x K may hold an older frame's keys
+  c->l32_clean = false;
   if (pooled(alg) && ll_grown) {
     // a new allocation is filled clean (every head 0, every link kUnset) by
     // the next clear, on its stream
@@ -738,7 +741,7 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
   std::memset(out, 0, sizeof *out);
   out->abuffer_bytes = c->bytes_abuf;
   out->aux_bytes = c->bytes_aux + (c->tl_head ? (uint64_t)c->T * 4u : 0u);
-  out->tail_bytes = (uint64_t)c->tl_cap * (4u + 16u + 8u);
+  out->tail_bytes = (uint64_t)c->tl_cap * (4u + 16u + 8u) + oit::tail_regions(c->tl_cap) * 4u;
   out->total_bytes = out->abuffer_bytes + out->aux_bytes + out->tail_bytes;
   out->ll_capacity = c->ll_cap;
   switch (c->cfg.algorithm) {
@@ -753,7 +756,7 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
   out->targets = c->T;
   out->resident_bytes = c->cap_ll_head * 4u + c->ll_alloc * 12u + c->cap_kb_count * 4u + c->cap_kb_lock * 4u +
                         c->cap_kb_depth * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_wb * 20u +
-                        c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u) +
+                        c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u) + oit::tail_regions(c->tl_cap) * 4u +
                         (c->cap_ll_mask + c->cap_kb_mask + c->cap_tl_mask) * 4u + c->cap_il_full * 16u +
                         c->cap_l32 * 12u + c->cap_opq * 8u + c->cap_opq_rgba * 16u;
   return OIT_OK;
@@ -790,11 +793,14 @@ int oit_clear(oit_ctx* c, void* stream) {
   int nf = 0;
   auto fill = [&](void* p, uint64_t n, uint32_t v) { fr[nf++] = oit::FillRange{static_cast<uint32_t*>(p), n, v}; };
   if (need_counters) fill(c->d_counters, 16, 0u);
-  bool l64_fill = false;
+  bool l64_fill = false, l32_fill = false;
   switch (c->cfg.algorithm) {
-    case OIT_LOOP:  // depths to all ones (oitRender.cpp:242-265), colour slots to 0
-      fill(c->l32_z, T * K, kEmpty);
-      fill(c->l32_c, 2 * T * K, 0u);
+    case OIT_LOOP:  // depths to all ones (oitRender.cpp:242-265), colour slots to 0; the composite leaves them so
+      l32_fill = !c->l32_clean;
+      if (l32_fill) {
+        fill(c->l32_z, T * K, kEmpty);
+        fill(c->l32_c, 2 * T * K, 0u);
+      }
       break;
     case OIT_INTERLOCK:
     case OIT_LINKEDLIST:  // head and counter to 0, oitRender.cpp:206-212; links of a dirty range to kUnset
@@ -826,6 +832,7 @@ int oit_clear(oit_ctx* c, void* stream) {
   }
   hipError_t e = staged(c, kStClear, s, [&]() -> hipError_t { return oit::launch_fill_ranges(fr, nf, s); });
   if (e == hipSuccess && l64_fill) c->l64_clean = true;
+  if (e == hipSuccess && l32_fill) c->l32_clean = true;
   if (e == hipSuccess && ll) {
     c->ll_clean = true;
     c->ll_dirty_upto = 0;
@@ -945,6 +952,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       break;
     }
     case OIT_LOOP: {  // depth pass now; the colour pass runs over the recorded batches at resolve
+      c->l32_clean = false;
       e = staged(c, kStStore, s, [&] { return oit::launch_loop32_depth(k, v, c->l32_z, s); });
       c->segments.push_back(v);
       break;
@@ -1066,6 +1074,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
       break;
     case OIT_LOOP:
       e = staged(c, kStResolve, s, [&] { return oit::launch_loop32_resolve(k, c->l32_z, c->l32_c, tbr, o, s); });
+      c->l32_clean = e == hipSuccess;  // the composite emptied every slot it read
       break;
     case OIT_INTERLOCK: {
       oit::LLBuf ll{c->ll_head, c->ll_cd, c->ll_next, c->ll_cap, c->d_counters + 1, c->d_counters};
@@ -1133,6 +1142,7 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
     c->ll_dirty_upto = c->ll_alloc;
     c->counters_clean = false;
     c->l64_clean = false;
+    c->l32_clean = false;
     return OIT_ERR_DEVICE;
   }
   return OIT_OK;

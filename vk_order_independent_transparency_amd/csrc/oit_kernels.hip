@@ -1824,11 +1824,12 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color(Common c, FragView f, c
 }
 
 // Composite (:186-222): the layers up to the first empty depth, front to back,
-// over the colour buffer.
+// over the colour buffer. The slots it read are set back to empty (depth
+// kEmpty, colour 0) for the next frame: the stored depths are a prefix, and
+// the colour pass writes only the slot of a stored depth.
 template <int K, bool kColorTail, bool kDebug>
-__global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint32_t* __restrict__ zs,
-                                                           const uint64_t* __restrict__ cs, TailBuf tb,
-                                                           ResolveOut o) {
+__global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, uint32_t* __restrict__ zs,
+                                                           uint64_t* __restrict__ cs, TailBuf tb, ResolveOut o) {
   __shared__ float s_dec[256];
   __shared__ float s_alpha[256];
   stage_decode(s_dec, s_alpha, c.tab);
@@ -1858,6 +1859,13 @@ __global__ __launch_bounds__(kBlock) void k_loop32_resolve(Common c, const uint3
   // the stored layers' colours, loaded together once n is known
 #pragma unroll
   for (int k = 0; k < K; ++k) col[k] = (uint32_t)k < n ? (uint32_t)cs[(uint64_t)k * c.T + t] : 0u;
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    if ((uint32_t)k < n) {
+      zs[(uint64_t)k * c.T + t] = kEmpty;
+      cs[(uint64_t)k * c.T + t] = 0ull;
+    }
+  }
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
   for (int k = 0; k < K; ++k)
@@ -3070,8 +3078,8 @@ struct Loop32Color {
 
 template <int K>
 struct Loop32Resolve {
-  static hipError_t run(const Common& c, const uint32_t* zs, const uint64_t* cs, const TailBuf& tb,
-                        const ResolveOut& o, hipStream_t s) {
+  static hipError_t run(const Common& c, uint32_t* zs, uint64_t* cs, const TailBuf& tb, const ResolveOut& o,
+                        hipStream_t s) {
     const bool dbg = o.fk_key || o.fk_val || o.counts;
     const bool ctail = c.tail && tb.head != nullptr;
     const dim3 g(blocks_for(c.T)), b(kBlock);
@@ -3110,8 +3118,8 @@ hipError_t launch_loop32_color(const Common& c, const FragView& f, const uint32_
   return dispatch_k<Loop32Color>(c.K, c, f, zs, cs, tb, s);
 }
 
-hipError_t launch_loop32_resolve(const Common& c, const uint32_t* zs, const uint64_t* cs, const TailBuf& tb,
-                                 const ResolveOut& o, hipStream_t s) {
+hipError_t launch_loop32_resolve(const Common& c, uint32_t* zs, uint64_t* cs, const TailBuf& tb, const ResolveOut& o,
+                                 hipStream_t s) {
   return dispatch_k<Loop32Resolve>(c.K, c, zs, cs, tb, o, s);
 }
 

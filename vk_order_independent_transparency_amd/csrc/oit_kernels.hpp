@@ -196,8 +196,9 @@ hipError_t launch_kbuf_resolve_cov(const Common& c, const KBuf& kb, const TailBu
 hipError_t launch_loop32_depth(const Common& c, const FragView& f, uint32_t* zs, hipStream_t s);
 hipError_t launch_loop32_color(const Common& c, const FragView& f, const uint32_t* zs, uint64_t* cs,
                                const TailBuf& tb, hipStream_t s);
-hipError_t launch_loop32_resolve(const Common& c, const uint32_t* zs, const uint64_t* cs, const TailBuf& tb,
-                                 const ResolveOut& o, hipStream_t s);
+// Also sets the slots it reads back to empty.
+hipError_t launch_loop32_resolve(const Common& c, uint32_t* zs, uint64_t* cs, const TailBuf& tb, const ResolveOut& o,
+                                 hipStream_t s);
 // ordered interlock: Linked List append with shaded colours, then per-pixel
 // critical sections replayed in primitive order
 hipError_t launch_interlock_resolve(const Common& c, const LLBuf& ll, const ResolveOut& o, hipStream_t s);

@@ -53,18 +53,20 @@ bool valid_layers(uint32_t k) { return k == 1 || k == 2 || k == 4 || k == 8 || k
 bool supported(uint32_t alg) { return alg <= OIT_WEIGHTED; }
 
 // Device bytes a context holds for one request (the sizes oit_configure
-// allocates; tail arrays 28 B per fragment, tail heads 4 B per target).
+// allocates; tail arrays 28 B per fragment and a 4 B queue counter per 1024
+// fragments, tail heads 4 B per target).
 uint64_t request_bytes(const oit_request& r, uint64_t F) {
   const uint64_t T = (uint64_t)r.scene.width * r.scene.height * r.scene.samples;
   const uint64_t K = r.algorithm == OIT_WEIGHTED ? 1 : r.layers;
   const bool tail = r.tail_blend != 0;
+  const uint64_t tails = tail ? 4 * T + 28 * F + 4 * ((F + 1023) / 1024) : 0;
   switch (r.algorithm) {
     case OIT_LINKEDLIST: return 4 * T + 12 * (F + 1) + (tail ? 4 * T : 0);
     case OIT_INTERLOCK: return 4 * T + 28 * (F + 1);
-    case OIT_LOOP: return 12 * K * T + (tail ? 4 * T + 28 * F : 0);
-    case OIT_LOOP64: return 8 * K * T + (tail ? 4 * T + 28 * F : 0);
-    case OIT_SIMPLE: return 4 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
-    case OIT_SPINLOCK: return 8 * T + 12 * K * T + (tail ? 4 * T + 28 * F : 0);
+    case OIT_LOOP: return 12 * K * T + tails;
+    case OIT_LOOP64: return 8 * K * T + tails;
+    case OIT_SIMPLE: return 4 * T + 12 * K * T + tails;
+    case OIT_SPINLOCK: return 8 * T + 12 * K * T + tails;
     default: return 20 * T;
   }
 }

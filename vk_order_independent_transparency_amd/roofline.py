@@ -37,12 +37,20 @@ def ll_frame_bytes_reference(F: int, P: int) -> int:
     return 64 * F + 24 * P
 
 
-def loop64_stage_bytes(F: int, P: int, K: int) -> dict:
-    return {
-        "clear": 8 * K * P,
-        "store": 20 * F + 16 * F,  # stream in + at least one slot read-modify-write
-        "resolve": 8 * K * P + 16 * P,
-    }
+def loop64_stage_bytes(F: int, P: int, K: int, occupied: int | None = None, tail: bool = True) -> dict:
+    """Loop64: no clear (the composite empties the slots it reads, and only
+    a reconfigured or unresolved A-buffer is filled); store: stream in 20F +
+    at least one 8-byte slot read-modify-write 16F; with tail blending the
+    classify pass reads (target, depth) 8F and one K-th slot per target 8P;
+    composite: the occupied slots read and set back to empty 16 x occupied,
+    the first empty slot 8P, RGBA out 16P. `occupied` = sum over targets of
+    min(fragments, K); min(F, KP) bounds it from above."""
+    occ = min(F, K * P) if occupied is None else occupied
+    b = {"clear": 0, "store": 20 * F + 16 * F}
+    if tail:
+        b["classify"] = 8 * F + 8 * P
+    b["resolve"] = 16 * occ + 8 * P + 16 * P
+    return b
 
 
 def kbuffer_stage_bytes(F: int, P: int, K: int, spin: bool) -> dict:
@@ -62,16 +70,20 @@ def wboit_stage_bytes(F: int, P: int) -> dict:
     }
 
 
-def loop32_stage_bytes(F: int, P: int, K: int) -> dict:
-    """Loop32 (two geometry passes): depth planes 4KP + (id, colour) planes 8KP
-    cleared; depth pass 8F in (target, depth) + at least one 4-byte slot
-    read-modify-write 8F; colour pass 24F in + the K-th depth 4F + one 8-byte
-    slot read-modify-write 16F; composite 12KP + 16P."""
+def loop32_stage_bytes(F: int, P: int, K: int, occupied: int | None = None) -> dict:
+    """Loop32 (two geometry passes): no clear (the composite empties the
+    depth and colour slots it reads); depth pass 8F in (target, depth) + at
+    least one 4-byte slot read-modify-write 8F; colour pass 24F in + the K-th
+    depth 4F + one 8-byte slot read-modify-write 16F; composite: occupied
+    depth and colour slots read and set back to empty 24 x occupied, the
+    first empty depth 4P, RGBA out 16P. `occupied` as for Loop64 (distinct
+    depths here; fragments bound them from above)."""
+    occ = min(F, K * P) if occupied is None else occupied
     return {
-        "clear": 12 * K * P,
+        "clear": 0,
         "store": 8 * F + 8 * F,
         "classify": 24 * F + 4 * F + 16 * F,
-        "resolve": 12 * K * P + 16 * P,
+        "resolve": 24 * occ + 4 * P + 16 * P,
     }
 
 
