@@ -1,20 +1,24 @@
 // oit_server.cpp - request loop over the OIT path (oit_server_*).
 //
 // Requests (a seeded scene, an algorithm, K) arrive at any time from any
-// thread. A dispatcher thread owns `slots` HIP streams; each slot keeps a
-// scene generator and one context per algorithm whose device buffers stay
-// resident and are reused by later requests. Per request a slot:
-//   1. stages the scene and counts its fragments on the device (async),
+// thread. The server owns `slots` HIP streams, each driven by a thread of
+// its own; each slot keeps a scene generator and its fragment-stream
+// buffers, and the OIT contexts form one pool whose device buffers stay
+// resident and are reused by later requests. A slot's thread takes the
+// oldest waiting request and:
+//   1. stages the scene and counts its fragments on the device,
 //   2. sizes the A-buffer from that exact count (Linked List pool = F + 1
 //      nodes, Loop64/k-buffer tail arrays = F) and asks the HBM budget for
-//      it: granted in arrival order, idle contexts are trimmed under
-//      pressure, a request larger than the whole budget is refused,
-//   3. generates the fragments into the slot's stream buffers and runs
-//      clear, append and resolve, then records its completion event.
+//      it: granted in arrival order (a ticket per request), idle contexts
+//      and idle slots' stream buffers are trimmed under pressure, a request
+//      larger than the whole budget is refused,
+//   3. generates the fragments into the slot's stream buffers, runs clear,
+//      append and resolve, and waits for its completion event.
 // Slots run concurrently (continuous batching: a request that arrives while
 // others run is admitted as soon as a slot and memory are free, and its
-// kernels overlap theirs on the device). Latency is arrival to the
-// dispatcher seeing the completion event, on the host steady clock.
+// kernels overlap theirs on the device), and their HIP calls are issued
+// from their own threads. Latency is arrival to the slot seeing the
+// completion event, on the host steady clock.
 //
 // No counterpart in the reference (a single interactive frame loop,
 // Sample::onRender, oitRender.cpp:28-154); the per-request work is the
@@ -34,6 +38,8 @@
 
 #include "oit.h"
 #include "oit_scene_gen.hpp"
+
+struct PoolCtx;
 
 namespace {
 
@@ -105,8 +111,6 @@ struct Done {
   uint64_t replay;
 };
 
-enum class SlotState { kIdle, kCounting, kSizing, kRunning };
-
 struct Slot {
   explicit Slot(int device) : gen(device) {}
   uint32_t index = 0;
@@ -116,17 +120,26 @@ struct Slot {
   // generation plus OIT passes (ev[2] .. done)
   hipEvent_t ev[3] = {};
   oit::SceneGen gen;
-  int ctx = -1;  // index of the server context the slot's request uses (sizing .. done)
+  std::thread thread;
+  // under the server's grant lock: the stream buffers (another slot's grant
+  // may release them while this slot is not launching) and `launching`
   uint32_t* target = nullptr;
   float* depth = nullptr;
   float4* rgba = nullptr;
   uint64_t cap_target = 0, cap_depth = 0, cap_rgba = 0;
   float4* out_own = nullptr;
   uint64_t cap_out = 0;
+  bool launching = false;  // granted and not finished: its buffers and context are in use
+  int ctx = -1;            // index of the pooled context the request uses (grant .. finish)
+  PoolCtx* pctx = nullptr;  // that context (a deque element: its address is stable)
+  // granted growth not yet allocated (the stream buffers', the context's):
+  // counted as held until the launch has grown them, so that a later grant
+  // does not hand the same bytes out again
+  uint64_t reserved_stream = 0, reserved_ctx = 0;
   uint64_t held_stream() const { return 4 * (cap_target + cap_depth) + 16 * (cap_rgba + cap_out); }
-  SlotState state = SlotState::kIdle;
+  // the slot thread's own
   Pending job{};
-  uint64_t seq = 0;  // admission order (memory is granted in this order)
+  uint64_t seq = 0;  // admission ticket (memory is granted in this order)
   uint64_t fragments = 0;
   uint64_t budgeted = 0;     // A-buffer bytes of the request's context
   uint64_t stream_need = 0;  // fragment-stream (and own output) bytes of the request
@@ -148,10 +161,14 @@ struct PoolCtx {
 
 struct oit_server {
   int device = 0;
-  std::vector<PoolCtx> ctxs;  // dispatcher only
   oit_server_config cfg{};
   Clock::time_point t0;
   std::vector<std::unique_ptr<Slot>> slots;
+  // grants: the pool, the slots' buffers, the admission ticket
+  std::mutex gmu;
+  std::condition_variable grant_cv;  // a ticket was used up, or memory was given back
+  std::deque<PoolCtx> ctxs;          // stable addresses: a slot uses its entry outside the lock
+  uint64_t next_grant = 0;
   // shared with submitters
   std::mutex mu;
   std::condition_variable cv;       // work arrived / stop
@@ -163,25 +180,23 @@ struct oit_server {
   uint64_t submitted = 0, completed = 0, failed = 0;
   uint64_t held = 0, peak_held = 0;
   uint32_t in_flight = 0, peak_in_flight = 0;
-  bool busy = false;  // the dispatcher has work in a slot
-  // dispatcher only
   uint64_t next_seq = 0;
-  std::thread thread;
 };
 
 namespace {
 
-// Device bytes the slots hold: contexts and fragment-stream buffers.
+// Device bytes the server holds for requests: contexts and the slots'
+// stream buffers. Under gmu.
 uint64_t held_bytes(const oit_server* s) {
   uint64_t h = 0;
-  for (const auto& sl : s->slots) h += sl->held_stream();
+  for (const auto& sl : s->slots) h += sl->held_stream() + sl->reserved_stream + sl->reserved_ctx;
   for (const auto& c : s->ctxs) h += c.resident;
   return h;
 }
 
 // The context a request of `alg` needing `need` bytes takes: the smallest
 // idle one that already holds enough, else the largest idle one (it grows),
-// else a new entry (created at launch). Returns its index.
+// else a new entry (created at launch). Returns its index. Under gmu.
 int pick_ctx(oit_server* s, uint32_t alg, uint64_t need) {
   int fit = -1, big = -1;
   for (int i = 0; i < (int)s->ctxs.size(); ++i) {
@@ -198,6 +213,7 @@ int pick_ctx(oit_server* s, uint32_t alg, uint64_t need) {
   return (int)s->ctxs.size() - 1;
 }
 
+// Under gmu.
 void publish_held(oit_server* s) {
   const uint64_t h = held_bytes(s);
   std::lock_guard<std::mutex> lk(s->mu);
@@ -226,17 +242,25 @@ void finish(oit_server* s, Slot& sl, int status) {
     else
       (void)hipGetLastError();
   }
-  sl.state = SlotState::kIdle;
-  if (sl.ctx >= 0) s->ctxs[sl.ctx].busy = false;  // finish runs on the dispatcher, as every ctxs access
-  sl.ctx = -1;
+  {
+    std::lock_guard<std::mutex> g(s->gmu);
+    if (sl.ctx >= 0) s->ctxs[sl.ctx].busy = false;
+    sl.ctx = -1;
+    sl.pctx = nullptr;
+    sl.launching = false;
+    sl.reserved_stream = sl.reserved_ctx = 0;
+  }
+  s->grant_cv.notify_all();  // its context and buffers may be trimmed for a waiting request
   std::lock_guard<std::mutex> lk(s->mu);
   s->done.push_back(Done{c, sl.job.replay});
   s->completed++;
   if (status != OIT_OK) s->failed++;
   s->in_flight--;
+  if (s->in_flight == 0 && s->queue.empty()) s->idle_cv.notify_all();
 }
 
 // Frees a slot's fragment-stream and output buffers, in order on its stream.
+// Under gmu, for a slot that is not launching (or at teardown).
 void release_stream(Slot& sl) {
   for (void* p : {(void*)sl.target, (void*)sl.depth, (void*)sl.rgba, (void*)sl.out_own})
     if (p) (void)hipFreeAsync(p, sl.stream);
@@ -247,9 +271,9 @@ void release_stream(Slot& sl) {
   sl.cap_target = sl.cap_depth = sl.cap_rgba = sl.cap_out = 0;
 }
 
-// Frees idle contexts (least recently used first) and idle slots' stream
-// buffers until `need` more bytes fit the budget; `keep` is the context the
-// request itself will use.
+// Frees idle contexts (least recently used first) and the stream buffers
+// of slots that are not launching until `need` more bytes fit the budget;
+// `keep` is the context the request itself will use. Under gmu.
 bool make_room(oit_server* s, Slot& self, int keep, uint64_t need) {
   uint64_t held = held_bytes(s);
   if (held + need <= s->cfg.budget_bytes) return true;
@@ -266,14 +290,14 @@ bool make_room(oit_server* s, Slot& self, int keep, uint64_t need) {
   for (auto& sp : s->slots) {
     if (held + need <= s->cfg.budget_bytes) break;
     Slot& sl = *sp;
-    if (&sl == &self || sl.state == SlotState::kRunning || !sl.held_stream()) continue;
+    if (&sl == &self || sl.launching || !sl.held_stream()) continue;
     held -= sl.held_stream();
     release_stream(sl);
   }
   return held + need <= s->cfg.budget_bytes;
 }
 
-// Enqueues generation and the OIT passes of the slot's request.
+// Enqueues generation and the OIT passes of the slot's granted request.
 int launch(oit_server* s, Slot& sl) {
   const oit_request& r = sl.job.req;
   const uint32_t alg = r.algorithm;
@@ -281,16 +305,20 @@ int launch(oit_server* s, Slot& sl) {
   const uint64_t T = (uint64_t)r.scene.width * r.scene.height * r.scene.samples;
   int rc;
   if ((rc = st(hipEventRecord(sl.ev[2], sl.stream)))) return rc;
-  if ((rc = grow(&sl.target, &sl.cap_target, F, sl.stream))) return rc;
-  if ((rc = grow(&sl.depth, &sl.cap_depth, F, sl.stream))) return rc;
-  if ((rc = grow(&sl.rgba, &sl.cap_rgba, F, sl.stream))) return rc;
   float* out = r.out;
-  if (!out) {
-    if ((rc = grow(&sl.out_own, &sl.cap_out, T, sl.stream))) return rc;
-    out = reinterpret_cast<float*>(sl.out_own);
+  {
+    std::lock_guard<std::mutex> g(s->gmu);  // others read the caps for the budget
+    if ((rc = grow(&sl.target, &sl.cap_target, F, sl.stream))) return rc;
+    if ((rc = grow(&sl.depth, &sl.cap_depth, F, sl.stream))) return rc;
+    if ((rc = grow(&sl.rgba, &sl.cap_rgba, F, sl.stream))) return rc;
+    if (!out) {
+      if ((rc = grow(&sl.out_own, &sl.cap_out, T, sl.stream))) return rc;
+      out = reinterpret_cast<float*>(sl.out_own);
+    }
+    sl.reserved_stream = 0;  // now in the caps
   }
   sl.out = out;
-  PoolCtx& pc = s->ctxs[sl.ctx];
+  PoolCtx& pc = *sl.pctx;  // this slot's while busy
   if (!pc.ctx && (rc = oit_create(&pc.ctx, s->device))) return rc;
   oit_config c;
   std::memset(&c, 0, sizeof c);
@@ -308,8 +336,13 @@ int launch(oit_server* s, Slot& sl) {
   c.wboit_view_span = 8.0f;
   if ((rc = oit_configure(pc.ctx, &c))) return rc;
   oit_sizes z;
-  if (oit_query_sizes(pc.ctx, &z) == OIT_OK) pc.resident = z.resident_bytes;
-  publish_held(s);
+  const bool sized = oit_query_sizes(pc.ctx, &z) == OIT_OK;
+  {
+    std::lock_guard<std::mutex> g(s->gmu);
+    if (sized) pc.resident = z.resident_bytes;
+    sl.reserved_ctx = 0;  // now in the context's resident bytes
+    publish_held(s);
+  }
   if ((rc = sl.gen.emit(sl.target, sl.depth, reinterpret_cast<float*>(sl.rgba), sl.cap_target, sl.stream))) return rc;
   oit_fragments f{sl.target, sl.depth, reinterpret_cast<const float*>(sl.rgba), F, nullptr};
   if ((rc = oit_clear(pc.ctx, sl.stream))) return rc;
@@ -318,111 +351,108 @@ int launch(oit_server* s, Slot& sl) {
   return st(hipEventRecord(sl.done, sl.stream));
 }
 
-// One pass over the slots and the queue; true if anything moved.
-bool step(oit_server* s) {
-  bool moved = false;
-  // completions and counts
-  for (auto& sp : s->slots) {
-    Slot& sl = *sp;
-    if (sl.state == SlotState::kRunning) {
-      const hipError_t e = hipEventQuery(sl.done);
-      if (e == hipSuccess) {
-        finish(s, sl, OIT_OK);
-        moved = true;
-      } else if (e != hipErrorNotReady) {
-        (void)hipGetLastError();
-        finish(s, sl, OIT_ERR_DEVICE);
-        moved = true;
-      }
-    } else if (sl.state == SlotState::kCounting && sl.gen.count_ready()) {
-      sl.fragments = sl.gen.fragments();
-      sl.budgeted = request_bytes(sl.job.req, sl.fragments);
-      const auto& sc = sl.job.req.scene;
-      sl.stream_need = stream_bytes(sl.fragments, (uint64_t)sc.width * sc.height * sc.samples, !sl.job.req.out);
-      sl.state = SlotState::kSizing;
-      moved = true;
-    }
-  }
-  // memory grants, oldest admission first; a later request waits behind an
-  // earlier one that does not fit yet
-  std::vector<Slot*> sizing;
-  for (auto& sp : s->slots)
-    if (sp->state == SlotState::kSizing) sizing.push_back(sp.get());
-  std::sort(sizing.begin(), sizing.end(), [](const Slot* a, const Slot* b) { return a->seq < b->seq; });
-  for (Slot* slp : sizing) {
-    Slot& sl = *slp;
+// Waits for the request's turn at the grants and for memory, then marks
+// the slot launching with its context. Returns OIT_OK when granted, else
+// the request's error; the ticket is used up either way.
+int grant(oit_server* s, Slot& sl, bool counted) {
+  std::unique_lock<std::mutex> g(s->gmu);
+  s->grant_cv.wait(g, [&] { return s->next_grant == sl.seq; });
+  int rc = OIT_OK;
+  if (!counted) {
+    rc = OIT_ERR_DEVICE;
+  } else if (sl.budgeted + sl.stream_need > s->cfg.budget_bytes) {
+    rc = OIT_ERR_OVER_BUDGET;
+  } else {
     const int alg = (int)sl.job.req.algorithm;
-    if (sl.budgeted + sl.stream_need > s->cfg.budget_bytes) {
-      finish(s, sl, OIT_ERR_OVER_BUDGET);
-      moved = true;
-      continue;
+    for (;;) {
+      // the context's growth plus the slot's stream buffers' growth (with
+      // the quarter of slack grow() adds)
+      const int ci = pick_ctx(s, (uint32_t)alg, sl.budgeted);
+      const uint64_t held_c = s->ctxs[ci].resident;
+      const uint64_t held_s = sl.held_stream();
+      const uint64_t want_s = sl.stream_need + sl.stream_need / 4;
+      const uint64_t grow_c = sl.budgeted > held_c ? sl.budgeted - held_c : 0;
+      const uint64_t grow_s = sl.stream_need > held_s ? want_s - held_s : 0;
+      if (make_room(s, sl, ci, grow_c + grow_s)) {
+        sl.reserved_ctx = grow_c;
+        sl.reserved_stream = grow_s;
+        sl.ctx = ci;
+        sl.pctx = &s->ctxs[ci];
+        s->ctxs[ci].busy = true;
+        s->ctxs[ci].last_use = sl.seq;
+        sl.launching = true;
+        break;
+      }
+      // a later ticket waits behind this one; running requests give memory back
+      s->grant_cv.wait(g);
     }
-    // the context's growth plus the slot's stream buffers' growth (with the
-    // quarter of slack grow() adds)
-    const int ci = pick_ctx(s, (uint32_t)alg, sl.budgeted);
-    const uint64_t held_c = s->ctxs[ci].resident;
-    const uint64_t held_s = sl.held_stream();
-    const uint64_t want_s = sl.stream_need + sl.stream_need / 4;
-    const uint64_t extra = (sl.budgeted > held_c ? sl.budgeted - held_c : 0) +
-                           (sl.stream_need > held_s ? want_s - held_s : 0);
-    if (!make_room(s, sl, ci, extra)) break;
-    sl.ctx = ci;
-    s->ctxs[ci].busy = true;
-    s->ctxs[ci].last_use = sl.seq;
-    sl.t_start = Clock::now();
-    sl.state = SlotState::kRunning;
-    const int rc = launch(s, sl);
-    if (rc) finish(s, sl, rc);
-    moved = true;
   }
-  // admissions
-  for (auto& sp : s->slots) {
-    Slot& sl = *sp;
-    if (sl.state != SlotState::kIdle) continue;
-    Pending p;
+  s->next_grant++;
+  g.unlock();
+  s->grant_cv.notify_all();
+  return rc;
+}
+
+// One request on a slot, from its count to its completion.
+void run_request(oit_server* s, Slot& sl) {
+  sl.fragments = 0;
+  sl.budgeted = 0;
+  sl.stream_need = 0;
+  sl.out = nullptr;
+  sl.t_start = Clock::now();
+  int rc = st(hipEventRecord(sl.ev[0], sl.stream));
+  if (!rc) rc = sl.gen.begin(&sl.job.req.scene, sl.stream);
+  if (!rc) rc = st(hipEventRecord(sl.ev[1], sl.stream));
+  if (!rc) {
+    while (!sl.gen.count_ready()) std::this_thread::yield();
+    sl.fragments = sl.gen.fragments();
+    sl.budgeted = request_bytes(sl.job.req, sl.fragments);
+    const auto& sc = sl.job.req.scene;
+    sl.stream_need = stream_bytes(sl.fragments, (uint64_t)sc.width * sc.height * sc.samples, !sl.job.req.out);
+  }
+  const int g = grant(s, sl, rc == OIT_OK);
+  if (g != OIT_OK) {
+    finish(s, sl, rc ? rc : g);
+    return;
+  }
+  sl.t_start = Clock::now();
+  rc = launch(s, sl);
+  if (rc) {
+    finish(s, sl, rc);
+    return;
+  }
+  // this thread's only job is this request: poll its completion back to
+  // back (a timed sleep of a few microseconds lasts tens on a loaded host)
+  for (;;) {
+    const hipError_t e = hipEventQuery(sl.done);
+    if (e == hipSuccess) {
+      finish(s, sl, OIT_OK);
+      return;
+    }
+    if (e != hipErrorNotReady) {
+      (void)hipGetLastError();
+      finish(s, sl, OIT_ERR_DEVICE);
+      return;
+    }
+    std::this_thread::yield();
+  }
+}
+
+void slot_main(oit_server* s, Slot* slp) {
+  Slot& sl = *slp;
+  (void)hipSetDevice(s->device);
+  for (;;) {
     {
-      std::lock_guard<std::mutex> lk(s->mu);
-      if (s->queue.empty()) break;
-      p = s->queue.front();
+      std::unique_lock<std::mutex> lk(s->mu);
+      s->cv.wait(lk, [&] { return s->stop || !s->queue.empty(); });
+      if (s->queue.empty()) return;  // stopping, and nothing left to run
+      sl.job = s->queue.front();
       s->queue.pop_front();
+      sl.seq = s->next_seq++;  // tickets in queue order
       s->in_flight++;
       s->peak_in_flight = std::max(s->peak_in_flight, s->in_flight);
     }
-    sl.job = p;
-    sl.seq = s->next_seq++;
-    sl.fragments = 0;
-    sl.budgeted = 0;
-    sl.out = nullptr;
-    sl.t_start = Clock::now();
-    sl.state = SlotState::kCounting;
-    int rc = st(hipEventRecord(sl.ev[0], sl.stream));
-    if (!rc) rc = sl.gen.begin(&p.req.scene, sl.stream);
-    if (!rc) rc = st(hipEventRecord(sl.ev[1], sl.stream));
-    if (rc) finish(s, sl, rc);
-    moved = true;
-  }
-  return moved;
-}
-
-void dispatcher(oit_server* s) {
-  (void)hipSetDevice(s->device);
-  for (;;) {
-    const bool moved = step(s);
-    bool active = false;
-    for (auto& sp : s->slots) active |= sp->state != SlotState::kIdle;
-    std::unique_lock<std::mutex> lk(s->mu);
-    s->busy = active || !s->queue.empty();
-    if (!s->busy) {
-      s->idle_cv.notify_all();
-      if (s->stop) return;
-      s->cv.wait_for(lk, std::chrono::milliseconds(2));
-      continue;
-    }
-    lk.unlock();
-    // requests in flight: poll again at once (a timed sleep of a few
-    // microseconds lasts tens on a loaded host and would sit between every
-    // step of every request); the core is given up only while idle
-    if (!moved) std::this_thread::yield();
+    run_request(s, sl);
   }
 }
 
@@ -460,9 +490,8 @@ int submit_tagged(oit_server* s, const oit_request& r, uint64_t replay) {
     if (s->stop) return OIT_ERR_STATE;
     s->queue.push_back(Pending{r, Clock::now(), replay});
     s->submitted++;
-    s->busy = true;
   }
-  s->cv.notify_all();
+  s->cv.notify_one();
   return OIT_OK;
 }
 
@@ -499,7 +528,7 @@ int oit_server_create(oit_server** out, int device, const oit_server_config* cfg
     delete s;
     return rc;
   }
-  s->thread = std::thread(dispatcher, s);
+  for (auto& sl : s->slots) sl->thread = std::thread(slot_main, s, sl.get());
   *out = s;
   return OIT_OK;
 }
@@ -511,7 +540,8 @@ int oit_server_destroy(oit_server* s) {
     s->stop = true;
   }
   s->cv.notify_all();
-  if (s->thread.joinable()) s->thread.join();
+  for (auto& sl : s->slots)
+    if (sl->thread.joinable()) sl->thread.join();
   (void)hipSetDevice(s->device);
   (void)hipDeviceSynchronize();
   release(s);
@@ -545,7 +575,8 @@ int oit_server_poll(oit_server* s, oit_completion* out, uint32_t max, uint32_t* 
 int oit_server_drain(oit_server* s, uint32_t timeout_ms) {
   if (!s) return OIT_ERR_INVALID_ARG;
   std::unique_lock<std::mutex> lk(s->mu);
-  const bool ok = s->idle_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms), [&] { return !s->busy; });
+  const bool ok = s->idle_cv.wait_for(lk, std::chrono::milliseconds(timeout_ms),
+                                      [&] { return s->in_flight == 0 && s->queue.empty(); });
   return ok ? OIT_OK : OIT_ERR_NOT_READY;
 }
 
