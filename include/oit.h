@@ -397,8 +397,9 @@ int oit_ll_render_noalloc(int device, const oit_batch_frame* frame, void* stream
 /* ---- Request loop ---------------------------------------------------------
  * No counterpart in the reference (one interactive frame loop,
  * Sample::onRender, oitRender.cpp:28-154). Requests (a seeded scene, an
- * algorithm, K) are taken from any thread; a dispatcher thread runs up to
- * `slots` of them at once on their own streams (continuous batching: a
+ * algorithm, K) are taken from any thread; up to `slots` of them run at
+ * once, each slot on its own stream driven by a thread of its own
+ * (continuous batching: a
  * request that arrives while others run starts as soon as a slot and memory
  * are free, and its kernels overlap theirs). Each request's fragments are
  * generated and counted on the device; its A-buffer is sized from that exact
@@ -432,7 +433,7 @@ typedef struct oit_completion {
   const float* out;           /* the image (a slot's own buffer stays valid until the slot's next request) */
   double   t_submit_us;       /* steady clock, microseconds since the server was created */
   double   t_start_us;        /* memory granted, kernels enqueued */
-  double   t_done_us;         /* completion seen by the dispatcher */
+  double   t_done_us;         /* completion seen by the slot's thread */
   double   device_us;         /* device time of the request's work (HIP events: the count, then generation
                                  and the OIT passes; host gaps between them excluded); 0 on error */
 } oit_completion;
