@@ -9,7 +9,7 @@ import numpy as np
 
 def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, samples=1, ll_pool_nodes=0,
             splits=1, background=(0.5, 0.5, 0.5, 1.0), max_fragments=0, renderer=None, variant=0,
-            coverage_samples=0, mask=None, opaque=None, opaque_splits=1):
+            coverage_samples=0, mask=None, opaque=None, opaque_splits=1, bounds=None):
     from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, Renderer
 
     T = width * height * samples
@@ -47,7 +47,8 @@ def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, s
                 n = int(b - a)
                 r.draw_opaque(ob[0].view(a, n), ob[1].view(a, n), ob[2].view(a, n), count=n,
                               coverage=None if ob[3] is None else ob[3].view(a, n))
-        bounds = np.linspace(0, F, splits + 1).astype(np.int64)
+        # appends: `splits` equal parts, or the given first ids (`bounds`, from 0 to F)
+        bounds = np.linspace(0, F, splits + 1).astype(np.int64) if bounds is None else np.asarray(bounds, np.int64)
         for a, b in zip(bounds[:-1], bounds[1:]):
             n = int(b - a)
             r.append(tg.view(a, n), dp.view(a, n), cl.view(a, n), count=n,
