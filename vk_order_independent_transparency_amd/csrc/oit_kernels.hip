@@ -2469,9 +2469,8 @@ __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, Re
 }
 
 // ---------------------------------------------------------------------------
-// Tail lists, phase 2: link the frame's records into per-target lists in
-// descending id (newest first, the order the fold associates in). Records
-// arrive roughly in id order, so most inserts succeed at the head.
+// Tail lists, phase 2: link the frame's queued records into per-target
+// lists in descending id (newest first, the order the fold associates in).
 // ---------------------------------------------------------------------------
 
 // Work item w of the grid is chunk (w mod 2) of 512 entries of queue

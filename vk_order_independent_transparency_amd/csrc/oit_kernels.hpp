@@ -182,9 +182,8 @@ hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& w
                               hipStream_t s);
 hipError_t launch_wboit_resolve(const Common& c, const WBBuf& wb, const ResolveOut& o, hipStream_t s);
 
-// Links the records of ids first .. frags - 1 (the ones that can be tails).
-// Links the tail queue's records of frame-local ids [first, frags) into the
-// per-target lists.
+// Links the tail queue's records of frame-local ids [first, frags) (the ones
+// that can be tails) into the per-target lists.
 hipError_t launch_tail_link(const TailBuf& tb, uint32_t first, uint32_t frags, hipStream_t s);
 // coverage shading (Common::cov samples per pixel, masks in the buffers)
 hipError_t launch_ll_resolve_cov(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o,
