@@ -2292,43 +2292,63 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f,
 // evicted or rejected (the colours oitLoop64.frag.glsl:131-136 tail-blends);
 // one equal to it is "boundary": identical (depth, colour) keys are the same
 // bits in the A-buffer, and the resolve keeps the lowest ids of them in front.
-template <int K>
+// Each wave takes kU chunks of 64 fragments: their targets and depths in one
+// round trip, their K-th slots in a second, and the colours of the
+// candidates (a depth at or past the K-th slot's: every fragment of a pixel
+// with fewer than K, where that slot is empty, is kept without one) in a
+// third, so only tail fragments and the K-th ones read a colour.
+template <int K, int kU>
 __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f, const uint64_t* __restrict__ slots,
                                                             TailBuf tb) {
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  bool valid = i < f.count;
-  // K <= 8: the colour is loaded with the target and depth (one round
-  // trip), since most waves hold a fragment past the K-th slot and would
-  // otherwise load it a round trip later; at K = 16 and 32 few do, and
-  // the colour is read only after the slot test
-  constexpr bool kEarlyColour = K <= 8;
-  uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
-  const float dv = valid ? ld_stream(&f.depth[i]) : 0.f;
-  const float4 cv0 = (kEarlyColour && valid) ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
-  if (t >= c.T) valid = false;
-  bool tail = false;
-  uint32_t flag = 0;
-  float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t key = valid ? depth_key(dv) : 0u;
-  if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) valid = false;
-  if (valid) {
-    // the key's high half is the depth: a fragment nearer than the K-th
-    // slot's depth is kept without its colour being encoded (every
-    // fragment of a pixel with fewer than K, where that slot is empty)
-    const uint64_t s = slots[(uint64_t)(K - 1) * c.T + t];
-    if (key >= (uint32_t)(s >> 32)) {
-      const uint32_t col = encode_color_lds(kEarlyColour ? cv0 : ld_stream(&f.rgba[i]), s_thr);
-      const uint64_t z = ((uint64_t)key << 32) | col;
-      if (z >= s) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t first = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
+  uint32_t t[kU], key[kU];
+  float dv[kU];
+  bool valid[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    valid[u] = i < f.count;
+    t[u] = valid[u] ? ld_stream(&f.target[i]) : kEmpty;
+    dv[u] = valid[u] ? ld_stream(&f.depth[i]) : 0.f;
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    if (t[u] >= c.T) valid[u] = false;
+    key[u] = valid[u] ? depth_key(dv[u]) : 0u;
+    if (valid[u] && !opaque_visible(c.opq_z, c.T, 1u, t[u], key[u], 1u)) valid[u] = false;
+  }
+  uint64_t sk[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) sk[u] = valid[u] ? slots[(uint64_t)(K - 1) * c.T + t[u]] : 0ull;
+  bool cand[kU];
+  float4 cv[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    // the key's high half is the depth: a nearer fragment stays in the slots
+    cand[u] = valid[u] && key[u] >= (uint32_t)(sk[u] >> 32);
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    cv[u] = cand[u] ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    bool tail = false;
+    uint32_t flag = 0;
+    float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (cand[u]) {
+      const uint32_t col = encode_color_lds(cv[u], s_thr);
+      const uint64_t z = ((uint64_t)key[u] << 32) | col;
+      if (z >= sk[u]) {
         tail = true;
-        flag = z == s ? 1u : 0u;
+        flag = z == sk[u] ? 1u : 0u;
         payload = decode_premult(col, c.tab);
       }
     }
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    tail_record_own(tail, t[u], f.id_base + (uint32_t)i, flag, payload, tb);
   }
-  tail_record_own(tail, t, f.id_base + i, flag, payload, tb);
 }
 
 // Composite (oitLoop64.frag.glsl:162-183): blend slots front to back until
@@ -2740,7 +2760,10 @@ struct Loop64Store {
 template <int K>
 struct Loop64Classify {
   static hipError_t run(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb, hipStream_t s) {
-    hipLaunchKernelGGL((k_loop64_classify<K>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, slots, tb);
+    if (f.count == 0) return hipSuccess;
+    constexpr int kU = 4;  // 64-fragment chunks per wave
+    const uint32_t blocks = (uint32_t)((f.count + kBlock * kU - 1) / (kBlock * kU));
+    hipLaunchKernelGGL((k_loop64_classify<K, kU>), dim3(blocks), dim3(kBlock), 0, s, c, f, slots, tb);
     return hipGetLastError();
   }
 };
