@@ -1712,62 +1712,111 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
 // consecutive lanes of one target (a sphere's front and back) run one chain
 // carrying both keys, as in k_loop64_store, with the set rule: a carried key
 // that meets its equal in a slot, or equals the other carried key, is done.
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, uint32_t* __restrict__ zs) {
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  const bool in = i < f.count;
-  uint32_t t = in ? ld_stream(&f.target[i]) : kEmpty;
-  const float dv = in ? ld_stream(&f.depth[i]) : 0.f;  // with the target: one round trip
-  if (t >= c.T) t = kEmpty;
-  uint32_t z = 0u;
-  if (t != kEmpty) {
-    z = depth_key(dv);
-    if (!opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) t = kEmpty;
-  }
-  const Run r = same_target_run(t);
-  const uint32_t zn = __shfl_down(z, 1u);
-  const bool lead = t != kEmpty && ((r.lane - r.start_lane) & 1u) == 0u;
-  if (!lead) return;  // the odd lanes of a run travel with their even neighbour
-  uint32_t a = z, b = 0u;
-  bool hasb = r.lane < r.end_lane && zn != z;
-  if (hasb) {
-    a = z < zn ? z : zn;
-    b = z < zn ? zn : z;
-  }
-  constexpr int kProbe = K / 2 < 8 ? K / 2 : 8;  // as in k_loop64_store
-  uint32_t probe[kProbe > 0 ? kProbe : 1];
+// Each wave takes kU chunks of 64 consecutive fragments: the chunks' loads,
+// pretests (with four probe slots) and atomic chains are in flight together,
+// one dependent round trip per chain step of all of them.
+template <int K, int kU>
+__global__ __launch_bounds__(kBlock) void k_loop32_depth_m(Common c, FragView f, uint32_t* __restrict__ zs) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t first = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
+  uint32_t t[kU];
+  float dv[kU];
 #pragma unroll
-  for (int k = 0; k < kProbe; ++k) probe[k] = zs[(uint64_t)k * c.T + t];
-  const uint32_t z_last = zs[(uint64_t)(K - 1) * c.T + t], z_mid = zs[(uint64_t)(K / 2) * c.T + t];
-  if (a > z_last) return;
-  int j = 0;
-  if (a > z_mid) {
-    j = K / 2;
-  } else {
-#pragma unroll
-    for (int k = 0; k < kProbe; ++k) j += probe[k] < a ? 1 : 0;
+  for (int u = 0; u < kU; ++u) {  // every chunk's target and depth in one round trip
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    const bool in = i < f.count;
+    t[u] = in ? ld_stream(&f.target[i]) : kEmpty;
+    dv[u] = in ? ld_stream(&f.depth[i]) : 0.f;
   }
-  for (; j < K; ++j) {
-    const uint32_t old =
-        __hip_atomic_fetch_min(&zs[(uint64_t)j * c.T + t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == kEmpty || old == a) {  // a is in slot j now (or was already): b goes on
-      if (!hasb) break;
-      a = b;
-      hasb = false;
-      continue;
+  uint32_t a[kU], b[kU];
+  bool hasb[kU], act[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    uint32_t tt = t[u] < c.T ? t[u] : kEmpty;
+    uint32_t z = 0u;
+    if (tt != kEmpty) {
+      z = depth_key(dv[u]);
+      if (!opaque_visible(c.opq_z, c.T, 1u, tt, z, 1u)) tt = kEmpty;
     }
-    if (old < a) continue;  // a smaller key holds slot j: a goes on unchanged
-    // a took slot j and old comes out
-    if (!hasb) {
-      a = old;
-    } else if (old == b) {  // the two carried keys are equal: one of them is done
-      a = b;
-      hasb = false;
-    } else if (old < b) {
-      a = old;
-    } else {
-      a = b;
-      b = old;
+    const Run r = same_target_run(tt);
+    const uint32_t zn = __shfl_down(z, 1u);
+    const bool lead = tt != kEmpty && ((r.lane - r.start_lane) & 1u) == 0u;
+    hasb[u] = lead && r.lane < r.end_lane && zn != z;
+    a[u] = z;
+    b[u] = 0u;
+    if (hasb[u]) {
+      a[u] = z < zn ? z : zn;
+      b[u] = z < zn ? zn : z;
+    }
+    act[u] = lead;
+    t[u] = tt;
+  }
+  // the pretests and the first slots of every chunk in one round trip
+  constexpr int kProbe = K / 2 < 4 ? K / 2 : 4;
+  int j[kU];
+  {
+    uint32_t probe[kU][kProbe > 0 ? kProbe : 1], z_last[kU], z_mid[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (act[u]) {
+#pragma unroll
+        for (int k = 0; k < kProbe; ++k) probe[u][k] = zs[(uint64_t)k * c.T + t[u]];
+        z_last[u] = zs[(uint64_t)(K - 1) * c.T + t[u]];
+        z_mid[u] = zs[(uint64_t)(K / 2) * c.T + t[u]];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      j[u] = 0;
+      if (act[u]) {
+        if (a[u] > z_last[u]) {
+          act[u] = false;
+        } else if (a[u] > z_mid[u]) {
+          j[u] = K / 2;
+        } else {
+#pragma unroll
+          for (int k = 0; k < kProbe; ++k) j[u] += probe[u][k] < a[u] ? 1 : 0;
+        }
+      }
+    }
+  }
+  // the chains, one step of every live chain per round trip
+  for (;;) {
+    uint32_t old[kU];
+    bool any = false;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (act[u]) {
+        old[u] = __hip_atomic_fetch_min(&zs[(uint64_t)j[u] * c.T + t[u]], a[u], __ATOMIC_RELAXED,
+                                        __HIP_MEMORY_SCOPE_AGENT);
+        any = true;
+      }
+    }
+    if (!any) break;
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      if (!act[u]) continue;
+      if (old[u] == kEmpty || old[u] == a[u]) {  // a is in slot j now (or was already): b goes on
+        if (hasb[u]) {
+          a[u] = b[u];
+          hasb[u] = false;
+        } else {
+          act[u] = false;
+        }
+      } else if (old[u] > a[u]) {  // a took slot j and old comes out
+        if (!hasb[u]) {
+          a[u] = old[u];
+        } else if (old[u] == b[u]) {  // the two carried keys are equal: one of them is done
+          a[u] = b[u];
+          hasb[u] = false;
+        } else if (old[u] < b[u]) {
+          a[u] = old[u];
+        } else {
+          a[u] = b[u];
+          b[u] = old[u];
+        }
+      }  // else a smaller key holds slot j: a goes on unchanged
+      if (++j[u] >= K) act[u] = false;
     }
   }
 }
@@ -1776,51 +1825,69 @@ __global__ __launch_bounds__(kBlock) void k_loop32_depth(Common c, FragView f, u
 // colour; else the slot of its depth by binary search, where the colour of the
 // latest fragment in primitive order is kept ((id + 1) << 32 | colour,
 // atomicMax), the result of the reference's plain store in draw order.
-template <int K, bool kTail>
-__global__ __launch_bounds__(kBlock) void k_loop32_color(Common c, FragView f, const uint32_t* __restrict__ zs,
-                                                         uint64_t* __restrict__ cs, TailBuf tb) {
+// Each wave takes kU chunks of 64 consecutive fragments: every chunk's
+// fields, then every chunk's lower-half depth slots, in one round trip each;
+// the upper half only where a pixel holds more than K/2 depths below the
+// fragment's.
+template <int K, bool kTail, int kU>
+__global__ __launch_bounds__(kBlock) void k_loop32_color_m(Common c, FragView f, const uint32_t* __restrict__ zs,
+                                                           uint64_t* __restrict__ cs, TailBuf tb) {
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  bool valid = i < f.count;
-  // the fragment's fields in one round trip, before any is tested
-  uint32_t t = valid ? ld_stream(&f.target[i]) : kEmpty;
-  const float dv = valid ? ld_stream(&f.depth[i]) : 0.f;
-  const float4 cv = valid ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
-  if (t >= c.T) {
-    valid = false;
-    t = kEmpty;
-  }
-  bool tail = false;
-  float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
-  const uint32_t z = valid ? depth_key(dv) : 0u;
-  if (valid && !opaque_visible(c.opq_z, c.T, 1u, t, z, 1u)) {
-    valid = false;
-    t = kEmpty;
-  }
-  if (valid) {
-    col = cv;
-    // the slot of z: the number of stored depths below it. Slots are sorted
-    // with the empty ones last, so the lower half loaded at once decides
-    // unless all of it lies below z (a pixel holding more than K/2 depths):
-    // only then is the upper half loaded
-    uint32_t below = 0;
-    constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t first = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
+  uint32_t t[kU], z[kU];
+  float dv[kU];
+  float4 cv[kU];
+  bool valid[kU];
 #pragma unroll
-    for (int k = 0; k < kHalf; ++k) below += zs[(uint64_t)k * c.T + t] < z ? 1u : 0u;
-    if (below == (uint32_t)kHalf) {
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    valid[u] = i < f.count;
+    t[u] = valid[u] ? ld_stream(&f.target[i]) : kEmpty;
+    dv[u] = valid[u] ? ld_stream(&f.depth[i]) : 0.f;
+    cv[u] = valid[u] ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+  }
 #pragma unroll
-      for (int k = kHalf; k < K; ++k) below += zs[(uint64_t)k * c.T + t] < z ? 1u : 0u;
-    }
-    if (below == (uint32_t)K) {
-      tail = true;
-    } else {
-      const int start = (int)below;
-      const uint64_t v = ((uint64_t)(f.id_base + i + 1u) << 32) | encode_color_lds(col, s_thr);
-      (void)__hip_atomic_fetch_max(&cs[(uint64_t)start * c.T + t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (int u = 0; u < kU; ++u) {
+    if (t[u] >= c.T) valid[u] = false;
+    z[u] = valid[u] ? depth_key(dv[u]) : 0u;
+    if (valid[u] && !opaque_visible(c.opq_z, c.T, 1u, t[u], z[u], 1u)) valid[u] = false;
+    if (!valid[u]) t[u] = kEmpty;
+  }
+  constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
+  uint32_t below[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    below[u] = 0;
+    if (valid[u]) {
+#pragma unroll
+      for (int k = 0; k < kHalf; ++k) below[u] += zs[(uint64_t)k * c.T + t[u]] < z[u] ? 1u : 0u;
     }
   }
-  if constexpr (kTail) tail_record_own(tail, t, f.id_base + i, 0u, col, tb);
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    if (valid[u] && below[u] == (uint32_t)kHalf) {
+#pragma unroll
+      for (int k = kHalf; k < K; ++k) below[u] += zs[(uint64_t)k * c.T + t[u]] < z[u] ? 1u : 0u;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    const uint32_t lid = f.id_base + (uint32_t)i;
+    bool tail = false;
+    if (valid[u]) {
+      if (below[u] == (uint32_t)K) {
+        tail = true;
+      } else {
+        const uint64_t v = ((uint64_t)(lid + 1u) << 32) | encode_color_lds(cv[u], s_thr);
+        (void)__hip_atomic_fetch_max(&cs[(uint64_t)below[u] * c.T + t[u]], v, __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
+    if constexpr (kTail) tail_record_own(tail, t[u], lid, 0u, valid[u] ? cv[u] : make_float4(0.f, 0.f, 0.f, 0.f), tb);
+  }
 }
 
 // Composite (:186-222): the layers up to the first empty depth, front to back,
@@ -3081,7 +3148,10 @@ namespace {
 template <int K>
 struct Loop32Depth {
   static hipError_t run(const Common& c, const FragView& f, uint32_t* zs, hipStream_t s) {
-    hipLaunchKernelGGL((k_loop32_depth<K>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, zs);
+    if (f.count == 0) return hipSuccess;
+    constexpr int kU = 2;  // 64-fragment chunks per wave
+    const uint32_t blocks = (uint32_t)((f.count + kBlock * kU - 1) / (kBlock * kU));
+    hipLaunchKernelGGL((k_loop32_depth_m<K, kU>), dim3(blocks), dim3(kBlock), 0, s, c, f, zs);
     return hipGetLastError();
   }
 };
@@ -3090,10 +3160,11 @@ template <int K>
 struct Loop32Color {
   static hipError_t run(const Common& c, const FragView& f, const uint32_t* zs, uint64_t* cs, const TailBuf& tb,
                         hipStream_t s) {
-    if (c.tail && tb.head) hipLaunchKernelGGL((k_loop32_color<K, true>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s,
-                                              c, f, zs, cs, tb);
-    else hipLaunchKernelGGL((k_loop32_color<K, false>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, zs, cs,
-                            tb);
+    if (f.count == 0) return hipSuccess;
+    constexpr int kU = 2;  // 64-fragment chunks per wave
+    const dim3 g((uint32_t)((f.count + kBlock * kU - 1) / (kBlock * kU))), b(kBlock);
+    if (c.tail && tb.head) hipLaunchKernelGGL((k_loop32_color_m<K, true, kU>), g, b, 0, s, c, f, zs, cs, tb);
+    else hipLaunchKernelGGL((k_loop32_color_m<K, false, kU>), g, b, 0, s, c, f, zs, cs, tb);
     return hipGetLastError();
   }
 };
