@@ -1,7 +1,7 @@
 """Reconfiguration reuses device buffers: a context that serves frames of
 changing size, K and algorithm keeps its allocations while they fit, stays
 bit-exact with the oracle on every frame (the Linked List pool, the Loop64
-and Loop32 slots and the colour-pass tail queue are kept clean across sizes,
+and Loop32 slots, the WBOIT accumulators and the colour-pass tail queue are kept clean across sizes,
 algorithms and frames cleared before their resolve), and oit_trim releases
 everything."""
 import numpy as np
@@ -73,6 +73,13 @@ def test_reconfigure_sequence_stays_exact(oracle):
             (LOOP32, 16, 1, 36, 63, 37, 150, 0, True),
             (LOOP32, 4, 0, 37, 48, 40, 120, 0, False),
             (LOOP32, 4, 1, 38, 48, 40, 200, 0, False),
+            # WBOIT accumulators: emptied by the composite, refilled after a
+            # reconfiguration or an unresolved frame
+            (WB, 1, 0, 39, 64, 48, 200, 0, False),
+            (WB, 1, 0, 40, 64, 48, 160, 0, False),
+            (WB, 1, 0, 41, 64, 48, 200, 0, True),
+            (WB, 1, 0, 42, 63, 37, 150, 0, False),
+            (WB, 1, 0, 43, 63, 37, 150, 0, False),
         ]
         resident = []
         for alg, K, tail, seed, w, h, sp, pf, dirty in seq:

@@ -111,6 +111,7 @@ struct oit_ctx {
   uint64_t* l64 = nullptr;
   bool l64_clean = false;  // every slot empty (a fill, or the last frame's composite)
   bool l32_clean = false;  // Loop32: likewise for the depth and colour slots
+  bool wb_clean = false;   // WBOIT: accumulators at the clear values (a fill, or the last composite)
   // WBOIT
   float4* wb_accum = nullptr;
   float* wb_reveal = nullptr;
@@ -264,6 +265,7 @@ void free_frame_buffers(oit_ctx* c) {
   c->l64_clean = false;
   dev_free(&c->wb_accum);
   dev_free(&c->wb_reveal);
+  c->wb_clean = false;
   dev_free(&c->tl_head);
   dev_free(&c->tl_next);
   dev_free(&c->tl_rgba);
@@ -703,6 +705,7 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   c->tl_head_valid = false;
   c->l64_clean = false;  // slots beyond the last frame's T x K may hold an older frame's keys
   c->l32_clean = false;
+  c->wb_clean = false;
   if (pooled(alg) && ll_grown) {
     // a new allocation is filled clean (every head 0, every link kUnset) by
     // the next clear, on its stream
@@ -793,7 +796,7 @@ int oit_clear(oit_ctx* c, void* stream) {
   int nf = 0;
   auto fill = [&](void* p, uint64_t n, uint32_t v) { fr[nf++] = oit::FillRange{static_cast<uint32_t*>(p), n, v}; };
   if (need_counters) fill(c->d_counters, 16, 0u);
-  bool l64_fill = false, l32_fill = false;
+  bool l64_fill = false, l32_fill = false, wb_fill = false;
   switch (c->cfg.algorithm) {
     case OIT_LOOP:  // depths to all ones (oitRender.cpp:242-265), colour slots to 0; the composite leaves them so
       l32_fill = !c->l32_clean;
@@ -821,9 +824,12 @@ int oit_clear(oit_ctx* c, void* stream) {
       l64_fill = !c->l64_clean;
       if (l64_fill) fill(c->l64, 2 * T * K, kEmpty);
       break;
-    case OIT_WEIGHTED:  // accum 0, reveal 1 (log2 0), oitRender.cpp:394-397
-      fill(c->wb_accum, 4 * T, 0u);
-      fill(c->wb_reveal, T, 0u);
+    case OIT_WEIGHTED:  // accum 0, reveal 1 (log2 0), oitRender.cpp:394-397; the composite leaves them so
+      wb_fill = !c->wb_clean;
+      if (wb_fill) {
+        fill(c->wb_accum, 4 * T, 0u);
+        fill(c->wb_reveal, T, 0u);
+      }
       break;
   }
   if (need_tail) {
@@ -833,6 +839,7 @@ int oit_clear(oit_ctx* c, void* stream) {
   hipError_t e = staged(c, kStClear, s, [&]() -> hipError_t { return oit::launch_fill_ranges(fr, nf, s); });
   if (e == hipSuccess && l64_fill) c->l64_clean = true;
   if (e == hipSuccess && l32_fill) c->l32_clean = true;
+  if (e == hipSuccess && wb_fill) c->wb_clean = true;
   if (e == hipSuccess && ll) {
     c->ll_clean = true;
     c->ll_dirty_upto = 0;
@@ -981,6 +988,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
     }
     case OIT_WEIGHTED: {
       oit::WBBuf wb{c->wb_accum, c->wb_reveal};
+      c->wb_clean = false;
       e = staged(c, kStStore, s, [&] {
         return oit::launch_wboit_store(k, v, wb, c->cfg.wboit_view_near, c->cfg.wboit_view_span, s);
       });
@@ -1093,6 +1101,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     case OIT_WEIGHTED: {
       oit::WBBuf wb{c->wb_accum, c->wb_reveal};
       e = staged(c, kStResolve, s, [&] { return oit::launch_wboit_resolve(k, wb, o, s); });
+      c->wb_clean = e == hipSuccess;  // the composite set every accumulator back to the clear values
       break;
     }
   }
@@ -1143,6 +1152,7 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
     c->counters_clean = false;
     c->l64_clean = false;
     c->l32_clean = false;
+    c->wb_clean = false;
     return OIT_ERR_DEVICE;
   }
   return OIT_OK;

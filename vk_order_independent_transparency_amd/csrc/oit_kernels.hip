@@ -2541,12 +2541,24 @@ __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WB
 
 // Composite (oitWeighted.frag.glsl:98-109) with the WEIGHTED_COMPOSITE blend
 // (main.cpp:578-588): dst = src * (1 - reveal) + dst * reveal.
+// The accumulators it read are set back to the clear values (0, and 0 for
+// the log2 revealage) for the next frame, where they are not already, so
+// the next clear need not fill them.
 __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, ResolveOut o) {
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
-  const float* acc = reinterpret_cast<const float*>(wb.accum);
+  float* acc = reinterpret_cast<float*>(wb.accum);
   const float4 A = make_float4(acc[t], acc[(uint64_t)c.T + t], acc[2ull * c.T + t], acc[3ull * c.T + t]);
-  const float r = exp2f(wb.reveal[t]);  // product of (1 - alpha), kept as a log2 sum
+  const float lr = wb.reveal[t];
+  if ((__float_as_uint(A.x) | __float_as_uint(A.y) | __float_as_uint(A.z) | __float_as_uint(A.w) |
+       __float_as_uint(lr)) != 0u) {
+    acc[t] = 0.f;
+    acc[(uint64_t)c.T + t] = 0.f;
+    acc[2ull * c.T + t] = 0.f;
+    acc[3ull * c.T + t] = 0.f;
+    wb.reveal[t] = 0.f;
+  }
+  const float r = exp2f(lr);  // product of (1 - alpha), kept as a log2 sum
   const float den = fmaxf(A.w, 1e-5f);
   const float4 s = make_float4(A.x / den, A.y / den, A.z / den, r);
   const float om = 1.0f - r;

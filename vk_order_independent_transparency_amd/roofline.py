@@ -63,10 +63,15 @@ def kbuffer_stage_bytes(F: int, P: int, K: int, spin: bool) -> dict:
 
 
 def wboit_stage_bytes(F: int, P: int) -> dict:
+    """WBOIT: no clear (the composite sets the accumulators it read back to
+    the clear values where they are not already; counted here for every
+    target, an upper bound); store: stream in 24F + accumulation
+    read-modify-write 40F; composite: accumulators read 20P and reset 20P,
+    RGBA out 16P."""
     return {
-        "clear": 20 * P,
+        "clear": 0,
         "store": 24 * F + 40 * F,  # stream in + accumulation read-modify-write
-        "resolve": 20 * P + 16 * P,
+        "resolve": 20 * P + 20 * P + 16 * P,
     }
 
 
