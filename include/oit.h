@@ -154,7 +154,10 @@ int oit_clear(oit_ctx* ctx, void* stream);
  * 313-322, 358-368, 384-414) running oitSimple/oitLinkedList/oitLoop64/
  * oitSpinlock/oitWeighted.frag.glsl with PASS == PASS_COLOR. The fragment
  * buffers must stay valid until oit_resolve has been enqueued when the
- * algorithm is OIT_LOOP64 with tail blending (the resolve re-reads them). */
+ * algorithm is OIT_LOOP64 or OIT_LOOP: Loop64 counts each pixel's fragments
+ * here and stores them in the resolve, once every append of the frame is
+ * counted (a pixel of at most K fragments keeps them all and needs no
+ * atomicMin chain); Loop32 runs its second geometry pass there. */
 int oit_append(oit_ctx* ctx, const oit_fragments* frags, void* stream);
 /* Composite pass: writes the final premultiplied RGBA per target
  * (composite over tail-blended colour buffer over background) to

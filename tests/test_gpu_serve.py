@@ -100,7 +100,7 @@ def test_budget_is_respected():
         if r.algorithm == _abi.OIT_LINKEDLIST:
             need.append(4 * T + 12 * (F + 1) + 4 * T)
         elif r.algorithm == _abi.OIT_LOOP64:
-            need.append(8 * r.layers * T + 4 * T + 28 * F + 4 * (-(-F // 1024)))
+            need.append(8 * r.layers * T + 4 * T + 8 * (-(-F // 64)) + 4 * T + 28 * F + 4 * (-(-F // 1024)))
         elif r.algorithm == _abi.OIT_SIMPLE:
             need.append(4 * T + 12 * r.layers * T + 4 * T + 28 * F + 4 * (-(-F // 1024)))
         elif r.algorithm == _abi.OIT_LOOP:

@@ -109,7 +109,9 @@ struct oit_ctx {
   uint32_t* kb_id = nullptr;
   // Loop64
   uint64_t* l64 = nullptr;
-  bool l64_clean = false;  // every slot empty (a fill, or the last frame's composite)
+  uint32_t* l64_cnt = nullptr;  // [T] the frame's fragments per target (0 = clean)
+  uint64_t* l64_big = nullptr;  // per 64-fragment chunk of the frame: lanes whose pixel holds more than K
+  bool l64_clean = false;  // every slot empty and every count 0 (a fill, or the last frame's composite)
   bool l32_clean = false;  // Loop32: likewise for the depth and colour slots
   bool wb_clean = false;   // WBOIT: accumulators at the clear values (a fill, or the last composite)
   // WBOIT
@@ -137,7 +139,7 @@ struct oit_ctx {
   // frame sizes stops allocating once it has seen the largest
   uint64_t cap_ll_head = 0, ll_alloc = 0;
   uint64_t cap_kb_count = 0, cap_kb_lock = 0, cap_kb_depth = 0, cap_kb_elems = 0;
-  uint64_t cap_l64 = 0, cap_wb = 0, cap_tl_head = 0;
+  uint64_t cap_l64 = 0, cap_l64_cnt = 0, cap_l64_big = 0, cap_wb = 0, cap_tl_head = 0;
   // coverage shading: sample masks of nodes, k-buffer slots and tail entries
   uint32_t* ll_mask = nullptr;
   float4* il_full = nullptr;   // ordered interlock: shaded colour per node [ll_alloc]
@@ -238,7 +240,7 @@ void free_frame_buffers(oit_ctx* c) {
   wait_idle(c);
   c->cap_ll_head = c->ll_alloc = 0;
   c->cap_kb_count = c->cap_kb_lock = c->cap_kb_depth = c->cap_kb_elems = 0;
-  c->cap_l64 = c->cap_wb = c->cap_tl_head = 0;
+  c->cap_l64 = c->cap_l64_cnt = c->cap_l64_big = c->cap_wb = c->cap_tl_head = 0;
   c->cap_ll_mask = c->cap_kb_mask = c->cap_tl_mask = 0;
   dev_free(&c->ll_mask);
   dev_free(&c->il_full);
@@ -262,6 +264,8 @@ void free_frame_buffers(oit_ctx* c) {
   dev_free(&c->kb_key);
   dev_free(&c->kb_id);
   dev_free(&c->l64);
+  dev_free(&c->l64_cnt);
+  dev_free(&c->l64_big);
   c->l64_clean = false;
   dev_free(&c->wb_accum);
   dev_free(&c->wb_reveal);
@@ -605,7 +609,11 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
     release(c, &c->l32_c, &c->cap_l32);
   }
   if (alg != OIT_SIMPLE && alg != OIT_SPINLOCK) release(c, &c->kb_mask, &c->cap_kb_mask);
-  if (alg != OIT_LOOP64) release(c, &c->l64, &c->cap_l64);
+  if (alg != OIT_LOOP64) {
+    release(c, &c->l64, &c->cap_l64);
+    release(c, &c->l64_cnt, &c->cap_l64_cnt);
+    release(c, &c->l64_big, &c->cap_l64_big);
+  }
   if (alg != OIT_WEIGHTED) {
     uint64_t d = c->cap_wb;
     release(c, &c->wb_accum, &d);
@@ -660,8 +668,9 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
     }
     case OIT_LOOP64: {  // oit.cpp:134-140
       rc = reserve(c, &c->l64, &c->cap_l64, T * K);
+      if (!rc) rc = reserve(c, &c->l64_cnt, &c->cap_l64_cnt, T);
       c->bytes_abuf = T * K * 8u;
-      c->bytes_aux = 0;
+      c->bytes_aux = T * 4u;
       break;
     }
     case OIT_WEIGHTED: {  // oit.h:234-235 (binary32 here)
@@ -758,7 +767,7 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
   }
   out->targets = c->T;
   out->resident_bytes = c->cap_ll_head * 4u + c->ll_alloc * 12u + c->cap_kb_count * 4u + c->cap_kb_lock * 4u +
-                        c->cap_kb_depth * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_wb * 20u +
+                        c->cap_kb_depth * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_l64_cnt * 4u + c->cap_l64_big * 8u + c->cap_wb * 20u +
                         c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u) + oit::tail_regions(c->tl_cap) * 4u +
                         (c->cap_ll_mask + c->cap_kb_mask + c->cap_tl_mask) * 4u + c->cap_il_full * 16u +
                         c->cap_l32 * 12u + c->cap_opq * 8u + c->cap_opq_rgba * 16u;
@@ -822,7 +831,10 @@ int oit_clear(oit_ctx* c, void* stream) {
       break;
     case OIT_LOOP64:  // every byte to 0xFF, oitRender.cpp:307; the composite leaves its slots empty
       l64_fill = !c->l64_clean;
-      if (l64_fill) fill(c->l64, 2 * T * K, kEmpty);
+      if (l64_fill) {
+        fill(c->l64, 2 * T * K, kEmpty);
+        fill(c->l64_cnt, T, 0u);
+      }
       break;
     case OIT_WEIGHTED:  // accum 0, reveal 1 (log2 0), oitRender.cpp:394-397; the composite leaves them so
       wb_fill = !c->wb_clean;
@@ -978,12 +990,10 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       e = staged(c, kStStore, s, [&] { return oit::launch_spin_store(k, v, kb, tb, s); });
       break;
     }
-    case OIT_LOOP64: {
-      const int chunks = (c->cfg.variant & OIT_VARIANT_STORE_CHUNKS_1) ? 1
-                         : (c->cfg.variant & OIT_VARIANT_STORE_CHUNKS_4) ? 4 : 2;
+    case OIT_LOOP64: {  // counted now; stored at resolve, once every pixel's count is known
       c->l64_clean = false;
-      e = staged(c, kStStore, s, [&] { return oit::launch_loop64_store(k, v, c->l64, chunks, s); });
-      if (c->cfg.tail_blend) c->segments.push_back(v);
+      e = staged(c, kStStore, s, [&] { return oit::launch_loop64_count(k, v, c->l64_cnt, s); });
+      c->segments.push_back(v);
       break;
     }
     case OIT_WEIGHTED: {
@@ -1019,10 +1029,28 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
   const oit::TailBuf tb = tail_of(c);
   oit::ResolveOut o{reinterpret_cast<float4*>(out_rgba), fk_key, fk_val, counts};
   hipError_t e = hipSuccess;
-  if (link_tails && alg == OIT_LOOP64 && c->cfg.tail_blend && tb.head) {
+  if (link_tails && alg == OIT_LOOP64 && !c->segments.empty()) {  // the stores of every segment (counted at append)
+    const int chunks = (c->cfg.variant & OIT_VARIANT_STORE_CHUNKS_1) ? 1
+                       : (c->cfg.variant & OIT_VARIANT_STORE_CHUNKS_4) ? 4 : 2;
+    uint64_t masks = 0;  // one per 64-fragment chunk of each segment
+    for (const auto& v : c->segments) masks += (v.count + 63u) / 64u;
+    int rc = reserve(c, &c->l64_big, &c->cap_l64_big, masks ? masks : 1);
+    if (rc) return rc;
+    uint64_t m0 = 0;
     for (const auto& v : c->segments) {
-      e = staged(c, kStClassify, s, [&] { return oit::launch_loop64_classify(k, v, c->l64, tb, s); });
+      uint64_t* big = c->l64_big + m0;
+      e = staged(c, kStStore, s, [&] { return oit::launch_loop64_store(k, v, c->l64, c->l64_cnt, big, chunks, s); });
       if (e != hipSuccess) return status_of(e);
+      m0 += (v.count + 63u) / 64u;
+    }
+    if (c->cfg.tail_blend && tb.head) {
+      m0 = 0;
+      for (const auto& v : c->segments) {
+        const uint64_t* big = c->l64_big + m0;
+        e = staged(c, kStClassify, s, [&] { return oit::launch_loop64_classify(k, v, c->l64, big, tb, s); });
+        if (e != hipSuccess) return status_of(e);
+        m0 += (v.count + 63u) / 64u;
+      }
     }
   }
   if (link_tails && alg == OIT_LOOP) {  // the second geometry pass (oitLoop.frag.glsl:105-180)
@@ -1077,7 +1105,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
       break;
     }
     case OIT_LOOP64:
-      e = staged(c, kStResolve, s, [&] { return oit::launch_loop64_resolve(k, c->l64, tbr, o, s); });
+      e = staged(c, kStResolve, s, [&] { return oit::launch_loop64_resolve(k, c->l64, c->l64_cnt, tbr, o, s); });
       c->l64_clean = e == hipSuccess;  // the composite emptied every slot it read
       break;
     case OIT_LOOP:

@@ -2164,90 +2164,88 @@ __global__ __launch_bounds__(kBlock, (K == 16 && !kCov) ? 6 : 1) void k_interloc
 // Loop64: oitLoop64.frag.glsl
 // ---------------------------------------------------------------------------
 
-// Colour pass (oitLoop64.frag.glsl:65-141): key = depth << 32 | colour (:79),
-// the two pretests of USE_EARLYDEPTH (:82-101, plain loads: slots only
-// decrease within a frame, so a stale value only lets more fragments try),
-// then the atomicMin chain (:109-121). Slots end as the K smallest keys for
-// any arrival order; evictions are classified afterwards.
-// Two consecutive lanes of one target (a sphere's front and back fragment)
-// run one chain that carries both keys: at each slot the smaller carried key
-// goes in, the key that comes out joins the larger one, and the chain ends
-// when both are placed or pushed past slot K-1. The multiset of slots and
-// carried keys is the same as after the two single chains, and the slots
-// stay sorted, so the result is identical; the pair costs one chain instead
-// of two and its lanes do not contend for the same slots.
-template <int K>
-__global__ __launch_bounds__(kBlock) void k_loop64_store(Common c, FragView f, uint64_t* __restrict__ slots) {
-  __shared__ float s_thr[257];
-  stage_thresholds(s_thr, c.tab);
-  const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-  // the fragment's fields in one round trip, before any is tested
-  const bool in = i < f.count;
-  uint32_t t = in ? ld_stream(&f.target[i]) : kEmpty;
-  const float dv = in ? ld_stream(&f.depth[i]) : 0.f;
-  const float4 cv = in ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
-  if (t >= c.T) t = kEmpty;
-  uint64_t z = kEmpty64;
-  if (t != kEmpty) {
-    const uint32_t key = depth_key(dv);
-    if (opaque_visible(c.opq_z, c.T, 1u, t, key, 1u)) z = ((uint64_t)key << 32) | encode_color_lds(cv, s_thr);
-    else t = kEmpty;
-  }
-  const Run r = same_target_run(t);
-  const uint32_t zn_lo = __shfl_down((uint32_t)z, 1u);
-  const uint32_t zn_hi = __shfl_down((uint32_t)(z >> 32), 1u);
-  const uint64_t zn = ((uint64_t)zn_hi << 32) | zn_lo;
-  const bool lead = t != kEmpty && ((r.lane - r.start_lane) & 1u) == 0u;
-  if (!lead) return;  // the odd lanes of a run travel with their even neighbour
-  bool hasb = r.lane < r.end_lane;
-  uint64_t a = z, b = 0;
-  if (hasb) {
-    a = z < zn ? z : zn;
-    b = z < zn ? zn : z;
-  }
-  // the pretests, and the first slots loaded with them: slots are sorted and
-  // only decrease, so every loaded slot below a is still below it and the
-  // chain can start past them (a slot that has dropped below a since only
-  // costs a no-op atomic)
-  constexpr int kProbe = K / 2 < 8 ? K / 2 : 8;
-  uint64_t probe[kProbe > 0 ? kProbe : 1];
+// Ascending sort of N (a power of two) values held in registers: a bitonic
+// network with compile-time indices (no spills, no dynamic indexing).
+template <int N>
+__device__ __forceinline__ void sort_u64(uint64_t* v) {
 #pragma unroll
-  for (int k = 0; k < kProbe; ++k) probe[k] = slots[(uint64_t)k * c.T + t];
-  const uint64_t s_last = slots[(uint64_t)(K - 1) * c.T + t], s_mid = slots[(uint64_t)(K / 2) * c.T + t];
-  if (a > s_last) return;
-  int j = 0;
-  if (a > s_mid) {
-    j = K / 2;
-  } else {
+  for (int size = 2; size <= N; size <<= 1) {
 #pragma unroll
-    for (int k = 0; k < kProbe; ++k) j += probe[k] < a ? 1 : 0;
-  }
-  for (; j < K; ++j) {
-    const uint64_t old =
-        __hip_atomic_fetch_min(&slots[(uint64_t)j * c.T + t], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == kEmpty64) {  // a placed in an empty slot: b (if any) goes to the next one
-      if (!hasb) break;
-      a = b;
-      hasb = false;
-      continue;
-    }
-    const uint64_t up = old > a ? old : a;  // the key that leaves slot j
-    if (hasb && b < up) {
-      a = b;
-      b = up;
-    } else {
-      a = up;
+    for (int stride = size >> 1; stride > 0; stride >>= 1) {
+#pragma unroll
+      for (int i = 0; i < N; ++i) {
+        const int j = i ^ stride;
+        if (j > i) {
+          const bool up = (i & size) == 0;
+          const uint64_t x = v[i], y = v[j];
+          const bool swap = up ? (x > y) : (x < y);
+          v[i] = swap ? y : x;
+          v[j] = swap ? x : y;
+        }
+      }
     }
   }
 }
 
-// k_loop64_store with kU chunks of 64 consecutive fragments per wave: the
-// chunks' loads, probes and atomic chains are in flight together, so a
-// wave waits for one dependent round trip per chain step of all its chains
-// instead of one per step of each. Same pairs, probes and chain rule per
-// chunk as k_loop64_store, so the slots end the same.
+// A chunk's mask, read as one wave-uniform (scalar) value.
+__device__ __forceinline__ uint64_t big_mask_load(const uint64_t* p) {
+  const uint64_t m = *p;
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)m);
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(m >> 32));
+  return ((uint64_t)hi << 32) | lo;
+}
+
+// Per-target count of the frame's fragments that pass the opaque depth
+// test: one returnless atomic per run of one target in a wave. Runs at
+// append, so that the stores, at resolve, know every pixel's final count.
+// kU chunks of 64 fragments per wave, their loads in one round trip; the
+// depths are read only when an opaque pass can cull.
+template <int kU>
+__global__ __launch_bounds__(kBlock) void k_loop64_count(Common c, FragView f, uint32_t* __restrict__ cnt) {
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t first = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
+  const bool culls = c.opq_z != nullptr;
+  uint32_t t[kU];
+  float dv[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t i = first + (uint64_t)u * 64u + lane;
+    const bool in = i < f.count;
+    t[u] = in ? ld_stream(&f.target[i]) : kEmpty;
+    dv[u] = (in && culls) ? ld_stream(&f.depth[i]) : 0.f;
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    uint32_t tt = t[u] < c.T ? t[u] : kEmpty;
+    if (culls && tt != kEmpty && !opaque_visible(c.opq_z, c.T, 1u, tt, depth_key(dv[u]), 1u)) tt = kEmpty;
+    const Run r = same_target_run(tt);
+    if (tt != kEmpty && r.start) atomicAdd(&cnt[tt], r.end_lane - r.lane + 1u);
+  }
+}
+
+// Colour pass (oitLoop64.frag.glsl:65-141): key = depth << 32 | colour (:79).
+// The slots end as the K smallest keys of each pixel for any arrival order;
+// this pass gets there by one of two routes per pixel, chosen by the
+// pixel's count of the frame's fragments:
+// - at most K fragments: every one of them is kept, so each takes a slot
+//   index from its count (one returning atomic per run of one target,
+//   counting down to 0) and stores its key unsorted; the composite sorts
+//   the slots. No fragment of such a pixel is in the tail.
+// - more than K: the two pretests of USE_EARLYDEPTH (:82-101, plain loads:
+//   slots only decrease within a frame, so a stale value only lets more
+//   fragments try), then the atomicMin chain (:109-121), which keeps the
+//   slots sorted. Two consecutive lanes of one target (a sphere's front and
+//   back fragment) run one chain that carries both keys: at each slot the
+//   smaller carried key goes in, the key that comes out joins the larger
+//   one, and the chain ends when both are placed or pushed past slot K-1.
+//   The multiset of slots and carried keys is the same as after the two
+//   single chains, so the result is identical; the pair costs one chain
+//   instead of two and its lanes do not contend for the same slots.
+// Each wave takes kU chunks of 64 consecutive fragments: the chunks' loads,
+// counts, pretests and chain steps are in flight together.
 template <int K, int kU>
-__global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f, uint64_t* __restrict__ slots) {
+__global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f, uint64_t* __restrict__ slots,
+                                                           uint32_t* __restrict__ cnt, uint64_t* __restrict__ big) {
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
   const uint32_t lane = threadIdx.x & 63u;
@@ -2263,32 +2261,62 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f,
     dv[u] = in ? ld_stream(&f.depth[i]) : 0.f;
     cv[u] = in ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
   }
+  uint64_t z[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    uint32_t tt = t[u] < c.T ? t[u] : kEmpty;
+    z[u] = kEmpty64;
+    if (tt != kEmpty) {
+      const uint32_t key = depth_key(dv[u]);
+      if (opaque_visible(c.opq_z, c.T, 1u, tt, key, 1u)) z[u] = ((uint64_t)key << 32) | encode_color_lds(cv[u], s_thr);
+      else tt = kEmpty;
+    }
+    t[u] = tt;
+  }
+  // every chunk's pixel counts in one round trip
+  uint32_t n[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) n[u] = t[u] != kEmpty ? cnt[t[u]] : 0u;
+  // pixels of at most K fragments: a slot index per fragment from the count
+  uint32_t base[kU];
+  bool small[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    small[u] = t[u] != kEmpty && n[u] <= (uint32_t)K;
+    const Run r = same_target_run(small[u] ? t[u] : kEmpty);
+    base[u] = 0u;
+    if (small[u] && r.start) base[u] = atomicSub(&cnt[t[u]], r.end_lane - r.lane + 1u);
+    base[u] = __shfl(base[u], (int)r.start_lane) - (r.lane - r.start_lane);  // this lane's index + 1
+  }
+  // pixels of more than K: their fragments' lanes, per 64-fragment chunk,
+  // for the classification (which skips the rest); pairs and chains
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t m = __ballot(t[u] != kEmpty && !small[u]);
+    if (lane == 0u && first + (uint64_t)u * 64u < f.count) big[first / 64u + u] = m;
+  }
   uint64_t a[kU], b[kU];
   bool hasb[kU], act[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    uint32_t tt = t[u] < c.T ? t[u] : kEmpty;
-    uint64_t z = kEmpty64;
-    if (tt != kEmpty) {
-      const uint32_t key = depth_key(dv[u]);
-      if (opaque_visible(c.opq_z, c.T, 1u, tt, key, 1u)) z = ((uint64_t)key << 32) | encode_color_lds(cv[u], s_thr);
-      else tt = kEmpty;
-    }
-    const Run r = same_target_run(tt);
-    const uint32_t zn_lo = __shfl_down((uint32_t)z, 1u);
-    const uint32_t zn_hi = __shfl_down((uint32_t)(z >> 32), 1u);
+    const uint32_t tb = small[u] ? kEmpty : t[u];
+    const Run r = same_target_run(tb);
+    const uint32_t zn_lo = __shfl_down((uint32_t)z[u], 1u);
+    const uint32_t zn_hi = __shfl_down((uint32_t)(z[u] >> 32), 1u);
     const uint64_t zn = ((uint64_t)zn_hi << 32) | zn_lo;
-    const bool lead = tt != kEmpty && ((r.lane - r.start_lane) & 1u) == 0u;
+    const bool lead = tb != kEmpty && ((r.lane - r.start_lane) & 1u) == 0u;
     hasb[u] = lead && r.lane < r.end_lane;
-    a[u] = z;
+    a[u] = z[u];
     b[u] = 0;
     if (hasb[u]) {
-      a[u] = z < zn ? z : zn;
-      b[u] = z < zn ? zn : z;
+      a[u] = z[u] < zn ? z[u] : zn;
+      b[u] = z[u] < zn ? zn : z[u];
     }
     act[u] = lead;
-    t[u] = tt;
   }
+#pragma unroll
+  for (int u = 0; u < kU; ++u)  // the stores of the kept fragments (index base - 1 < K)
+    if (small[u] && base[u] - 1u < (uint32_t)K) slots[(uint64_t)(base[u] - 1u) * c.T + t[u]] = z[u];
   // the pretests and the first slots of every chunk in one round trip
   constexpr int kProbe = K / 2 < 4 ? K / 2 : 4;
   int j[kU];
@@ -2359,43 +2387,47 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f,
 // evicted or rejected (the colours oitLoop64.frag.glsl:131-136 tail-blends);
 // one equal to it is "boundary": identical (depth, colour) keys are the same
 // bits in the A-buffer, and the resolve keeps the lowest ids of them in front.
-// Each wave takes kU chunks of 64 fragments: their targets and depths in one
-// round trip, their K-th slots in a second, and the colours of the
-// candidates (a depth at or past the K-th slot's: every fragment of a pixel
-// with fewer than K, where that slot is empty, is kept without one) in a
-// third, so only tail fragments and the K-th ones read a colour.
+// Only pixels of more than K fragments can have tails (the others keep every
+// fragment): the store left, per 64-fragment chunk, the mask of the lanes
+// whose fragment belongs to such a pixel, and a chunk with an empty mask is
+// skipped. Each wave takes kU chunks and loads, one round trip each, for
+// the masked lanes: target, depth and then the K-th slot; for those at or
+// past the K-th depth, the colour.
 template <int K, int kU>
 __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f, const uint64_t* __restrict__ slots,
-                                                            TailBuf tb) {
+                                                            const uint64_t* __restrict__ bigm, TailBuf tb) {
   __shared__ float s_thr[257];
   stage_thresholds(s_thr, c.tab);
-  const uint32_t lane = threadIdx.x & 63u;
+  const uint32_t lane = lane_id();
   const uint64_t first = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
-  uint32_t t[kU], key[kU];
+  bool big[kU];
+  uint64_t any = 0;
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint64_t m = first + (uint64_t)u * 64u < f.count ? big_mask_load(&bigm[first / 64u + u]) : 0ull;
+    big[u] = (m >> lane) & 1ull;
+    any |= m;
+  }
+  if (any == 0ull) return;  // wave-uniform: no fragment of this wave can be in the tail
+  uint32_t t[kU];
   float dv[kU];
-  bool valid[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     const uint64_t i = first + (uint64_t)u * 64u + lane;
-    valid[u] = i < f.count;
-    t[u] = valid[u] ? ld_stream(&f.target[i]) : kEmpty;
-    dv[u] = valid[u] ? ld_stream(&f.depth[i]) : 0.f;
-  }
-#pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    if (t[u] >= c.T) valid[u] = false;
-    key[u] = valid[u] ? depth_key(dv[u]) : 0u;
-    if (valid[u] && !opaque_visible(c.opq_z, c.T, 1u, t[u], key[u], 1u)) valid[u] = false;
+    t[u] = big[u] ? ld_stream(&f.target[i]) : 0u;
+    dv[u] = big[u] ? ld_stream(&f.depth[i]) : 0.f;
   }
   uint64_t sk[kU];
 #pragma unroll
-  for (int u = 0; u < kU; ++u) sk[u] = valid[u] ? slots[(uint64_t)(K - 1) * c.T + t[u]] : 0ull;
+  for (int u = 0; u < kU; ++u) sk[u] = big[u] ? slots[(uint64_t)(K - 1) * c.T + t[u]] : 0ull;
+  uint32_t key[kU];
   bool cand[kU];
   float4 cv[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
+    key[u] = big[u] ? depth_key(dv[u]) : 0u;
     // the key's high half is the depth: a nearer fragment stays in the slots
-    cand[u] = valid[u] && key[u] >= (uint32_t)(sk[u] >> 32);
+    cand[u] = big[u] && key[u] >= (uint32_t)(sk[u] >> 32);
     const uint64_t i = first + (uint64_t)u * 64u + lane;
     cv[u] = cand[u] ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
   }
@@ -2414,7 +2446,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
       }
     }
     const uint64_t i = first + (uint64_t)u * 64u + lane;
-    tail_record_own(tail, t[u], f.id_base + (uint32_t)i, flag, payload, tb);
+    tail_record_own(tail, big[u] ? t[u] : kEmpty, f.id_base + (uint32_t)i, flag, payload, tb);
   }
 }
 
@@ -2423,8 +2455,8 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
 // frame (the occupied ones are a prefix: slots stay sorted, and no key is
 // the empty key), so the next clear need not write the whole A-buffer.
 template <int K, bool kColorTail, bool kDebug>
-__global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, uint64_t* __restrict__ slots, TailBuf tb,
-                                                           ResolveOut o) {
+__global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, uint64_t* __restrict__ slots,
+                                                           uint32_t* __restrict__ cnt, TailBuf tb, ResolveOut o) {
   __shared__ float s_dec[256];
   __shared__ float s_alpha[256];
   stage_decode(s_dec, s_alpha, c.tab);
@@ -2435,12 +2467,19 @@ __global__ __launch_bounds__(kBlock) void k_loop64_resolve(Common c, uint64_t* _
   // where the lower half is full
   constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
   const uint32_t th = kColorTail ? tb.head[t] : 0u;  // tail list head, with the slots
+  const uint32_t n_frame = cnt[t];  // more than K: chained, sorted; else 0 and the slots unsorted
   uint64_t s[K];
 #pragma unroll
   for (int k = 0; k < kHalf; ++k) s[k] = slots[(uint64_t)k * c.T + t];
   const bool upper = (uint32_t)(s[kHalf - 1] >> 32) != kEmpty;
 #pragma unroll
   for (int k = kHalf; k < K; ++k) s[k] = upper ? slots[(uint64_t)k * c.T + t] : kEmpty64;
+  // a pixel of at most K fragments stored them unsorted (empty slots last,
+  // the empty key being the largest): sort; sorted slots stay as they are.
+  // The network's size is wave-uniform: half of K where no lane holds more.
+  if (__ballot(upper)) sort_u64<K>(s);
+  else sort_u64<kHalf>(s);
+  if (n_frame) cnt[t] = 0u;
   float4 col = make_float4(0.f, 0.f, 0.f, 0.f);
   uint32_t n = 0;
   bool alive = true;
@@ -2822,38 +2861,39 @@ struct SpinStore {
 
 template <int K>
 struct Loop64Store {
-  static hipError_t run(const Common& c, const FragView& f, uint64_t* slots, int chunks, hipStream_t s) {
-    if (chunks <= 1) {
-      hipLaunchKernelGGL((k_loop64_store<K>), dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, slots);
-    } else {
-      const int u = chunks >= 4 ? 4 : 2;
-      const uint64_t per_block = (uint64_t)kBlock * u;
-      const dim3 g((uint32_t)((f.count + per_block - 1) / per_block)), b(kBlock);
-      if (u == 4) hipLaunchKernelGGL((k_loop64_store_m<K, 4>), g, b, 0, s, c, f, slots);
-      else hipLaunchKernelGGL((k_loop64_store_m<K, 2>), g, b, 0, s, c, f, slots);
-    }
+  static hipError_t run(const Common& c, const FragView& f, uint64_t* slots, uint32_t* cnt, uint64_t* big,
+                        int chunks, hipStream_t s) {
+    if (f.count == 0) return hipSuccess;
+    const int u = chunks >= 4 ? 4 : (chunks >= 2 ? 2 : 1);
+    const uint64_t per_block = (uint64_t)kBlock * u;
+    const dim3 g((uint32_t)((f.count + per_block - 1) / per_block)), b(kBlock);
+    if (u == 4) hipLaunchKernelGGL((k_loop64_store_m<K, 4>), g, b, 0, s, c, f, slots, cnt, big);
+    else if (u == 2) hipLaunchKernelGGL((k_loop64_store_m<K, 2>), g, b, 0, s, c, f, slots, cnt, big);
+    else hipLaunchKernelGGL((k_loop64_store_m<K, 1>), g, b, 0, s, c, f, slots, cnt, big);
     return hipGetLastError();
   }
 };
 
 template <int K>
 struct Loop64Classify {
-  static hipError_t run(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb, hipStream_t s) {
+  static hipError_t run(const Common& c, const FragView& f, const uint64_t* slots, const uint64_t* big,
+                        const TailBuf& tb, hipStream_t s) {
     if (f.count == 0) return hipSuccess;
     constexpr int kU = 4;  // 64-fragment chunks per wave
     const uint32_t blocks = (uint32_t)((f.count + kBlock * kU - 1) / (kBlock * kU));
-    hipLaunchKernelGGL((k_loop64_classify<K, kU>), dim3(blocks), dim3(kBlock), 0, s, c, f, slots, tb);
+    hipLaunchKernelGGL((k_loop64_classify<K, kU>), dim3(blocks), dim3(kBlock), 0, s, c, f, slots, big, tb);
     return hipGetLastError();
   }
 };
 
 template <int K>
 struct Loop64Resolve {
-  static hipError_t run(const Common& c, uint64_t* slots, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
+  static hipError_t run(const Common& c, uint64_t* slots, uint32_t* cnt, const TailBuf& tb, const ResolveOut& o,
+                        hipStream_t s) {
     const bool dbg = o.fk_key || o.fk_val || o.counts;
     const bool ctail = c.tail && tb.head != nullptr;
     const dim3 g(blocks_for(c.T)), b(kBlock);
-#define OIT_L64(CT, DBG) hipLaunchKernelGGL((k_loop64_resolve<K, CT, DBG>), g, b, 0, s, c, slots, tb, o)
+#define OIT_L64(CT, DBG) hipLaunchKernelGGL((k_loop64_resolve<K, CT, DBG>), g, b, 0, s, c, slots, cnt, tb, o)
     if (ctail) { if (dbg) OIT_L64(true, true); else OIT_L64(true, false); }
     else { if (dbg) OIT_L64(false, true); else OIT_L64(false, false); }
 #undef OIT_L64
@@ -3004,20 +3044,28 @@ hipError_t launch_kbuf_resolve(const Common& c, const KBuf& kb, const TailBuf& t
   return dispatch_k<KBufResolve>(c.K, c, kb, tb, o, s);
 }
 
-hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, int chunks, hipStream_t s) {
+hipError_t launch_loop64_count(const Common& c, const FragView& f, uint32_t* cnt, hipStream_t s) {
   if (f.count == 0) return hipSuccess;
-  return dispatch_k<Loop64Store>(c.K, c, f, slots, chunks, s);
+  constexpr int kU = 4;
+  const uint32_t blocks = (uint32_t)((f.count + kBlock * kU - 1) / (kBlock * kU));
+  hipLaunchKernelGGL((k_loop64_count<kU>), dim3(blocks), dim3(kBlock), 0, s, c, f, cnt);
+  return hipGetLastError();
 }
 
-hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb,
-                                  hipStream_t s) {
-  if (f.count == 0) return hipSuccess;
-  return dispatch_k<Loop64Classify>(c.K, c, f, slots, tb, s);
+hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, uint32_t* cnt, uint64_t* big,
+                               int chunks, hipStream_t s) {
+  return dispatch_k<Loop64Store>(c.K, c, f, slots, cnt, big, chunks, s);
 }
 
-hipError_t launch_loop64_resolve(const Common& c, uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
-                                 hipStream_t s) {
-  return dispatch_k<Loop64Resolve>(c.K, c, slots, tb, o, s);
+hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const uint64_t* big,
+                                  const TailBuf& tb, hipStream_t s) {
+  if (f.count == 0) return hipSuccess;
+  return dispatch_k<Loop64Classify>(c.K, c, f, slots, big, tb, s);
+}
+
+hipError_t launch_loop64_resolve(const Common& c, uint64_t* slots, uint32_t* cnt, const TailBuf& tb,
+                                 const ResolveOut& o, hipStream_t s) {
+  return dispatch_k<Loop64Resolve>(c.K, c, slots, cnt, tb, o, s);
 }
 
 hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& wb, float view_near, float view_span,

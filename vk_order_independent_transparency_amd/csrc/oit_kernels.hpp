@@ -171,12 +171,18 @@ hipError_t launch_kbuf_resolve(const Common& c, const KBuf& kb, const TailBuf& t
 
 // chunks: 64-fragment chunks per wave (1: one fragment per lane; 2 or 4:
 // every chunk's loads and chain steps in flight together)
-hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, int chunks, hipStream_t s);
-hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const TailBuf& tb,
-                                  hipStream_t s);
+// Per-target counts of a segment's fragments (at append).
+hipError_t launch_loop64_count(const Common& c, const FragView& f, uint32_t* cnt, hipStream_t s);
+// The stores of a segment, once every segment of the frame is counted;
+// `big` gets one mask per 64-fragment chunk of the segment (lanes whose
+// pixel holds more than K fragments).
+hipError_t launch_loop64_store(const Common& c, const FragView& f, uint64_t* slots, uint32_t* cnt, uint64_t* big,
+                               int chunks, hipStream_t s);
+hipError_t launch_loop64_classify(const Common& c, const FragView& f, const uint64_t* slots, const uint64_t* big,
+                                  const TailBuf& tb, hipStream_t s);
 // Also sets the slots it reads back to empty.
-hipError_t launch_loop64_resolve(const Common& c, uint64_t* slots, const TailBuf& tb, const ResolveOut& o,
-                                 hipStream_t s);
+hipError_t launch_loop64_resolve(const Common& c, uint64_t* slots, uint32_t* cnt, const TailBuf& tb,
+                                 const ResolveOut& o, hipStream_t s);
 
 hipError_t launch_wboit_store(const Common& c, const FragView& f, const WBBuf& wb, float view_near, float view_span,
                               hipStream_t s);

@@ -70,7 +70,7 @@ uint64_t request_bytes(const oit_request& r, uint64_t F) {
     case OIT_LINKEDLIST: return 4 * T + 12 * (F + 1) + (tail ? 4 * T : 0);
     case OIT_INTERLOCK: return 4 * T + 28 * (F + 1);
     case OIT_LOOP: return 12 * K * T + tails;
-    case OIT_LOOP64: return 8 * K * T + tails;
+    case OIT_LOOP64: return 8 * K * T + 4 * T + 8 * ((F + 63) / 64) + tails;  // slots, counts, chunk masks
     case OIT_SIMPLE: return 4 * T + 12 * K * T + tails;
     case OIT_SPINLOCK: return 8 * T + 12 * K * T + tails;
     default: return 20 * T;
@@ -348,6 +348,11 @@ int launch(oit_server* s, Slot& sl) {
   if ((rc = oit_clear(pc.ctx, sl.stream))) return rc;
   if ((rc = oit_append(pc.ctx, &f, sl.stream))) return rc;
   if ((rc = oit_resolve(pc.ctx, out, sl.stream))) return rc;
+  if (oit_query_sizes(pc.ctx, &z) == OIT_OK) {  // the resolve may size per-frame buffers (Loop64's chunk masks)
+    std::lock_guard<std::mutex> g(s->gmu);
+    pc.resident = z.resident_bytes;
+    publish_held(s);
+  }
   return st(hipEventRecord(sl.done, sl.stream));
 }
 
