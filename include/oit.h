@@ -180,7 +180,8 @@ int oit_resolve_debug(oit_ctx* ctx, float* out_rgba, uint32_t* frontk_key,
  * (oit_get_stats) instead of hanging. `index` is checked against the
  * structure's allocated words (OIT_ERR_INVALID_ARG beyond it). */
 enum {
-  OIT_DEBUG_LL_HEAD = 0,    /* Linked List / interlock head per target */
+  OIT_DEBUG_LL_HEAD = 0,    /* Linked List / interlock head per target (with samples > 1 the
+                               heads are pixel-major: pixel * samples + sample) */
   OIT_DEBUG_LL_NEXT = 1,    /* Linked List / interlock link word per node */
   OIT_DEBUG_SPIN_LOCK = 2   /* Spinlock lock word per target (count << 1 | held) */
 };

@@ -214,6 +214,59 @@ def test_weighted_vs_fp64(oracle):
     assert err32.max() <= 1e-5
 
 
+@pytest.mark.parametrize("samples", [2, 4, 8])
+def test_weighted_sample_shading_vs_fp64(oracle, samples):
+    """Sample-shaded WBOIT keeps its accumulators pixel-major; the output is
+    sample-major like every other algorithm's. Two frames in a row, so the
+    second one runs on accumulators the first composite set back to zero."""
+    from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, Renderer
+
+    scenes = [deep_scene(seed=16 + samples + 10 * i, width=26, height=18, spheres=n, samples=samples)
+              for i, n in enumerate((90, 40))]
+    W, H = scenes[0][0].width, scenes[0][0].height
+    with Renderer(0) as r:
+        r.configure(FrameConfig(algorithm=WB, layers=1, tail_blend=False, width=W, height=H, samples=samples))
+        out = DeviceBuffer((W * H * samples, 4), np.float32)
+        for idx, (p, tg, d, c) in enumerate(scenes):
+            _say("WBOIT samples", samples, "frame", idx, len(tg))
+            bufs = [DeviceBuffer.from_numpy(np.ascontiguousarray(a)) for a in (tg, d, c)]
+            r.clear()
+            r.append(*bufs)
+            r.resolve(out)
+            o64 = oracle.run_fp64(_oracle_cfg(oracle, oracle.WEIGHTED, 1, 1, p), tg, d, c)
+            err = _rel_err(out.numpy(), o64["out"], 1e-3)
+            _say("WBOIT", samples, "frame", idx, "max rel err vs fp64", err.max())
+            assert err.max() <= 1e-5
+
+
+@pytest.mark.parametrize("samples", [2, 8])
+@pytest.mark.parametrize("alg,variant", [(LL, 0), (LL, 0x1), (LL, 0x2), (LL, 0x8), (5, 0)])
+def test_sample_shading_list_heads(oracle, alg, variant, samples):
+    """Sample-shaded Linked List and interlock keep their list heads pixel-major
+    (pixel * samples + sample) and composite in that order; the output stays
+    sample-major. Every append and resolve form, two frames on one context (the
+    second smaller), each bit-exact against the oracle."""
+    from vk_order_independent_transparency_amd import DeviceBuffer, FrameConfig, Renderer
+
+    scenes = [deep_scene(seed=40 + samples + i, width=22, height=14, spheres=n, samples=samples)
+              for i, n in enumerate((100, 45))]
+    cap = max(len(x[1]) for x in scenes) + 1
+    W, H = scenes[0][0].width, scenes[0][0].height
+    K = 4
+    with Renderer(0) as r:
+        r.configure(FrameConfig(algorithm=alg, layers=K, tail_blend=True, width=W, height=H, samples=samples,
+                                ll_pool_nodes=cap if alg == LL else 0, variant=variant))
+        out = DeviceBuffer((W * H * samples, 4), np.float32)
+        for idx, (p, tg, d, c) in enumerate(scenes):
+            _say("list heads", alg, variant, samples, "frame", idx, len(tg))
+            bufs = [DeviceBuffer.from_numpy(np.ascontiguousarray(a)) for a in (tg, d, c)]
+            r.clear()
+            r.append(*bufs)
+            r.resolve(out)
+            want = oracle.run(_oracle_cfg(oracle, alg, K, 1, p, cap if alg == LL else 0), tg, d, c)["out"]
+            _assert_same(f"frame {idx}", out.numpy(), want)
+
+
 @pytest.mark.parametrize("alg", [LL, LOOP64, SPIN, SIMPLE])
 def test_sample_shading_4x(oracle, alg):
     p, tg, d, c = deep_scene(seed=15, width=24, height=20, spheres=80, samples=4)

@@ -321,6 +321,7 @@ oit::Common common_of(const oit_ctx* c) {
   k.background = make_float4(c->cfg.background[0], c->cfg.background[1], c->cfg.background[2], c->cfg.background[3]);
   k.tab = c->d_tab;
   k.cov = c->cfg.coverage_samples > 1 ? c->cfg.coverage_samples : 0;
+  k.spp = c->cfg.samples > 1 ? c->cfg.samples : 1;
   if (c->opq_active) {
     k.opq_z = c->opq_z;
     k.opq_rgba = c->opq_rgba;

@@ -287,6 +287,28 @@ __device__ __forceinline__ float4 tail_fold(const TailBuf& tb, uint32_t t, float
   return tail_fold_from(tb, tb.head[t], bg, skip);
 }
 
+// Sample shading with spp samples per pixel lays targets out sample-major
+// (sample * pixels + pixel). pm_index(t) is target t's place in pixel-major
+// order (pixel * spp + sample) and pm_target(a) its inverse. The stream gives
+// a fragment's covered samples consecutive ids, so structures indexed
+// pixel-major (the WBOIT accumulators) take one wave's atomics on one run of
+// entries, and composites that walk targets in pixel-major order give a
+// pixel's samples neighbouring lanes, whose list nodes share cache lines.
+// spp is 1, 2, 4 or 8: shifts and compares, no divisions.
+__device__ __forceinline__ uint32_t pm_index(uint32_t t, uint32_t T, uint32_t spp) {
+  if (spp <= 1u) return t;
+  const uint32_t sh = (uint32_t)__builtin_ctz(spp);
+  const uint32_t px = T >> sh;
+  uint32_t smp = 0;
+  for (uint32_t k = 1; k < spp; ++k) smp += t >= k * px ? 1u : 0u;
+  return ((t - smp * px) << sh) | smp;
+}
+__device__ __forceinline__ uint32_t pm_target(uint32_t a, uint32_t T, uint32_t spp) {
+  if (spp <= 1u) return a;
+  const uint32_t sh = (uint32_t)__builtin_ctz(spp);
+  return (a & (spp - 1u)) * (T >> sh) + (a >> sh);
+}
+
 // Colour buffer of target t before the colour-pass tail: the colour of the
 // frame's opaque pass, else the clear colour.
 __device__ __forceinline__ float4 dest_of(const Common& c, uint32_t t) {
@@ -491,41 +513,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_link(Common c, FragView f, LLBuf 
   // chain inside the run; the run start lane's link is written by the insert
   if (valid && !r.start) st_relaxed(&ll.next[n], n - 1u);
   const bool ins = valid && r.end;
-  list_insert_desc(ins, ins ? &ll.head[t] : nullptr, ll.next, n - (r.lane - r.start_lane), n, (uint32_t)ll.cap,
-                   ll.err);
-}
-
-// Composite pass, oitLinkedList.frag.glsl:106-172: walk the list (newest
-// first), keep the K nearest in registers, tail-blend the ones pushed out
-// (:131-143), blend front to back (:162-166), over the tail (:170), then the
-// PREMULTIPLIED framebuffer blend onto the colour buffer (oit.cpp:723-726).
-template <int K, bool kTail, bool kColorTail, bool kDebug>
-__global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailBuf tb, ResolveOut o) {
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= c.T) return;
-  FrontK<K> a;
-  a.init();
-  float4 tailc = make_float4(0.f, 0.f, 0.f, 0.f);
-  uint32_t p = ll.head[t];
-  uint32_t cnt = 0;
-  while (p != 0u && p < ll.cap) {  // a link outside the pool ends the walk
-    const uint64_t cd = ll.cd[p];
-    const uint32_t nx = ll.next[p];
-    const uint64_t x = (cd & 0xFFFFFFFF00000000ull) | p;  // (depth key, id + 1)
-    uint64_t ev;
-    uint32_t evc;
-    a.insert(x, (uint32_t)cd, ev, evc);
-    if (kTail && cnt >= (uint32_t)K) tailc = over(tailc, decode_premult(evc, c.tab));
-    ++cnt;
-    p = descend(nx, p, ll.err);
-  }
-  const uint32_t n = cnt < (uint32_t)K ? cnt : (uint32_t)K;
-  float4 col = blend_front<K>(a, n, c.tab);
-  col = over(col, tailc);
-  float4 d = dest_of(c, t);
-  if (kColorTail) d = tail_fold(tb, t, d, 0u);
-  o.out[t] = over(col, d);
-  if (kDebug) write_debug<K>(o, t, a, n, cnt, true, false);
+  list_insert_desc(ins, ins ? &ll.head[pm_index(t, c.T, c.spp)] : nullptr, ll.next, n - (r.lane - r.start_lane), n,
+                   (uint32_t)ll.cap, ll.err);
 }
 
 // Fused colour pass of the Linked List: store and ordered link in one
@@ -537,8 +526,8 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve(Common c, LLBuf ll, TailB
 // frame's resetting resolve, or by the clear), so a walker never follows a
 // stale link; the run chain n -> n-1 and the inserts are relaxed atomics.
 // One wave's share of the fused append: fragments first .. first + 64 * kU - 1
-// of a source stream whose fragment i is node node0 + i and targets
-// tbase + target[i] of the pool's head array.
+// of a source stream whose fragment i is node node0 + i and targets head
+// tbase + pm_index(target[i]) of the pool's head array.
 struct AppendSrc {
   const uint32_t* target;
   const float* depth;
@@ -553,6 +542,7 @@ struct AppendSrc {
   uint32_t full = 0;               // the pixel's sample bits
   const uint64_t* opq = nullptr;   // the frame's opaque depth planes (early depth test), or null
   uint32_t cov = 0;                // coverage samples (opaque plane count)
+  uint32_t spp = 1;                // sample shading: samples per pixel (heads pixel-major)
 };
 
 template <bool kTail, int kU>
@@ -611,7 +601,7 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     old[u] = 0;
-    if (ins[u]) old[u] = max_relaxed(&ll.head[f.tbase + t[u]], hi[u]);
+    if (ins[u]) old[u] = max_relaxed(&ll.head[f.tbase + pm_index(t[u], f.T, f.spp)], hi[u]);
   }
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
@@ -657,6 +647,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_append(Common c, FragView f, LLBu
   a.full = c.cov > 1u ? (1u << c.cov) - 1u : 0xFFFFFFFFu;
   a.opq = c.opq_z;
   a.cov = c.cov;
+  a.spp = c.spp;
   ll_append_wave<kTail, kU>(a, s_thr, ll, tb);
 }
 
@@ -805,10 +796,11 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll,
   }
   __syncthreads();
   const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < c.T; i += stride) {
+    const uint32_t t = pm_target(i, c.T, c.spp);
     FrontK<K> a;
     uint32_t cnt;
-    const float4 col = ll_walk_reset<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
+    const float4 col = ll_walk_reset<K, kTail, kPair>(i, ll, s_dec, s_alpha, a, cnt);
     float4 d = dest_of(c, t);
     if (kColorTail) d = tail_fold(tb, t, d, 0u);
     o.out[t] = over(col, d);
@@ -827,10 +819,11 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_reset_keys(Common c, LLBu
   }
   __syncthreads();
   const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < c.T; i += stride) {
+    const uint32_t t = pm_target(i, c.T, c.spp);
     FrontKeys<K> a;
     uint32_t cnt;
-    const float4 col = ll_walk_reset_keys<K, kTail, kPair>(t, ll, s_dec, s_alpha, a, cnt);
+    const float4 col = ll_walk_reset_keys<K, kTail, kPair>(i, ll, s_dec, s_alpha, a, cnt);
     float4 d = dest_of(c, t);
     if (kColorTail) d = tail_fold(tb, t, d, 0u);
     o.out[t] = over(col, d);
@@ -865,6 +858,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_resolve_cov(Common c, LLBuf ll, T
   }
   __syncthreads();
   const uint32_t stride = gridDim.x * kBlock;
+  // coverage shading has one target per pixel (spp = 1): head index = target
   for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
     FrontK<K> a;
     uint32_t cnt;
@@ -2001,10 +1995,11 @@ __global__ __launch_bounds__(kBlock, (K == 16 && !kCov) ? 6 : 1) void k_interloc
   }
   __syncthreads();
   const uint32_t stride = gridDim.x * kBlock;
-  for (uint32_t t = blockIdx.x * kBlock + threadIdx.x; t < c.T; t += stride) {
+  for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < c.T; i += stride) {
+    const uint32_t t = pm_target(i, c.T, c.spp);  // heads are pixel-major
     const uint32_t tx = threadIdx.x;
-    uint32_t p = ll.head[t];
-    if (p != 0u) ll.head[t] = 0u;
+    uint32_t p = ll.head[i];
+    if (p != 0u) ll.head[i] = 0u;
     uint32_t prev = 0;
     uint32_t depth = 0;
     auto take = [&](uint32_t node, uint64_t cdv, uint32_t mv) {
@@ -2570,11 +2565,12 @@ __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WB
   }
   if (t != kEmpty && r.end) {  // channel planes: each atomic instruction of a wave stays in few cache lines
     float* acc = reinterpret_cast<float*>(wb.accum);
-    unsafeAtomicAdd(acc + t, wc.x);
-    unsafeAtomicAdd(acc + (uint64_t)c.T + t, wc.y);
-    unsafeAtomicAdd(acc + 2ull * c.T + t, wc.z);
-    unsafeAtomicAdd(acc + 3ull * c.T + t, wc.w);
-    unsafeAtomicAdd(&wb.reveal[t], lr);
+    const uint32_t a = pm_index(t, c.T, c.spp);
+    unsafeAtomicAdd(acc + a, wc.x);
+    unsafeAtomicAdd(acc + (uint64_t)c.T + a, wc.y);
+    unsafeAtomicAdd(acc + 2ull * c.T + a, wc.z);
+    unsafeAtomicAdd(acc + 3ull * c.T + a, wc.w);
+    unsafeAtomicAdd(&wb.reveal[a], lr);
   }
 }
 
@@ -2583,19 +2579,21 @@ __global__ __launch_bounds__(kBlock) void k_wboit_store(Common c, FragView f, WB
 // The accumulators it read are set back to the clear values (0, and 0 for
 // the log2 revealage) for the next frame, where they are not already, so
 // the next clear need not fill them.
+// Threads run over the accumulators in their (pixel-major) order.
 __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, ResolveOut o) {
-  const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
-  if (t >= c.T) return;
+  const uint32_t a = blockIdx.x * kBlock + threadIdx.x;
+  if (a >= c.T) return;
+  const uint32_t t = pm_target(a, c.T, c.spp);
   float* acc = reinterpret_cast<float*>(wb.accum);
-  const float4 A = make_float4(acc[t], acc[(uint64_t)c.T + t], acc[2ull * c.T + t], acc[3ull * c.T + t]);
-  const float lr = wb.reveal[t];
+  const float4 A = make_float4(acc[a], acc[(uint64_t)c.T + a], acc[2ull * c.T + a], acc[3ull * c.T + a]);
+  const float lr = wb.reveal[a];
   if ((__float_as_uint(A.x) | __float_as_uint(A.y) | __float_as_uint(A.z) | __float_as_uint(A.w) |
        __float_as_uint(lr)) != 0u) {
-    acc[t] = 0.f;
-    acc[(uint64_t)c.T + t] = 0.f;
-    acc[2ull * c.T + t] = 0.f;
-    acc[3ull * c.T + t] = 0.f;
-    wb.reveal[t] = 0.f;
+    acc[a] = 0.f;
+    acc[(uint64_t)c.T + a] = 0.f;
+    acc[2ull * c.T + a] = 0.f;
+    acc[3ull * c.T + a] = 0.f;
+    wb.reveal[a] = 0.f;
   }
   const float r = exp2f(lr);  // product of (1 - alpha), kept as a log2 sum
   const float den = fmaxf(A.w, 1e-5f);
@@ -2818,24 +2816,6 @@ hipError_t dispatch_k(uint32_t K, A&&... args) {
 }
 
 template <int K>
-struct LLResolve {
-  static hipError_t run(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
-    const bool dbg = o.fk_key || o.fk_val || o.counts;
-    const bool ctail = tb.head != nullptr;
-    const dim3 g(blocks_for(c.T)), b(kBlock);
-#define OIT_LL(TAIL, CT, DBG) hipLaunchKernelGGL((k_ll_resolve<K, TAIL, CT, DBG>), g, b, 0, s, c, ll, tb, o)
-    if (c.tail) {
-      if (ctail) { if (dbg) OIT_LL(true, true, true); else OIT_LL(true, true, false); }
-      else { if (dbg) OIT_LL(true, false, true); else OIT_LL(true, false, false); }
-    } else {
-      if (dbg) OIT_LL(false, false, true); else OIT_LL(false, false, false);
-    }
-#undef OIT_LL
-    return hipGetLastError();
-  }
-};
-
-template <int K>
 struct KBufResolve {
   static hipError_t run(const Common& c, const KBuf& kb, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
     const bool dbg = o.fk_key || o.fk_val || o.counts;
@@ -2999,10 +2979,6 @@ hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, h
   if (f.count == 0) return hipSuccess;
   hipLaunchKernelGGL(k_ll_link, dim3(blocks_for(f.count)), dim3(kBlock), 0, s, c, f, ll);
   return hipGetLastError();
-}
-
-hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s) {
-  return dispatch_k<LLResolve>(c.K, c, ll, tb, o, s);
 }
 
 hipError_t launch_ll_append(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, int chunks,

@@ -37,7 +37,7 @@ struct TailBuf {
 
 // Per-algorithm A-buffer views.
 struct LLBuf {
-  uint32_t* head;     // [T] 0 = empty (node 0 is the terminator)
+  uint32_t* head;     // [T] 0 = empty (node 0 is the terminator); pixel-major under sample shading
   uint64_t* cd;       // [cap] (depth key << 32) | packed colour
   uint32_t* next;     // [cap]
   uint64_t cap;       // nodes including the terminator
@@ -62,6 +62,7 @@ struct KBuf {          // Simple / Spinlock
 struct WBBuf {
   float4* accum;      // 4 T floats: planes of r, g, b, a (plane k at ((float*)accum) + k T)
   float* reveal;      // [T] log2 of the revealage (sum of log2(1 - alpha)); 0 = fully revealed
+  // accum and reveal are indexed pixel-major under sample shading (Common::spp)
 };
 
 struct ResolveOut {
@@ -78,6 +79,7 @@ struct Common {
   float4 background;
   const CodecTables* tab;
   uint32_t cov = 0;        // coverage shading: samples per pixel (0 = off)
+  uint32_t spp = 1;        // sample shading: samples per pixel, targets sample-major (1 = off)
   // the frame's opaque pass (null when it has none), per sample target:
   // (depth key << 32 | id) and colour; with coverage shading sample s of
   // pixel t is entry s * T + t
@@ -153,7 +155,6 @@ hipError_t launch_bin_resolve(const BatchArgs& a, const LLBuf& ll, uint32_t K, b
 
 hipError_t launch_ll_store(const Common& c, const FragView& f, const LLBuf& ll, const TailBuf& tb, hipStream_t s);
 hipError_t launch_ll_link(const Common& c, const FragView& f, const LLBuf& ll, hipStream_t s);
-hipError_t launch_ll_resolve(const Common& c, const LLBuf& ll, const TailBuf& tb, const ResolveOut& o, hipStream_t s);
 // Fused colour pass: store + ordered link in one kernel. Requires next[] of
 // the frame's nodes to hold kUnset on entry (left so by a resetting resolve
 // or by the clear).
