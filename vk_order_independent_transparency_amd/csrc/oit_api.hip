@@ -87,7 +87,7 @@ struct oit_ctx {
   oit_config cfg{};
   uint32_t T = 0;
   oit::CodecTables* d_tab = nullptr;
-  uint32_t* d_counters = nullptr;  // [0] tail records, [1] LL overflow; [16..31] last frame (LL)
+  uint32_t* d_counters = nullptr;  // [1] LL overflow; [16..31] last frame (LL)
   // Linked List
   uint32_t* ll_head = nullptr;
   uint64_t* ll_cd = nullptr;
@@ -122,7 +122,7 @@ struct oit_ctx {
   uint32_t* tl_next = nullptr;
   float4* tl_rgba = nullptr;
   uint64_t* tl_rec = nullptr;
-  uint32_t* tl_qn = nullptr;  // entries per tail-queue region [oit::tail_regions(tl_cap)]
+  uint32_t* tl_qn = nullptr;  // tail fragments (queue entries) per region [oit::tail_regions(tl_cap)]
   uint32_t tl_cap = 0;
   bool tl_head_valid = false;  // cleared for the current frame
   // frame state: oit_clear opens a frame; the first resolve links the tail
@@ -342,7 +342,6 @@ oit::TailBuf tail_of(const oit_ctx* c) {
   t.next = c->tl_next;
   t.rgba = c->tl_rgba;
   t.rec = c->tl_rec;
-  t.counter = c->d_counters;
   t.qn = c->tl_qn;
   t.cap = c->tl_cap;
   t.mask = c->cfg.coverage_samples > 1 ? c->tl_mask : nullptr;
@@ -1064,7 +1063,9 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
   // end (id >= cap - 1) can be tails
   const uint64_t first_tail = alg == OIT_LINKEDLIST ? (c->ll_cap ? c->ll_cap - 1 : 0) : 0;
   const bool no_tails = !tb.next || first_tail >= c->frame_frags;
-  if (link_tails && tb.head && !no_tails) {
+  // Loop64's classification, which runs after every store, links its tail
+  // fragments itself; the other algorithms queue them for this pass
+  if (link_tails && tb.head && !no_tails && alg != OIT_LOOP64) {
     e = staged(c, kStTailLink, s, [&] {
       return oit::launch_tail_link(tb, (uint32_t)first_tail, (uint32_t)c->frame_frags, s);
     });
@@ -1165,10 +1166,19 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
   if (hipMemcpyAsync(h, src, sizeof h, hipMemcpyDeviceToHost, s) != hipSuccess) return OIT_ERR_DEVICE;
   if (hipMemcpyAsync(&err, dev_err(c), sizeof err, hipMemcpyDeviceToHost, s) != hipSuccess) return OIT_ERR_DEVICE;
   if (hipStreamSynchronize(s) != hipSuccess) return OIT_ERR_DEVICE;
+  // tail fragments: counted per region of 1024 ids where they were found
+  uint64_t tails = 0;
+  if (c->tl_qn && c->tl_head_valid) {
+    std::vector<uint32_t> rg(oit::tail_regions(c->tl_cap));
+    if (hipMemcpyAsync(rg.data(), c->tl_qn, rg.size() * sizeof(uint32_t), hipMemcpyDeviceToHost, s) != hipSuccess ||
+        hipStreamSynchronize(s) != hipSuccess)
+      return OIT_ERR_DEVICE;
+    for (uint32_t v : rg) tails += v;
+  }
   std::memset(out, 0, sizeof *out);
   out->fragments = c->frame_frags;
   out->ll_overflow = h[1];
-  out->tail_fragments = h[0];
+  out->tail_fragments = tails;
   out->appends = c->appends;
   out->device_errors = err;
   if (err != 0u) {

@@ -44,30 +44,28 @@ __device__ __forceinline__ Run same_target_run(uint32_t t) {
 }
 
 // Colour-pass tail list value of frame-local fragment lid: ((lid + 1) << 1)
-// | boundary flag, so that 0 can terminate the (descending) lists; its link
-// word is next[lid].
+// | boundary flag, so that 0 can terminate the lists; its link word is
+// next[lid]. Values order as ids do.
 __device__ __forceinline__ uint32_t tail_value(uint32_t lid, uint32_t flag) { return ((lid + 1u) << 1) | flag; }
 __device__ __forceinline__ uint32_t tail_index(uint32_t v) { return (v >> 1) - 1u; }
 // values below this have link words in arrays of `cap` fragments
 __device__ __forceinline__ uint32_t tail_value_cap(uint32_t cap) { return (cap + 1u) << 1; }
 
 // Colour-pass tail lists, phase 1: a tail fragment `lid` stores its
-// premultiplied colour and a kUnset link word at index lid, and its record
-// (target << 32) | tail value goes to the frame's tail queue. The queue is
-// cut into regions of 1024 fragment ids, each with its own counter, and a
-// record goes to the region of its fragment's id: a fragment enters the
-// tail once, so region r's entries fit in rec[r * 1024 ..], and the link
-// pass, which takes the regions in order, sees the records roughly in id
-// order. The lanes of one region share one atomic. The
-// queue holds only tail fragments, so the link pass reads nothing of the
-// others. Called by every active lane of the wave that may hold a tail
-// fragment.
+// premultiplied colour at index lid, and its record (target << 32) | tail
+// value goes to the frame's tail queue. The queue is cut into regions of
+// 1024 fragment ids, each with its own counter, and a record goes to the
+// region of its fragment's id: a fragment enters the tail once, so region
+// r's entries fit in rec[r * 1024 ..]. The lanes of one region share one
+// returning atomic, so a wave waits for one round trip however many tail
+// fragments it finds. The queue holds only tail fragments, so the link pass
+// reads nothing of the others. Called by every active lane of the wave that
+// may hold a tail fragment.
 __device__ __forceinline__ void tail_queue_push(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
                                                 const TailBuf& tb, uint32_t mask) {
   tail = tail && lid < tb.cap && tb.qn != nullptr;
   if (tail) {
     tb.rgba[lid] = payload;
-    tb.next[lid] = kUnset;
     if (tb.mask) tb.mask[lid] = mask;
   }
   const uint32_t r = lid >> kTailRegionShift;
@@ -105,6 +103,34 @@ __device__ __forceinline__ void tail_record_own(bool tail, uint32_t t, uint32_t 
 __device__ __forceinline__ void tail_record_evict(bool tail, uint32_t t, uint32_t lid, uint32_t flag,
                                                   float4 payload, const TailBuf& tb, uint32_t mask = 0xFFFFFFFFu) {
   tail_queue_push(tail, t, lid, flag, payload, tb, mask);
+}
+
+// A converged wave's tail fragments straight onto their targets' lists,
+// for a pass that runs after every fragment of the frame is stored (no link
+// pass follows): neighbouring lanes of one target are chained in the wave
+// and published with one exchange on the list head (k_tail_link does the
+// same for queued records); the wave's count goes to its region's counter,
+// which the queue would otherwise have filled. Must be reached by all 64
+// lanes (lanes without a tail fragment pass tail = false).
+__device__ __forceinline__ void tail_list_push(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
+                                               const TailBuf& tb) {
+  tail = tail && lid < tb.cap;
+  if (tail) {
+    tb.rgba[lid] = payload;
+    if (tb.mask) tb.mask[lid] = 0xFFFFFFFFu;
+  }
+  const uint32_t v = tail_value(lid, flag);
+  const Run r = same_target_run(tail ? t : kEmpty);
+  const uint32_t v_prev = __shfl_up(v, 1u);
+  const uint32_t lo = __shfl(v, (int)r.start_lane);
+  if (tail && !r.start) st_relaxed(&tb.next[lid], v_prev);
+  if (tail && r.end) {
+    const uint32_t old = __hip_atomic_exchange(&tb.head[t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    st_relaxed(&tb.next[tail_index(lo)], old);
+  }
+  const uint64_t m = __ballot(tail);
+  if (m != 0ull && lane_id() == (uint32_t)__builtin_ctzll(m))
+    atomicAdd(&tb.qn[lid >> kTailRegionShift], (uint32_t)__builtin_popcountll(m));
 }
 
 // Sorted register array of K (sort key, colour) pairs, ascending; inserting x
@@ -252,33 +278,84 @@ __device__ __forceinline__ float4 blend_front(const FrontK<K>& a, uint32_t n, co
   return c;
 }
 
+// Tail values held in registers per walk of an unordered tail list.
+constexpr int kTailWindow = 16;
+
+// One walk of an unordered tail list: the kTailWindow largest values below
+// `bound`, in descending order (0 = none), by an insertion network in
+// registers; returns how many values of the list lie below `bound`, and in
+// `flagged` the number of boundary-flagged values of the whole list. A walk
+// longer than the arrays hold (a cycle) reports kErrWalk and sets `bad`.
+__device__ __forceinline__ uint32_t tail_window(const TailBuf& tb, uint32_t head, uint32_t vcap, uint32_t bound,
+                                                uint32_t (&w)[kTailWindow], uint32_t& flagged, bool& bad) {
+#pragma unroll
+  for (int k = 0; k < kTailWindow; ++k) w[k] = 0u;
+  uint32_t n = 0, fl = 0, hops = 0;
+  uint32_t v = head;
+  while (v != 0u && v < vcap) {  // a link outside the arrays ends the walk
+    if (++hops > tb.cap) {
+      report_error(tb.err, kErrWalk);
+      bad = true;
+      break;
+    }
+    const uint32_t nx = tb.next[tail_index(v)];
+    fl += v & 1u;
+    if (v < bound) {
+      ++n;
+      uint32_t x = v;
+#pragma unroll
+      for (int k = 0; k < kTailWindow; ++k) {
+        const uint32_t hi = w[k] > x ? w[k] : x;
+        x = w[k] > x ? x : w[k];
+        w[k] = hi;
+      }
+    }
+    v = nx;
+  }
+  flagged = fl;
+  return n;
+}
+
 // Colour buffer after the colour pass: the tail fragments of target t over
 // the background in id order (the ROP's primitive order, main.cpp:548-558),
-// folded newest first along the descending list: acc = t_n over ... over
-// t_1, then acc over the background. `skip` boundary-flagged entries are in
-// the A-buffer and not blended (Loop64 identical keys, see
-// k_loop64_classify): the ones with the lowest ids, i.e. the last flagged
-// entries of the walk.
+// folded newest first: acc = t_n over ... over t_1, then acc over the
+// background. The list is unordered, so each walk takes the next
+// kTailWindow values in descending order (one walk for most targets; a
+// longer tail takes one walk per window) and loads their colours four at a
+// time. `skip` boundary-flagged entries are in the A-buffer and not blended
+// (Loop64 identical keys, see k_loop64_classify): the flagged ones with the
+// lowest ids.
 // tail_fold with the list's head value already loaded (a composite issues
 // that load with its own first loads instead of after them).
 __device__ __forceinline__ float4 tail_fold_from(const TailBuf& tb, uint32_t head, float4 bg, uint32_t skip) {
   const uint32_t vcap = tail_value_cap(tb.cap);
-  uint32_t keep = 0xFFFFFFFFu;  // flagged entries to blend
-  if (skip > 0u) {
-    uint32_t flagged = 0;
-    for (uint32_t v = head; v != 0u && v < vcap; v = descend(tb.next[tail_index(v)], v, tb.err)) flagged += v & 1u;
-    keep = flagged > skip ? flagged - skip : 0u;
-  }
   float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
-  uint32_t v = head;
-  while (v != 0u && v < vcap) {  // a link outside the arrays ends the walk
-    const uint32_t id = tail_index(v);
-    const uint32_t nx = tb.next[id];
-    if (!(v & 1u) || keep > 0u) {
-      acc = over(acc, tb.rgba[id]);
-      if (v & 1u) --keep;
+  if (head == 0u || head >= vcap) return over(acc, bg);
+  uint32_t keep = 0xFFFFFFFFu;  // flagged entries to blend
+  uint32_t bound = 0xFFFFFFFFu;
+  bool bad = false;
+  for (bool first = true;; first = false) {
+    uint32_t w[kTailWindow];
+    uint32_t flagged;
+    const uint32_t n = tail_window(tb, head, vcap, bound, w, flagged, bad);
+    if (first && skip > 0u) keep = flagged > skip ? flagged - skip : 0u;
+#pragma unroll
+    for (int g = 0; g < kTailWindow; g += 4) {
+      if (w[g] == 0u) break;
+      float4 c[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) c[j] = w[g + j] ? tb.rgba[tail_index(w[g + j])] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const uint32_t v = w[g + j];
+        if (v != 0u && (!(v & 1u) || keep > 0u)) {
+          acc = over(acc, c[j]);
+          if (v & 1u) --keep;
+        }
+      }
     }
-    v = descend(nx, v, tb.err);
+    if (bad || n <= (uint32_t)kTailWindow) break;
+    bound = w[kTailWindow - 1];
   }
   return over(acc, bg);
 }
@@ -340,20 +417,37 @@ __device__ __forceinline__ uint32_t opaque_visible(const uint64_t* z, uint32_t T
 __device__ __forceinline__ float4 tail_fold_cov(const TailBuf& tb, uint32_t t, const Common& c) {
   const uint32_t S = c.cov;
   const uint32_t vcap = tail_value_cap(tb.cap);
-  uint32_t v = tb.head ? tb.head[t] : 0u;
-  if ((v == 0u || v >= vcap) && !c.opq_rgba) return c.background;
+  const uint32_t head = tb.head ? tb.head[t] : 0u;
+  const bool empty = head == 0u || head >= vcap;
+  if (empty && !c.opq_rgba) return c.background;
   float4 acc[8];
 #pragma unroll
   for (int s = 0; s < 8; ++s) acc[s] = make_float4(0.f, 0.f, 0.f, 0.f);
-  while (v != 0u && v < vcap) {  // a link outside the arrays ends the walk
-    const uint32_t id = tail_index(v);
-    const uint32_t nx = tb.next[id];
-    const uint32_t m = tb.mask[id];
-    const float4 c = tb.rgba[id];
+  uint32_t bound = 0xFFFFFFFFu;
+  bool bad = false;
+  while (!empty) {  // windows of the unordered list, newest first (tail_fold_from)
+    uint32_t w[kTailWindow];
+    uint32_t flagged;
+    const uint32_t n = tail_window(tb, head, vcap, bound, w, flagged, bad);
 #pragma unroll
-    for (int s = 0; s < 8; ++s)
-      if ((uint32_t)s < S && ((m >> s) & 1u)) acc[s] = over(acc[s], c);
-    v = descend(nx, v, tb.err);
+    for (int g = 0; g < kTailWindow; g += 4) {
+      if (w[g] == 0u) break;
+      float4 col[4];
+      uint32_t msk[4];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const bool h = w[g + j] != 0u;
+        col[j] = h ? tb.rgba[tail_index(w[g + j])] : make_float4(0.f, 0.f, 0.f, 0.f);
+        msk[j] = h ? tb.mask[tail_index(w[g + j])] : 0u;
+      }
+#pragma unroll
+      for (int j = 0; j < 4; ++j)
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+          if ((uint32_t)s < S && ((msk[j] >> s) & 1u)) acc[s] = over(acc[s], col[j]);
+    }
+    if (bad || n <= (uint32_t)kTailWindow) break;
+    bound = w[kTailWindow - 1];
   }
   float4 sum = make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
@@ -2441,7 +2535,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
       }
     }
     const uint64_t i = first + (uint64_t)u * 64u + lane;
-    tail_record_own(tail, big[u] ? t[u] : kEmpty, f.id_base + (uint32_t)i, flag, payload, tb);
+    tail_list_push(tail, t[u], f.id_base + (uint32_t)i, flag, payload, tb);
   }
 }
 
@@ -2605,28 +2699,22 @@ __global__ __launch_bounds__(kBlock) void k_wboit_resolve(Common c, WBBuf wb, Re
 }
 
 // ---------------------------------------------------------------------------
-// Tail lists, phase 2: link the frame's queued records into per-target
-// lists in descending id (newest first, the order the fold associates in).
+// Tail lists, phase 2: push the frame's queued records onto per-target
+// lists, unordered (the composite restores id order, tail_fold_from).
 // ---------------------------------------------------------------------------
 
 // Work item w of the grid is chunk (w mod 2) of 512 entries of queue
-// region r0 + w / 2, taken in that order, so the frame's records are linked
-// roughly in id order, as they were produced, and most inserts succeed at
-// the head. A wave loads its 8 x 64 records together and has their first
-// head attempts in flight together; only lanes that find a larger value at
-// the head walk. Neighbouring entries of one target with consecutive ids (a
-// sphere's front and back fragment) are chained here, newest to oldest, and
-// published with one insert: no other fragment's id lies between theirs.
+// region r0 + w / 2. Neighbouring entries of one target in a wave are
+// chained in the wave and published with one exchange on the list head,
+// whose old value becomes the chain's last link: no insert walks a list or
+// waits for another fragment's link.
 template <int kU>
 __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t r0, uint32_t r1) {
-  __shared__ uint32_t s_count[kBlock / 64];
   constexpr uint32_t kChunk = 64u * kU;
   constexpr uint32_t kPerRegion = (1u << kTailRegionShift) / kChunk;
-  const uint32_t vcap = tail_value_cap(tb.cap);
   const uint32_t lane = lane_id();
   const uint64_t items = (uint64_t)(r1 - r0 + 1u) * kPerRegion;
   const uint64_t waves = (uint64_t)gridDim.x * (kBlock / 64);
-  uint32_t mine = 0;
   for (uint64_t w = (uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6); w < items; w += waves) {
     const uint32_t reg = r0 + (uint32_t)(w / kPerRegion);
     const uint32_t base = (uint32_t)(w % kPerRegion) * kChunk;
@@ -2647,44 +2735,19 @@ __global__ __launch_bounds__(kBlock) void k_tail_link(TailBuf tb, uint32_t r0, u
       const bool tail = rec[u] != kEmpty64;
       t[u] = tail ? (uint32_t)(rec[u] >> 32) : kEmpty;
       v[u] = (uint32_t)rec[u];
-      mine += tail ? 1u : 0u;
-      const uint32_t id1 = v[u] >> 1;  // id + 1
-      const uint32_t tp = __shfl_up(t[u], 1u), tn = __shfl_down(t[u], 1u);
-      const uint32_t ip = __shfl_up(id1, 1u), in = __shfl_down(id1, 1u);
+      const Run r = same_target_run(t[u]);
       const uint32_t v_prev = __shfl_up(v[u], 1u);
-      const bool start = lane == 0u || tp != t[u] || ip + 1u != id1;
-      const bool end = lane == 63u || tn != t[u] || id1 + 1u != in;
-      const uint64_t starts = __ballot(start);
-      const uint32_t start_lane = 63u - (uint32_t)__builtin_clzll(starts & ((2ull << lane) - 1ull));
-      lo[u] = __shfl(v[u], (int)start_lane);
-      if (tail && !start) st_relaxed(&tb.next[tail_index(v[u])], v_prev);
-      ins[u] = tail && end;
+      lo[u] = __shfl(v[u], (int)r.start_lane);
+      if (tail && !r.start) st_relaxed(&tb.next[tail_index(v[u])], v_prev);
+      ins[u] = tail && r.end;
     }
     uint32_t old[kU];
 #pragma unroll
-    for (int u = 0; u < kU; ++u) old[u] = ins[u] ? max_relaxed(&tb.head[t[u]], v[u]) : 0u;
+    for (int u = 0; u < kU; ++u)
+      old[u] = ins[u] ? __hip_atomic_exchange(&tb.head[t[u]], v[u], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) : 0u;
 #pragma unroll
-    for (int u = 0; u < kU; ++u) {
-      if (ins[u] && old[u] < lo[u]) {
-        st_relaxed(&tb.next[tail_index(lo[u])], old[u]);
-        ins[u] = false;
-      }
-    }
-#pragma unroll
-    for (int u = 0; u < kU; ++u) {  // a larger value was already at the head: continue behind it
-      const bool walk = ins[u] && old[u] < vcap;
-      list_insert_desc_from<1, 1>(walk, walk ? &tb.next[tail_index(old[u])] : nullptr, tb.next, lo[u], v[u], vcap,
-                                  tb.err);
-    }
-  }
-  // tail fragments of the frame: one atomic per block
-  for (int o = 32; o > 0; o >>= 1) mine += __shfl_xor(mine, o);
-  if (lane == 0) s_count[threadIdx.x >> 6] = mine;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    uint32_t total = 0;
-    for (uint32_t w = 0; w < kBlock / 64; ++w) total += s_count[w];
-    if (total) atomicAdd(tb.counter, total);
+    for (int u = 0; u < kU; ++u)
+      if (ins[u]) st_relaxed(&tb.next[tail_index(lo[u])], old[u]);
   }
 }
 

@@ -18,18 +18,17 @@ struct FragView {
   const uint32_t* mask = nullptr;  // coverage shading: sample mask per fragment (gl_SampleMaskIn)
 };
 
-// Colour-pass tail lists (per target, descending fragment id: newest first).
-// Tail queue regions: 1024 frame-local fragment ids each.
+// Colour-pass tail lists: per target, unordered (the composite restores id
+// order). Tail queue regions: 1024 frame-local fragment ids each.
 constexpr uint32_t kTailRegionShift = 10;
 inline uint64_t tail_regions(uint64_t cap) { return (cap + (1u << kTailRegionShift) - 1) >> kTailRegionShift; }
 
 struct TailBuf {
-  uint32_t* head;     // [T] newest tail value, 0 = empty
+  uint32_t* head;     // [T] last linked tail value, 0 = empty
   uint32_t* next;     // [Fcap] link words, indexed by frame-local id
   float4* rgba;       // [Fcap] premultiplied colour the tail blend uses
   uint64_t* rec;      // [Fcap] tail queue: (target << 32) | ((id + 1) << 1 | boundary), by region of 1024 ids
-  uint32_t* counter;  // tail fragments of the frame (counted by the link kernel)
-  uint32_t* qn;       // [tail_regions(Fcap)] entries of each region, zero at the frame's start
+  uint32_t* qn;       // [tail_regions(Fcap)] tail fragments of each region (its queue entries), zero at the frame's start
   uint32_t cap;       // Fcap
   uint32_t* mask = nullptr;  // [Fcap] coverage shading: samples the ROP blend writes
   uint32_t* err = nullptr;   // device error word (kErr* bits), null: none
