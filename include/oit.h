@@ -123,7 +123,8 @@ typedef struct oit_stats {
   uint64_t tail_fragments;    /* fragments recorded in the colour-pass tail lists */
   uint64_t appends;           /* oit_append calls this frame */
   uint64_t device_errors;     /* bits of device-side waits that gave up since the last oit_get_stats: 1 a list
-                                 insert, 2 a Spinlock lock, 4 a list walk that met a non-descending link; the
+                                 insert, 2 a Spinlock lock, 4 a list walk that met a non-descending link (or a
+                                 colour-pass tail walk longer than the frame's fragments); the
                                  call then returns OIT_ERR_DEVICE and the affected frames' images are undefined */
 } oit_stats;
 
