@@ -108,7 +108,11 @@ def test_spinlock_gives_up_on_a_lock_never_released(oracle):
     bufs = _bufs(tg, d, c)
     with Renderer(0) as r:
         r.configure(FrameConfig(algorithm=OIT_SPINLOCK, layers=K, tail_blend=True, width=p.width, height=p.height))
-        t0 = int(tg[0])
+        # fragments below the K-th of a pixel are stored without the lock: the
+        # planted lock is on the deepest pixel, whose later fragments wait for it
+        per = np.bincount(tg.astype(np.int64), minlength=p.targets)
+        t0 = int(per.argmax())
+        assert per[t0] > K
         out = DeviceBuffer((p.targets, 4), np.float32)
         r.clear()
         _poke(r, _abi.DEBUG_SPIN_LOCK, t0, 1)  # held, count 0

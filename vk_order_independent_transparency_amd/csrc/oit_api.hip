@@ -104,6 +104,7 @@ struct oit_ctx {
   uint32_t* kb_count = nullptr;
   uint32_t* kb_lock = nullptr;
   uint32_t* kb_depth = nullptr;
+  uint32_t* kb_filled = nullptr;  // Spinlock: completed stores below K per target
   uint32_t* kb_col = nullptr;
   uint32_t* kb_key = nullptr;
   uint32_t* kb_id = nullptr;
@@ -138,7 +139,7 @@ struct oit_ctx {
   // buffer while the new configuration fits, so a context that serves many
   // frame sizes stops allocating once it has seen the largest
   uint64_t cap_ll_head = 0, ll_alloc = 0;
-  uint64_t cap_kb_count = 0, cap_kb_lock = 0, cap_kb_depth = 0, cap_kb_elems = 0;
+  uint64_t cap_kb_count = 0, cap_kb_lock = 0, cap_kb_depth = 0, cap_kb_filled = 0, cap_kb_elems = 0;
   uint64_t cap_l64 = 0, cap_l64_cnt = 0, cap_l64_big = 0, cap_wb = 0, cap_tl_head = 0;
   // coverage shading: sample masks of nodes, k-buffer slots and tail entries
   uint32_t* ll_mask = nullptr;
@@ -239,7 +240,7 @@ void release(oit_ctx* c, T** p, uint64_t* cap) {
 void free_frame_buffers(oit_ctx* c) {
   wait_idle(c);
   c->cap_ll_head = c->ll_alloc = 0;
-  c->cap_kb_count = c->cap_kb_lock = c->cap_kb_depth = c->cap_kb_elems = 0;
+  c->cap_kb_count = c->cap_kb_lock = c->cap_kb_depth = c->cap_kb_filled = c->cap_kb_elems = 0;
   c->cap_l64 = c->cap_l64_cnt = c->cap_l64_big = c->cap_wb = c->cap_tl_head = 0;
   c->cap_ll_mask = c->cap_kb_mask = c->cap_tl_mask = 0;
   dev_free(&c->ll_mask);
@@ -260,6 +261,7 @@ void free_frame_buffers(oit_ctx* c) {
   dev_free(&c->kb_count);
   dev_free(&c->kb_lock);
   dev_free(&c->kb_depth);
+  dev_free(&c->kb_filled);
   dev_free(&c->kb_col);
   dev_free(&c->kb_key);
   dev_free(&c->kb_id);
@@ -600,6 +602,7 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   if (alg != OIT_SPINLOCK) {
     release(c, &c->kb_lock, &c->cap_kb_lock);
     release(c, &c->kb_depth, &c->cap_kb_depth);
+    release(c, &c->kb_filled, &c->cap_kb_filled);
   }
   if (!pooled(alg)) release(c, &c->ll_mask, &c->cap_ll_mask);
   if (alg != OIT_INTERLOCK) release(c, &c->il_full, &c->cap_il_full);
@@ -660,10 +663,11 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
       }
       if (!rc && alg == OIT_SPINLOCK) rc = reserve(c, &c->kb_lock, &c->cap_kb_lock, T);
       if (!rc && alg == OIT_SPINLOCK) rc = reserve(c, &c->kb_depth, &c->cap_kb_depth, T);
+      if (!rc && alg == OIT_SPINLOCK) rc = reserve(c, &c->kb_filled, &c->cap_kb_filled, T);
       if (!rc && cov > 1) rc = reserve(c, &c->kb_mask, &c->cap_kb_mask, T * K);
       if (!rc && cov <= 1) release(c, &c->kb_mask, &c->cap_kb_mask);
       c->bytes_abuf = T * K * (cov > 1 ? 16u : 12u);
-      c->bytes_aux = T * (alg == OIT_SPINLOCK ? 8u : 4u);
+      c->bytes_aux = T * (alg == OIT_SPINLOCK ? 12u : 4u);
       break;
     }
     case OIT_LOOP64: {  // oit.cpp:134-140
@@ -767,7 +771,7 @@ int oit_query_sizes(const oit_ctx* c, oit_sizes* out) {
   }
   out->targets = c->T;
   out->resident_bytes = c->cap_ll_head * 4u + c->ll_alloc * 12u + c->cap_kb_count * 4u + c->cap_kb_lock * 4u +
-                        c->cap_kb_depth * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_l64_cnt * 4u + c->cap_l64_big * 8u + c->cap_wb * 20u +
+                        c->cap_kb_depth * 4u + c->cap_kb_filled * 4u + c->cap_kb_elems * 12u + c->cap_l64 * 8u + c->cap_l64_cnt * 4u + c->cap_l64_big * 8u + c->cap_wb * 20u +
                         c->cap_tl_head * 4u + (uint64_t)c->tl_cap * (4u + 16u + 8u) + oit::tail_regions(c->tl_cap) * 4u +
                         (c->cap_ll_mask + c->cap_kb_mask + c->cap_tl_mask) * 4u + c->cap_il_full * 16u +
                         c->cap_l32 * 12u + c->cap_opq * 8u + c->cap_opq_rgba * 16u;
@@ -828,6 +832,7 @@ int oit_clear(oit_ctx* c, void* stream) {
     case OIT_SPINLOCK:  // count, spin to 0, depth to ~0, oitRender.cpp:343-354 (the count is in the lock word)
       fill(c->kb_lock, T, 0u);
       fill(c->kb_depth, T, kEmpty);
+      fill(c->kb_filled, T, 0u);
       break;
     case OIT_LOOP64:  // every byte to 0xFF, oitRender.cpp:307; the composite leaves its slots empty
       l64_fill = !c->l64_clean;
@@ -986,6 +991,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
     case OIT_SPINLOCK: {
       oit::KBuf kb{nullptr, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
       kb.err = dev_err(c);
+      kb.filled = c->kb_filled;
       kb.mask = coverage(c) ? c->kb_mask : nullptr;
       e = staged(c, kStStore, s, [&] { return oit::launch_spin_store(k, v, kb, tb, s); });
       break;

@@ -268,34 +268,30 @@ __device__ __forceinline__ float4 ld_stream(const float4* p) {
                      __builtin_nontemporal_load(q + 3));
 }
 
+// Waits until every memory operation of the lane has completed, so the
+// stores before it are visible (at the memory side, for agent-scope relaxed
+// accesses) before any memory operation after it is issued.
+__device__ __forceinline__ void store_fence() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unconstrained
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Per-target lock for critical sections whose shared data is read and
 // written only with agent-scope relaxed atomics (coherent across the XCDs'
 // L2 caches, so the section needs ordering but no L2 writeback or
 // invalidate, which an agent-scope acquire/release would add to every lock
-// operation). Acquire: a relaxed exchange; the section's accesses are issued
+// operation). Bit 0 of the word is the lock; the bits above hold a count
+// that lanes add to whether the lock is held or not. Acquire: a relaxed OR
+// of bit 0, taken when the bit was clear; the section's accesses are issued
 // only after its value has returned. Release: wait until every memory
-// operation of the lane has completed, then a relaxed store of 0.
+// operation of the lane has completed, then clear bit 0 with a relaxed AND.
 __device__ __forceinline__ bool lock_try(uint32_t* l) {
-  return __hip_atomic_exchange(l, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0u;
+  return (__hip_atomic_fetch_or(l, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & 1u) == 0u;
 }
 __device__ __forceinline__ void lock_release(uint32_t* l) {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unconstrained
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __hip_atomic_store(l, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// The same lock with a counter in the lock word (count << 1 | held): the
-// acquiring atomicOr returns the count, the release stores the new one.
-__device__ __forceinline__ bool lock_try_count(uint32_t* l, uint32_t& count) {
-  const uint32_t old = __hip_atomic_fetch_or(l, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  count = old >> 1;
-  return (old & 1u) == 0u;
-}
-__device__ __forceinline__ void lock_release_count(uint32_t* l, uint32_t count) {
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0), expcnt and lgkmcnt unconstrained
-  __atomic_signal_fence(__ATOMIC_SEQ_CST);
-  __hip_atomic_store(l, count << 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  store_fence();
+  __hip_atomic_fetch_and(l, ~1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // list_insert_desc for a lane that already tried the head and learned that a

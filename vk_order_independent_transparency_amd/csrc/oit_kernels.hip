@@ -1522,29 +1522,20 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
 // Spinlock: oitSpinlock.frag.glsl
 // ---------------------------------------------------------------------------
 
-// One critical section of oitSpinlock.frag.glsl:86-123 for the fragment
-// (key, col, id, mask) of target t, run by the lane that holds t's lock:
-// fill while fewer than K are stored (:89-93), else replace the furthest
-// (first index wins, :96-107) when the newcomer is nearer (:109-117). Sets
-// the tail output: the evicted element, or the fragment itself.
+// The critical section of oitSpinlock.frag.glsl:95-123 for the fragment
+// (key, col, id, mask) of target t, run by the lane that holds t's lock on a
+// full buffer: replace the furthest element (first index wins, :96-107) when
+// the newcomer is nearer (:109-117). Sets the tail output: the evicted
+// element, or the fragment itself.
 template <int K>
-__device__ __forceinline__ void spin_section(const Common& c, const KBuf& kb, uint32_t t, uint32_t key, uint32_t col,
-                                             uint32_t id, uint32_t mask, uint32_t& count, bool& tail,
-                                             uint32_t& tail_id, uint32_t& tail_col, uint32_t& tail_mask) {
-  const uint32_t cnt = count++;  // :86-87; the count lives in the lock word (count << 1 | held)
-  if (cnt < (uint32_t)K) {
-    const uint64_t idx = (uint64_t)cnt * c.T + t;
-    st_relaxed(&kb.col[idx], col);
-    st_relaxed(&kb.key[idx], key);
-    st_relaxed(&kb.id[idx], id);
-    if (kb.mask) st_relaxed(&kb.mask[idx], mask);
-    return;
-  }
+__device__ __forceinline__ void spin_replace(const Common& c, const KBuf& kb, uint32_t t, uint32_t key, uint32_t col,
+                                             uint32_t id, uint32_t mask, bool& tail, uint32_t& tail_id,
+                                             uint32_t& tail_col, uint32_t& tail_mask) {
   uint32_t fur = 0;
   uint64_t mx = 0;
   // the scan as a loop of eight loads per trip (K/2 below 16): unrolled in
   // full it held 100-256 VGPRs (three or four waves per SIMD at K = 8, 16),
-  // as a loop 49-63 (eight waves); only sections of a full buffer scan
+  // as a loop 49-63 (eight waves)
   constexpr int kScanUnroll = K >= 16 ? 8 : (K >= 2 ? K / 2 : 1);
 #pragma unroll kScanUnroll
   for (int j = 0; j < K; ++j) {
@@ -1572,21 +1563,27 @@ __device__ __forceinline__ void spin_section(const Common& c, const KBuf& kb, ui
   }
 }
 
-// Colour pass (oitSpinlock.frag.glsl:49-129). The critical section runs under
-// a per-target lock taken with an atomic exchange; one lane of a wave holds
-// the lock while its neighbours retry on the next trip. Elements are ordered
+// Colour pass (oitSpinlock.frag.glsl:49-129). The count of the reference
+// (:86-87) lives in the lock word (count << 1 | held) and is taken first:
+// one returning add per run of one target in the wave gives each fragment
+// its place. A fragment whose place is below K is stored there without the
+// lock (:89-93 take no decision that another fragment could change: the
+// buffer is not full), and the wave then adds its stores, once they have
+// completed, to the target's `filled` counter. A fragment whose place is K
+// or more meets a full buffer: the early depth test (:66-69, read with the
+// count) rejects it against the bound, else it takes the lock with an
+// atomic OR, and runs the replacement section once `filled` shows all K
+// stores, so a section never reads a slot that is still being written.
+// The lock loop is driven by a wave-wide ballot: a lane that takes the lock
+// runs its section (two: see the pairs below) and releases it in the same
+// trip. Elements are ordered
 // by (depth key, id); evicted and rejected fragments are recorded for the
-// tail with their packed sRGB8 colour. Two consecutive lanes of one target (a
-// sphere's front and back) run their critical sections back to back under
-// one acquisition by the even lane (the two sections in id order, one of the
-// orders the lock allows), so they do not contend for the lock; the odd lane
-// gets its tail output back by shuffle. The early depth test (:66-69) reads
-// the bound in the same round trip as the first lock attempt; a fragment it
-// rejects skips its section (a lock taken for nothing is released with the
-// count unchanged). A fragment the test rejects is one its section would
-// reject too (the bound is the key of an evicted element, and the stored
-// elements only get nearer), so the test changes the count beyond K, as in
-// the reference, and never the stored set or the tail.
+// tail with their packed sRGB8 colour. A fragment the early test rejects is
+// one its section would reject too (the bound is the key of an evicted
+// element, and the stored elements only get nearer), so the test never
+// changes the stored set or the tail; the count counts every fragment, so
+// beyond K it differs from the reference's, which skips rejected ones
+// (min(count, K), the composite's n, does not).
 template <int K, bool kTail>
 __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBuf kb, TailBuf tb) {
   __shared__ float s_thr[257];
@@ -1618,48 +1615,63 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
     my_col = encode_color_lds(cv, s_thr);
     tail_col = my_col;
   }
-  // pairs: the even lane of a same-target run takes its odd neighbour along
+  // places: one add per run of one target, the early-test bound in the same round trip
   const Run r = same_target_run(valid ? t : kEmpty);
-  const bool lead = valid && ((r.lane - r.start_lane) & 1u) == 0u && r.lane < r.end_lane;
-  const bool follower = valid && ((r.lane - r.start_lane) & 1u) == 1u;
+  uint32_t old = 0u, bound = 0u;
+  if (valid && r.end)
+    old = __hip_atomic_fetch_add(&kb.lock[t], 2u * (r.lane - r.start_lane + 1u), __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+  if (valid) bound = ld_relaxed(&kb.depth[t]);
+  old = __shfl(old, (int)r.end_lane);
+  const uint32_t place = (old >> 1) + (r.lane - r.start_lane);
+  const bool fill = valid && place < (uint32_t)K;
+  if (fill) {
+    const uint64_t idx = (uint64_t)place * c.T + t;
+    st_relaxed(&kb.col[idx], my_col);
+    st_relaxed(&kb.key[idx], my_key);
+    st_relaxed(&kb.id[idx], lid);
+    if (kb.mask) st_relaxed(&kb.mask[idx], my_mask);
+  }
+  if (__ballot(fill)) {  // wave-uniform: publish the stores, one add per run's last stored place
+    store_fence();
+    if (fill && (r.lane == r.end_lane || place + 1u == (uint32_t)K))
+      __hip_atomic_fetch_add(&kb.filled[t], r.lane - r.start_lane + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  // the places of a run grow with the lane, so its lanes at places K and
+  // beyond are its last ones; they go in pairs: the first of a pair (a
+  // sphere's front and back at one pixel) runs both sections, in id order,
+  // under one acquisition, and the second gets its tail output back by
+  // shuffle
+  const uint32_t base = old >> 1;
+  const uint32_t first_locker = r.start_lane + (base < (uint32_t)K ? (uint32_t)K - base : 0u);
+  const bool locker = valid && !fill;
+  const bool lead = locker && ((r.lane - first_locker) & 1u) == 0u && r.lane < r.end_lane;
+  const bool follower = locker && ((r.lane - first_locker) & 1u) == 1u;
   const uint32_t p_key = __shfl_down(my_key, 1u), p_col = __shfl_down(my_col, 1u);
   const uint32_t p_mask = __shfl_down(my_mask, 1u);
-  bool pending = valid && !follower;
-  bool p_tail = false;
+  // the early test (its bound read with the places)
+  const bool me_pass = locker && my_key <= bound;
+  const bool p_pass = lead && p_key <= bound;
+  bool p_tail = lead && !p_pass;  // the follower rejected: its own colour goes to the tail
+  if (locker && !follower && !me_pass) tail = true;
   uint32_t p_tail_id = lid + 1u, p_tail_col = p_col, p_tail_mask = p_mask;
-  bool tested = false, me_pass = false, p_pass = false;
+  bool pending = !follower && (me_pass || p_pass);
   uint32_t trips = 0;
-  // The lock loop (:75-123) is driven by a wave-wide ballot: a lane that takes
-  // the lock runs the critical section and releases it in the same trip, and
-  // the compiler cannot hoist the exchange into an inner spin that a lock
-  // holder of the same wave would wait behind.
   while (__ballot(pending)) {
     if (pending && ++trips > kSpinLimit) {  // a lock that is never released: give up with an error
       report_error(kb.err, kErrLockWait);
       pending = false;
     }
     if (pending) {
-      uint32_t bound = 0u;
-      if (!tested) bound = ld_relaxed(&kb.depth[t]);  // :66-69, issued with the first attempt
-      uint32_t count;
-      const bool got = lock_try_count(&kb.lock[t], count);
-      if (!tested) {
-        tested = true;
-        me_pass = my_key <= bound;
-        p_pass = lead && p_key <= bound;
-        if (!me_pass) tail = true;           // rejected: its own colour goes to the tail
-        if (lead && !p_pass) p_tail = true;  // the follower rejected likewise
-      }
-      if (got) {
-        if (me_pass)
-          spin_section<K>(c, kb, t, my_key, my_col, lid, my_mask, count, tail, tail_id, tail_col, tail_mask);
-        if (p_pass)
-          spin_section<K>(c, kb, t, p_key, p_col, lid + 1u, p_mask, count, p_tail, p_tail_id, p_tail_col,
-                          p_tail_mask);
-        lock_release_count(&kb.lock[t], count);
-        pending = false;
-      } else if (!me_pass && !p_pass) {
-        pending = false;  // both rejected by the early test: no section to run
+      const uint32_t stored = ld_relaxed(&kb.filled[t]);
+      if (lock_try(&kb.lock[t])) {
+        if (stored >= (uint32_t)K) {
+          if (me_pass) spin_replace<K>(c, kb, t, my_key, my_col, lid, my_mask, tail, tail_id, tail_col, tail_mask);
+          if (p_pass)
+            spin_replace<K>(c, kb, t, p_key, p_col, lid + 1u, p_mask, p_tail, p_tail_id, p_tail_col, p_tail_mask);
+          pending = false;
+        }
+        lock_release(&kb.lock[t]);
       }
     }
   }

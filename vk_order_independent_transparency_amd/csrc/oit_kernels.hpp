@@ -56,6 +56,7 @@ struct KBuf {          // Simple / Spinlock
   uint32_t* id;       // [K][T]
   uint32_t* mask = nullptr;  // [K][T] coverage shading
   uint32_t* err = nullptr;   // device error word (kErr* bits), null: none
+  uint32_t* filled = nullptr;  // [T] Spinlock: stored places below K whose stores have completed
 };
 
 struct WBBuf {

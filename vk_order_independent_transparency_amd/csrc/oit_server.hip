@@ -72,7 +72,7 @@ uint64_t request_bytes(const oit_request& r, uint64_t F) {
     case OIT_LOOP: return 12 * K * T + tails;
     case OIT_LOOP64: return 8 * K * T + 4 * T + 8 * ((F + 63) / 64) + tails;  // slots, counts, chunk masks
     case OIT_SIMPLE: return 4 * T + 12 * K * T + tails;
-    case OIT_SPINLOCK: return 8 * T + 12 * K * T + tails;
+    case OIT_SPINLOCK: return 12 * T + 12 * K * T + tails;
     default: return 20 * T;
   }
 }
