@@ -880,7 +880,12 @@ __device__ __forceinline__ void stage_decode(float* s_dec, float* s_alpha, const
 // consecutive fragments of one target (a sphere's front and back) are
 // consecutive nodes chained p -> p-1, so one round trip consumes both.
 template <int K, bool kTail, bool kColorTail, bool kDebug, bool kPair>
-__global__ __launch_bounds__(kBlock) void k_ll_resolve_reset(Common c, LLBuf ll, TailBuf tb, ResolveOut o) {
+// At K = 16 without a colour-pass tail the front K and the walk take 81
+// VGPRs (five waves per SIMD); the bound holds them to 80 without spills,
+// for six waves (with the tail fold it would spill).
+__global__ __launch_bounds__(kBlock, (K == 16 && !kColorTail) ? 6 : 1) void k_ll_resolve_reset(Common c, LLBuf ll,
+                                                                                               TailBuf tb,
+                                                                                               ResolveOut o) {
   __shared__ float s_dec[256];
   __shared__ float s_alpha[256];
   stage_decode(s_dec, s_alpha, c.tab);
