@@ -153,8 +153,9 @@ def test_spinlock_bitexact(oracle, K, tail):
     _say("Spinlock", K, tail, len(tg))
     g = run_gpu(SPIN, K, tail, p.width, p.height, tg, d, c)
     o = oracle.run(_oracle_cfg(oracle, oracle.SPINLOCK, K, tail, p), tg, d, c)
-    # the counter counts fragments that passed the early-depth test, which is
-    # arrival dependent beyond K; min(count, K) is not
+    # beyond K the count depends on arrival order (the reference's skips the
+    # fragments its early depth test rejects; this one counts every fragment);
+    # min(count, K) does not
     _assert_same("min(count,K)", np.minimum(g["count"], K), np.minimum(o["count"], K))
     _assert_same("frontk_key", g["frontk_key"], o["frontk_key"])
     _assert_same("frontk_val", g["frontk_val"], o["frontk_val"])
@@ -175,7 +176,14 @@ def test_simple_parity_and_replay(oracle, K, tail):
     small = n <= K
     _assert_same("out (n<=K)", g["out"][small], o["out"][small])
     _assert_same("frontk_val (n<=K)", g["frontk_val"][small], o["frontk_val"][small])
-    # replay: the GPU's stored set through the oracle's composite, the rest as tail
+    _assert_same("out (replay)", g["out"], _simple_replay(oracle, _oracle_cfg(oracle, oracle.SIMPLE, K, tail, p),
+                                                          g, tg, d, c, K, T))
+
+
+def _simple_replay(oracle, cfg, g, tg, d, c, K, T):
+    """Simple's stored set is the first K fragments to arrive: the GPU's stored
+    set goes through the oracle's composite, the other fragments as its tail."""
+    n = per_target_counts(tg, T)
     stored = g["frontk_val"]
     tail_flag = np.ones(len(tg), np.uint8)
     ids = stored[stored != 0xFFFFFFFF].astype(np.int64)
@@ -191,9 +199,8 @@ def test_simple_parity_and_replay(oracle, K, tail):
             ab_col[t, k] = oracle.encode(c[i])
             ab_key[t, k] = oracle.depth_key(float(d[i]))
             ab_id[t, k] = i
-    rep = oracle.replay_kbuffer(_oracle_cfg(oracle, oracle.SIMPLE, K, tail, p), tg, c, n.astype(np.uint32),
-                                ab_col, ab_key, ab_id, tail_flag, tail_quantized=False)
-    _assert_same("out (replay)", g["out"], rep)
+    return oracle.replay_kbuffer(cfg, tg, c, n.astype(np.uint32), ab_col, ab_key, ab_id, tail_flag,
+                                 tail_quantized=False)
 
 
 def _rel_err(got, want, floor):
@@ -328,6 +335,35 @@ def test_identical_keys_and_negative_depth(oracle, alg):
         o = oracle.run(oracle.make_config(alg, K, 1, 3, 2, ll_capacity=cap), tg, d, c)
         _assert_same(f"out K={K}", g["out"], o["out"])
         _assert_same(f"frontk_key K={K}", g["frontk_key"], o["frontk_key"])
+
+
+@pytest.mark.parametrize("alg", [LOOP64, SPIN, SIMPLE, 2])  # 2: Loop32
+def test_long_tails_bit_exact(oracle, alg):
+    """Tails of up to a few hundred fragments per pixel: the composite sorts
+    such a tail list in place instead of walking it once per register window.
+    Runs of identical (depth, colour) keys put boundary-flagged entries in the
+    Loop64 tails; short tails on the other pixels keep the window path busy."""
+    rng = np.random.default_rng(23)
+    W, H = 4, 2
+    per = np.array([300, 41, 33, 32, 9, 120, 2, 0])  # fragments per pixel
+    tg = np.repeat(np.arange(W * H, dtype=np.uint32), per)
+    d = rng.uniform(0.0, 1.0, len(tg)).astype(np.float32)
+    c = rng.uniform(0.05, 0.4, (len(tg), 4)).astype(np.float32)
+    c[:, :3] *= c[:, 3:4]
+    dup = np.flatnonzero(tg == 5)[:60]  # pixel 5: 60 fragments with one key
+    d[dup] = 0.625
+    c[dup] = c[dup[0]]
+    order = rng.permutation(len(tg))
+    tg, d, c = tg[order], d[order], c[order]
+    for K in (1, 8):
+        _say("long tails", alg, K, len(tg))
+        g = run_gpu(alg, K, 1, W, H, tg, d, c)
+        o = oracle.run(oracle.make_config(alg, K, 1, W, H), tg, d, c)
+        if alg == SIMPLE:
+            cfg = oracle.make_config(alg, K, 1, W, H)
+            _assert_same(f"out K={K} (replay)", g["out"], _simple_replay(oracle, cfg, g, tg, d, c, K, W * H))
+        else:
+            _assert_same(f"out K={K}", g["out"], o["out"])
 
 
 @pytest.mark.parametrize("alg", [LL, LOOP64, SPIN, SIMPLE])

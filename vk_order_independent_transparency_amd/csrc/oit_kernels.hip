@@ -278,8 +278,12 @@ __device__ __forceinline__ float4 blend_front(const FrontK<K>& a, uint32_t n, co
   return c;
 }
 
-// Tail values held in registers per walk of an unordered tail list.
-constexpr int kTailWindow = 16;
+// Tail values held in registers per walk of an unordered tail list, and
+// the walks a fold takes before it sorts a longer list instead. At 1080p
+// with K = 8 the longest tail is 14 (1 % of the tails need a second walk);
+// a window of 16 cost the K = 8 composites two to three waves per SIMD.
+constexpr int kTailWindow = 8;
+constexpr uint32_t kTailWalks = 4;
 
 // One walk of an unordered tail list: the kTailWindow largest values below
 // `bound`, in descending order (0 = none), by an insertion network in
@@ -316,13 +320,57 @@ __device__ __forceinline__ uint32_t tail_window(const TailBuf& tb, uint32_t head
   return n;
 }
 
+// A tail list longer than kTailWindow * kTailWalks, sorted into descending
+// order in place: a bottom-up merge sort of the singly linked list (runs of
+// 1, 2, 4, ... merged pass by pass, each pass relinking every node once),
+// O(n log n) link reads instead of one walk per window. The list is the
+// composite lane's own and acyclic (tail_window walked it to its end first);
+// a link at or above vcap ends it. Returns the new head.
+__device__ __forceinline__ uint32_t tail_sort_desc(const TailBuf& tb, uint32_t head, uint32_t vcap) {
+  auto nxt = [&](uint32_t v) -> uint32_t {
+    const uint32_t x = tb.next[tail_index(v)];
+    return x < vcap ? x : 0u;
+  };
+  for (uint32_t run = 1;; run <<= 1) {
+    uint32_t p = head, out = 0u, last = 0u, merges = 0u;
+    while (p != 0u) {
+      ++merges;
+      uint32_t q = p, np = 0u;
+      while (np < run && q != 0u) {
+        ++np;
+        q = nxt(q);
+      }
+      uint32_t nq = run;
+      while (np > 0u || (nq > 0u && q != 0u)) {
+        uint32_t e;
+        if (np == 0u || (nq > 0u && q != 0u && q > p)) {  // descending: the larger value first
+          e = q;
+          q = nxt(q);
+          --nq;
+        } else {
+          e = p;
+          p = nxt(p);
+          --np;
+        }
+        if (last == 0u) out = e;
+        else tb.next[tail_index(last)] = e;
+        last = e;
+      }
+      p = q;
+    }
+    if (last != 0u) tb.next[tail_index(last)] = 0u;
+    head = out;
+    if (merges <= 1u) return head;
+  }
+}
+
 // Colour buffer after the colour pass: the tail fragments of target t over
 // the background in id order (the ROP's primitive order, main.cpp:548-558),
 // folded newest first: acc = t_n over ... over t_1, then acc over the
 // background. The list is unordered, so each walk takes the next
-// kTailWindow values in descending order (one walk for most targets; a
-// longer tail takes one walk per window) and loads their colours four at a
-// time. `skip` boundary-flagged entries are in the A-buffer and not blended
+// kTailWindow values in descending order (one walk for most targets, up to
+// kTailWalks) and loads their colours four at a time; a longer tail is
+// sorted in place and folded in one walk. `skip` boundary-flagged entries are in the A-buffer and not blended
 // (Loop64 identical keys, see k_loop64_classify): the flagged ones with the
 // lowest ids.
 // tail_fold with the list's head value already loaded (a composite issues
@@ -339,6 +387,18 @@ __device__ __forceinline__ float4 tail_fold_from(const TailBuf& tb, uint32_t hea
     uint32_t flagged;
     const uint32_t n = tail_window(tb, head, vcap, bound, w, flagged, bad);
     if (first && skip > 0u) keep = flagged > skip ? flagged - skip : 0u;
+    if (first && !bad && n > (uint32_t)kTailWindow * kTailWalks) {  // a long tail: sorted, then one walk
+      for (uint32_t v = tail_sort_desc(tb, head, vcap); v != 0u && v < vcap;) {
+        const uint32_t id = tail_index(v);
+        const uint32_t nx = tb.next[id];
+        if (!(v & 1u) || keep > 0u) {
+          acc = over(acc, tb.rgba[id]);
+          if (v & 1u) --keep;
+        }
+        v = nx;
+      }
+      break;
+    }
 #pragma unroll
     for (int g = 0; g < kTailWindow; g += 4) {
       if (w[g] == 0u) break;
@@ -429,6 +489,19 @@ __device__ __forceinline__ float4 tail_fold_cov(const TailBuf& tb, uint32_t t, c
     uint32_t w[kTailWindow];
     uint32_t flagged;
     const uint32_t n = tail_window(tb, head, vcap, bound, w, flagged, bad);
+    if (bound == 0xFFFFFFFFu && !bad && n > (uint32_t)kTailWindow * kTailWalks) {  // a long tail: sorted, one walk
+      for (uint32_t v = tail_sort_desc(tb, head, vcap); v != 0u && v < vcap;) {
+        const uint32_t id = tail_index(v);
+        const uint32_t nx = tb.next[id];
+        const uint32_t m = tb.mask[id];
+        const float4 col = tb.rgba[id];
+#pragma unroll
+        for (int s = 0; s < 8; ++s)
+          if ((uint32_t)s < S && ((m >> s) & 1u)) acc[s] = over(acc[s], col);
+        v = nx;
+      }
+      break;
+    }
 #pragma unroll
     for (int g = 0; g < kTailWindow; g += 4) {
       if (w[g] == 0u) break;
