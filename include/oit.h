@@ -170,8 +170,10 @@ int oit_resolve(oit_ctx* ctx, float* out_rgba, void* stream);
 /* Like oit_resolve, and additionally writes per target the sorted front-K
  * depth keys and values (fragment ids; packed colours for OIT_LOOP64) to
  * frontk_key/frontk_val[targets * layers] (0xFFFFFFFF padding) and the
- * algorithm's per-target count to counts[targets]. Any of the three extra
- * pointers may be NULL. For parity tests; not a reference entry point. */
+ * algorithm's per-target count to counts[targets] (OIT_SPINLOCK: every
+ * fragment that reached the target, where the reference skips those its early
+ * depth test rejects, so beyond `layers` the two differ). Any of the three
+ * extra pointers may be NULL. For parity tests; not a reference entry point. */
 int oit_resolve_debug(oit_ctx* ctx, float* out_rgba, uint32_t* frontk_key,
                       uint32_t* frontk_val, uint32_t* counts, void* stream);
 /* Failure-handling tests only (no reference counterpart): writes `value`
