@@ -786,25 +786,16 @@ int oit_clear(oit_ctx* c, void* stream) {
   const uint64_t T = c->T, K = c->cfg.layers;
   const bool ll = pooled(c->cfg.algorithm);
   const bool need_ll_fill = ll && !c->ll_clean;
-  const bool need_counters = !ll || !c->counters_clean;
+  // only the Linked List's kernels write the frame counters (its resolve
+  // leaves them at 0): the other algorithms' stay as the last fill left them
+  const bool need_counters = !c->counters_clean;
   const bool need_tail = c->tl_head && (!ll || c->tl_next);
-  if (!need_ll_fill && !need_counters && !need_tail) {
-    // steady state of the Linked List: the previous resolve left heads at 0,
-    // links at kUnset and the counters at 0 (the clear of oitRender.cpp:
-    // 206-212 is done by the reader). Tail-list heads were not cleared: they
-    // are filled on first use (ensure_tail_capacity).
-    c->tl_head_valid = false;
-    c->frame_frags = 0;
-    c->appends = 0;
-    c->opq_active = false;
-    c->opq_frags = 0;
-    c->segments.clear();
-    c->frame_open = true;
-    c->resolved = false;
-    c->stats_moved = false;
-    return OIT_OK;
-  }
-  // every fill of the clear in one launch
+  // every fill of the clear in one launch, none where nothing needs one: the
+  // steady state of the Linked List (its resolve left heads at 0, links at
+  // kUnset and the counters at 0: the clear of oitRender.cpp:206-212 is done
+  // by the reader) and of WBOIT (its composite left the accumulators at 0).
+  // Tail-list heads not filled here are filled on first use
+  // (ensure_tail_capacity).
   oit::FillRange fr[8];
   int nf = 0;
   auto fill = [&](void* p, uint64_t n, uint32_t v) { fr[nf++] = oit::FillRange{static_cast<uint32_t*>(p), n, v}; };
@@ -853,14 +844,15 @@ int oit_clear(oit_ctx* c, void* stream) {
     fill(c->tl_head, T, 0u);  // empty tail lists
     if (c->tl_qn) fill(c->tl_qn, oit::tail_regions(c->tl_cap), 0u);  // and an empty tail queue
   }
-  hipError_t e = staged(c, kStClear, s, [&]() -> hipError_t { return oit::launch_fill_ranges(fr, nf, s); });
+  hipError_t e = hipSuccess;
+  if (nf > 0) e = staged(c, kStClear, s, [&]() -> hipError_t { return oit::launch_fill_ranges(fr, nf, s); });
   if (e == hipSuccess && l64_fill) c->l64_clean = true;
   if (e == hipSuccess && l32_fill) c->l32_clean = true;
   if (e == hipSuccess && wb_fill) c->wb_clean = true;
+  if (e == hipSuccess) c->counters_clean = true;
   if (e == hipSuccess && ll) {
     c->ll_clean = true;
     c->ll_dirty_upto = 0;
-    c->counters_clean = true;
   }
   c->stats_moved = false;
   c->tl_head_valid = e == hipSuccess && need_tail;
@@ -871,7 +863,7 @@ int oit_clear(oit_ctx* c, void* stream) {
   c->segments.clear();
   c->frame_open = e == hipSuccess;
   c->resolved = false;
-  mark_work(c, s);
+  if (nf > 0) mark_work(c, s);
   return status_of(e);
 }
 
