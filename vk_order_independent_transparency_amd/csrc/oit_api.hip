@@ -113,6 +113,7 @@ struct oit_ctx {
   uint32_t* l64_cnt = nullptr;  // [T] the frame's fragments per target (0 = clean)
   uint64_t* l64_big = nullptr;  // per 64-fragment chunk of the frame: lanes whose pixel holds more than K
   bool l64_clean = false;  // every slot empty and every count 0 (a fill, or the last frame's composite)
+  bool kb_clean = false;   // Simple / Spinlock: counts, lock words and bounds at their clear values, likewise
   bool l32_clean = false;  // Loop32: likewise for the depth and colour slots
   bool wb_clean = false;   // WBOIT: accumulators at the clear values (a fill, or the last composite)
   // WBOIT
@@ -269,6 +270,7 @@ void free_frame_buffers(oit_ctx* c) {
   dev_free(&c->l64_cnt);
   dev_free(&c->l64_big);
   c->l64_clean = false;
+  c->kb_clean = false;
   dev_free(&c->wb_accum);
   dev_free(&c->wb_reveal);
   c->wb_clean = false;
@@ -719,6 +721,7 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
   c->l64_clean = false;  // slots beyond the last frame's T x K may hold an older frame's keys
   c->l32_clean = false;
   c->wb_clean = false;
+  c->kb_clean = false;
   if (pooled(alg) && ll_grown) {
     // a new allocation is filled clean (every head 0, every link kUnset) by
     // the next clear, on its stream
@@ -800,7 +803,7 @@ int oit_clear(oit_ctx* c, void* stream) {
   int nf = 0;
   auto fill = [&](void* p, uint64_t n, uint32_t v) { fr[nf++] = oit::FillRange{static_cast<uint32_t*>(p), n, v}; };
   if (need_counters) fill(c->d_counters, 16, 0u);
-  bool l64_fill = false, l32_fill = false, wb_fill = false;
+  bool l64_fill = false, l32_fill = false, wb_fill = false, kb_fill = false;
   switch (c->cfg.algorithm) {
     case OIT_LOOP:  // depths to all ones (oitRender.cpp:242-265), colour slots to 0; the composite leaves them so
       l32_fill = !c->l32_clean;
@@ -817,13 +820,17 @@ int oit_clear(oit_ctx* c, void* stream) {
         fill(c->ll_next, c->ll_dirty_upto ? c->ll_dirty_upto : c->ll_alloc, oit::kUnset);
       }
       break;
-    case OIT_SIMPLE:  // count to 0, oitRender.cpp:161-169
-      fill(c->kb_count, T, 0u);
+    case OIT_SIMPLE:  // count to 0, oitRender.cpp:161-169; the composite leaves it so
+      kb_fill = !c->kb_clean;
+      if (kb_fill) fill(c->kb_count, T, 0u);
       break;
-    case OIT_SPINLOCK:  // count, spin to 0, depth to ~0, oitRender.cpp:343-354 (the count is in the lock word)
-      fill(c->kb_lock, T, 0u);
-      fill(c->kb_depth, T, kEmpty);
-      fill(c->kb_filled, T, 0u);
+    case OIT_SPINLOCK:  // count, spin to 0, depth to ~0, oitRender.cpp:343-354 (the count is in the lock word); likewise
+      kb_fill = !c->kb_clean;
+      if (kb_fill) {
+        fill(c->kb_lock, T, 0u);
+        fill(c->kb_depth, T, kEmpty);
+        fill(c->kb_filled, T, 0u);
+      }
       break;
     case OIT_LOOP64:  // every byte to 0xFF, oitRender.cpp:307; the composite leaves its slots empty
       l64_fill = !c->l64_clean;
@@ -849,6 +856,7 @@ int oit_clear(oit_ctx* c, void* stream) {
   if (e == hipSuccess && l64_fill) c->l64_clean = true;
   if (e == hipSuccess && l32_fill) c->l32_clean = true;
   if (e == hipSuccess && wb_fill) c->wb_clean = true;
+  if (e == hipSuccess && kb_fill) c->kb_clean = true;
   if (e == hipSuccess) c->counters_clean = true;
   if (e == hipSuccess && ll) {
     c->ll_clean = true;
@@ -974,6 +982,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       break;
     }
     case OIT_SIMPLE: {
+      c->kb_clean = false;
       oit::KBuf kb{c->kb_count, nullptr, nullptr, c->kb_col, c->kb_key, c->kb_id};
       kb.err = dev_err(c);
       kb.mask = coverage(c) ? c->kb_mask : nullptr;
@@ -981,6 +990,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
       break;
     }
     case OIT_SPINLOCK: {
+      c->kb_clean = false;
       oit::KBuf kb{nullptr, c->kb_lock, c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
       kb.err = dev_err(c);
       kb.filled = c->kb_filled;
@@ -1096,12 +1106,14 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
     case OIT_SPINLOCK: {
       oit::KBuf kb{alg == OIT_SPINLOCK ? nullptr : c->kb_count, alg == OIT_SPINLOCK ? c->kb_lock : nullptr,
                    c->kb_depth, c->kb_col, c->kb_key, c->kb_id};
+      kb.filled = alg == OIT_SPINLOCK ? c->kb_filled : nullptr;
       if (coverage(c)) {
         kb.mask = c->kb_mask;
         e = staged(c, kStResolve, s, [&] { return oit::launch_kbuf_resolve_cov(k, kb, tbr, o, s); });
       } else {
         e = staged(c, kStResolve, s, [&] { return oit::launch_kbuf_resolve(k, kb, tbr, o, s); });
       }
+      c->kb_clean = e == hipSuccess;  // the composite set every word it read back to its clear value
       break;
     }
     case OIT_LOOP64:
@@ -1190,6 +1202,7 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
     c->l64_clean = false;
     c->l32_clean = false;
     c->wb_clean = false;
+    c->kb_clean = false;
     return OIT_ERR_DEVICE;
   }
   return OIT_OK;
@@ -1211,6 +1224,10 @@ int oit_debug_set_word(oit_ctx* c, uint32_t what, uint64_t index, uint32_t value
   hipStream_t s = static_cast<hipStream_t>(stream);
   const hipError_t e = oit::launch_fill_u32(base + index, 1, value, s);
   mark_work(c, s);
+  // a planted word is not the clear value a later clear may assume
+  c->ll_clean = false;
+  c->ll_dirty_upto = c->ll_alloc;
+  c->kb_clean = false;
   return status_of(e);
 }
 

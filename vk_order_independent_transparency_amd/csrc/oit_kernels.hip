@@ -1774,6 +1774,22 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
   }
 }
 
+// The target's Simple / Spinlock words back to their clear values for the
+// next frame (the clears of oitRender.cpp:161-169, 343-354, done by the
+// composite that read them): Simple's count; the Spinlock's lock word, its
+// completed-store count and its early-depth bound, which only an eviction
+// sets, so only where the count passed K.
+__device__ __forceinline__ void kbuf_reset(const KBuf& kb, uint32_t t, uint32_t count, uint32_t K) {
+  if (count == 0u) return;
+  if (kb.lock) {
+    kb.lock[t] = 0u;
+    if (kb.filled) kb.filled[t] = 0u;
+    if (count > K) kb.depth[t] = kEmpty;
+  } else {
+    kb.count[t] = 0u;
+  }
+}
+
 // Composite of Simple and Spinlock (oitSimple.frag.glsl:115-169 ==
 // oitSpinlock.frag.glsl:153-207): n = min(count, K) elements, sorted,
 // blended front to back, over the colour buffer.
@@ -1786,6 +1802,7 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve(Common c, KBuf kb, Tail
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
   const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
+  kbuf_reset(kb, t, count, (uint32_t)K);
   const uint32_t th = kColorTail ? tb.head[t] : 0u;                   // tail list head, with the count
   const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
   // layers past the wave's largest count are skipped with a scalar branch,
@@ -1839,6 +1856,7 @@ __global__ __launch_bounds__(kBlock) void k_kbuf_resolve_cov(Common c, KBuf kb, 
   const uint32_t t = blockIdx.x * kBlock + threadIdx.x;
   if (t >= c.T) return;
   const uint32_t count = kb.lock ? kb.lock[t] >> 1 : kb.count[t];  // Spinlock: the count in the lock word
+  kbuf_reset(kb, t, count, (uint32_t)K);
   const uint32_t n = count < (uint32_t)K ? count : (uint32_t)K;
   const uint32_t nmax = wave_max_u32(n);
   uint32_t lk[K], li[K];  // loaded before the first insert: one round trip
