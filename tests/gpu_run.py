@@ -67,3 +67,25 @@ def run_gpu(algorithm, layers, tail_blend, width, height, target, depth, rgba, s
     finally:
         if own:
             r.close()
+
+
+def simple_replay(oracle, cfg, stored, tg, d, c, K, T):
+    """Simple's stored set is the first K fragments to arrive: the GPU's stored
+    set goes through the oracle's composite, the other fragments as its tail."""
+    n = np.bincount(np.asarray(tg, np.int64), minlength=T)
+    tail_flag = np.ones(len(tg), np.uint8)
+    ids = stored[stored != 0xFFFFFFFF].astype(np.int64)
+    assert len(np.unique(ids)) == len(ids)
+    assert np.array_equal(np.sort(tg[ids]), np.sort(np.repeat(np.arange(T), np.minimum(n, K))))
+    tail_flag[ids] = 0
+    ab_col = np.zeros((T, K), np.uint32)
+    ab_key = np.zeros((T, K), np.uint32)
+    ab_id = np.zeros((T, K), np.uint32)
+    for t in np.nonzero(n > 0)[0]:
+        for k in range(min(n[t], K)):
+            i = int(stored[t, k])
+            ab_col[t, k] = oracle.encode(c[i])
+            ab_key[t, k] = oracle.depth_key(float(d[i]))
+            ab_id[t, k] = i
+    return oracle.replay_kbuffer(cfg, tg, c, n.astype(np.uint32), ab_col, ab_key, ab_id, tail_flag,
+                                 tail_quantized=False)

@@ -12,7 +12,7 @@ import numpy as np
 import pytest
 
 from conftest import deep_scene, per_target_counts
-from gpu_run import run_gpu
+from gpu_run import run_gpu, simple_replay
 
 pytestmark = pytest.mark.gpu
 
@@ -176,31 +176,8 @@ def test_simple_parity_and_replay(oracle, K, tail):
     small = n <= K
     _assert_same("out (n<=K)", g["out"][small], o["out"][small])
     _assert_same("frontk_val (n<=K)", g["frontk_val"][small], o["frontk_val"][small])
-    _assert_same("out (replay)", g["out"], _simple_replay(oracle, _oracle_cfg(oracle, oracle.SIMPLE, K, tail, p),
-                                                          g, tg, d, c, K, T))
-
-
-def _simple_replay(oracle, cfg, g, tg, d, c, K, T):
-    """Simple's stored set is the first K fragments to arrive: the GPU's stored
-    set goes through the oracle's composite, the other fragments as its tail."""
-    n = per_target_counts(tg, T)
-    stored = g["frontk_val"]
-    tail_flag = np.ones(len(tg), np.uint8)
-    ids = stored[stored != 0xFFFFFFFF].astype(np.int64)
-    assert len(np.unique(ids)) == len(ids)
-    assert np.array_equal(np.sort(tg[ids]), np.sort(np.repeat(np.arange(T), np.minimum(n, K))))
-    tail_flag[ids] = 0
-    ab_col = np.zeros((T, K), np.uint32)
-    ab_key = np.zeros((T, K), np.uint32)
-    ab_id = np.zeros((T, K), np.uint32)
-    for t in np.nonzero(n > 0)[0]:
-        for k in range(min(n[t], K)):
-            i = int(stored[t, k])
-            ab_col[t, k] = oracle.encode(c[i])
-            ab_key[t, k] = oracle.depth_key(float(d[i]))
-            ab_id[t, k] = i
-    return oracle.replay_kbuffer(cfg, tg, c, n.astype(np.uint32), ab_col, ab_key, ab_id, tail_flag,
-                                 tail_quantized=False)
+    rep = simple_replay(oracle, _oracle_cfg(oracle, oracle.SIMPLE, K, tail, p), g["frontk_val"], tg, d, c, K, T)
+    _assert_same("out (replay)", g["out"], rep)
 
 
 def _rel_err(got, want, floor):
@@ -361,7 +338,8 @@ def test_long_tails_bit_exact(oracle, alg):
         o = oracle.run(oracle.make_config(alg, K, 1, W, H), tg, d, c)
         if alg == SIMPLE:
             cfg = oracle.make_config(alg, K, 1, W, H)
-            _assert_same(f"out K={K} (replay)", g["out"], _simple_replay(oracle, cfg, g, tg, d, c, K, W * H))
+            rep = simple_replay(oracle, cfg, g["frontk_val"], tg, d, c, K, W * H)
+            _assert_same(f"out K={K} (replay)", g["out"], rep)
         else:
             _assert_same(f"out K={K}", g["out"], o["out"])
 
