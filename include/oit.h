@@ -52,7 +52,8 @@ enum oit_algorithm {
 #define OIT_VARIANT_LL_SPLIT_APPEND  0x1u /* Linked List: separate store and link kernels */
 #define OIT_VARIANT_LL_SINGLE_WALK   0x2u /* Linked List: resolve walks one node per round trip */
 #define OIT_VARIANT_LL_APPEND_8      0x4u /* Linked List: fused append takes 8 chunks per wave (default 4) */
-#define OIT_VARIANT_LL_KEYS_ONLY     0x8u /* Linked List: resolve keeps sort keys only and reloads colours by node */
+#define OIT_VARIANT_LL_KEYS_ONLY     0x8u /* Linked List: resolve keeps sort keys only and reloads colours by node
+                                             (always at 32 layers) */
 #define OIT_VARIANT_STORE_CHUNKS_1   0x10u /* Loop64: store one fragment per lane (default: two 64-fragment chunks per wave) */
 #define OIT_VARIANT_STORE_CHUNKS_4   0x20u /* Loop64: store four 64-fragment chunks per wave */
 
