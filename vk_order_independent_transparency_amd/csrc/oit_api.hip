@@ -127,6 +127,7 @@ struct oit_ctx {
   uint32_t* tl_qn = nullptr;  // tail fragments (queue entries) per region [oit::tail_regions(tl_cap)]
   uint32_t tl_cap = 0;
   bool tl_head_valid = false;  // cleared for the current frame
+  bool tl_clean = false;       // tail heads and region counters all 0: filled, and no tail pushed since
   // frame state: oit_clear opens a frame; the first resolve links the tail
   // lists, after which the frame takes no more appends.
   bool frame_open = false;
@@ -281,6 +282,7 @@ void free_frame_buffers(oit_ctx* c) {
   dev_free(&c->tl_qn);
   c->tl_cap = 0;
   c->tl_head_valid = false;
+  c->tl_clean = false;
   c->ll_cap = 0;
   c->bytes_abuf = c->bytes_aux = 0;
   c->configured = false;
@@ -718,6 +720,7 @@ int oit_configure(oit_ctx* c, const oit_config* cfg) {
     return rc;
   }
   c->tl_head_valid = false;
+  c->tl_clean = false;
   c->l64_clean = false;  // slots beyond the last frame's T x K may hold an older frame's keys
   c->l32_clean = false;
   c->wb_clean = false;
@@ -792,7 +795,9 @@ int oit_clear(oit_ctx* c, void* stream) {
   // only the Linked List's kernels write the frame counters (its resolve
   // leaves them at 0): the other algorithms' stay as the last fill left them
   const bool need_counters = !c->counters_clean;
-  const bool need_tail = c->tl_head && (!ll || c->tl_next);
+  // tail lists: filled unless no tail was pushed since the last fill (a
+  // Linked List frame whose pool could not overflow)
+  const bool need_tail = c->tl_head && (!ll || c->tl_next) && !c->tl_clean;
   // every fill of the clear in one launch, none where nothing needs one: the
   // steady state of the Linked List (its resolve left heads at 0, links at
   // kUnset and the counters at 0: the clear of oitRender.cpp:206-212 is done
@@ -863,7 +868,8 @@ int oit_clear(oit_ctx* c, void* stream) {
     c->ll_dirty_upto = 0;
   }
   c->stats_moved = false;
-  c->tl_head_valid = e == hipSuccess && need_tail;
+  if (e == hipSuccess && need_tail) c->tl_clean = true;
+  c->tl_head_valid = e == hipSuccess && c->tl_head != nullptr && c->tl_clean;
   c->frame_frags = 0;
   c->appends = 0;
   c->opq_active = false;
@@ -930,6 +936,7 @@ int oit_append(oit_ctx* c, const oit_fragments* f, void* stream) {
     if (need) {
       int rc = ensure_tail_capacity(c, end, s);
       if (rc) return rc;
+      c->tl_clean = false;  // this append may push tail fragments
     }
   }
   oit::FragView v;
@@ -1032,6 +1039,7 @@ int oit_resolve_debug(oit_ctx* c, float* out_rgba, uint32_t* fk_key, uint32_t* f
   if (link_tails && alg == OIT_LOOP64 && c->cfg.tail_blend && !c->segments.empty()) {
     int rc = ensure_tail_capacity(c, c->frame_frags, s);
     if (rc) return rc;
+    c->tl_clean = false;  // the classification pushes tail fragments
   }
   const oit::Common k = common_of(c);
   const oit::TailBuf tb = tail_of(c);
@@ -1205,6 +1213,7 @@ int oit_get_stats(oit_ctx* c, oit_stats* out, void* stream) {
     c->l32_clean = false;
     c->wb_clean = false;
     c->kb_clean = false;
+    c->tl_clean = false;
     return OIT_ERR_DEVICE;
   }
   return OIT_OK;
