@@ -26,7 +26,7 @@ sys.path.insert(0, ROOT)
 # HBM bytes per launch of the dominant kernel, measured with rocprofv3 --pmc
 # FETCH_SIZE and WRITE_SIZE (scaled by a 1 GiB copy: FETCH x 2, WRITE x 1) on
 # this workload (profiles/linkedlist_k16_1920x1080_1000spheres.md).
-MEASURED_TRAFFIC = {("linkedlist_k16_1920x1080_1000spheres", "store", 0): 559212646}
+MEASURED_TRAFFIC = {("linkedlist_k16_1920x1080_1000spheres", "store", 0): 559654502}
 
 
 def log(*a):
