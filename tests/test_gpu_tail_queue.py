@@ -1,6 +1,8 @@
 """Colour-pass tail queue: a tail fragment's record goes to the region of
-1024 fragment ids that holds its id, and the link pass takes the regions in
-order. Frames whose tails fill many regions (K = 1: nearly every fragment),
+1024 fragment ids that holds its id, and the link pass pushes the records
+onto unordered per-target lists (Loop64's classification pushes its own);
+the composites fold each list in id order. Frames whose tails fill many
+regions (K = 1: nearly every fragment),
 a partial last region, appends whose first ids fall inside a region, and a
 Linked List pool that overflows inside a region: bit-exact with the oracle,
 with the same tail count as one append of the whole stream."""

@@ -18,15 +18,17 @@ constexpr uint64_t kEmpty64 = 0xFFFFFFFFFFFFFFFFull;
 
 // Device error bits (oit_stats.device_errors). Every loop in which a lane
 // waits for other lanes (a list insert meeting a link word that is not yet
-// written, the Spinlock lock) gives up after kSpinLimit trips, and every list
-// walk requires strictly descending values; a lane that gives up sets a bit
+// written, the Spinlock lock) gives up after kSpinLimit trips, every Linked
+// List walk requires strictly descending values, and a walk of an (unordered)
+// colour-pass tail list ends after as many hops as the arrays hold
+// fragments; a lane that gives up sets a bit
 // in the context's error word and leaves the loop, so a corrupted structure
 // ends the kernel with an error instead of hanging the queue. A correct
 // frame waits at most a few trips per lane for a link word and, for the
 // lock, at most the trips of the critical sections queued before it.
 constexpr uint32_t kErrListWait = 1u;  // a list insert exceeded kSpinLimit trips
 constexpr uint32_t kErrLockWait = 2u;  // a Spinlock section waited kSpinLimit trips for its lock
-constexpr uint32_t kErrWalk = 4u;      // a list walk met a link that does not descend
+constexpr uint32_t kErrWalk = 4u;      // a list walk met a link that does not descend (or a tail walk overran)
 constexpr uint32_t kSpinLimit = 1u << 20;
 
 __device__ __forceinline__ void report_error(uint32_t* err, uint32_t bit) {
