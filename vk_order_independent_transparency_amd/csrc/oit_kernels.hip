@@ -92,19 +92,6 @@ __device__ __forceinline__ void tail_queue_push(bool tail, uint32_t t, uint32_t 
     tb.rec[((uint64_t)r << kTailRegionShift) + q + lanes_below(my_group)] = ((uint64_t)t << 32) | tail_value(lid, flag);
 }
 
-// tail_record_own: the lane's own fragment goes to the tail.
-__device__ __forceinline__ void tail_record_own(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
-                                                const TailBuf& tb, uint32_t mask = 0xFFFFFFFFu) {
-  tail_queue_push(tail, t, lid, flag, payload, tb, mask);
-}
-
-// tail_record_evict: fragment `lid` (the lane's own or one it pushed out of
-// the A-buffer) goes to the tail.
-__device__ __forceinline__ void tail_record_evict(bool tail, uint32_t t, uint32_t lid, uint32_t flag,
-                                                  float4 payload, const TailBuf& tb, uint32_t mask = 0xFFFFFFFFu) {
-  tail_queue_push(tail, t, lid, flag, payload, tb, mask);
-}
-
 // A converged wave's tail fragments straight onto their targets' lists,
 // for a pass that runs after every fragment of the frame is stored (no link
 // pass follows): neighbouring lanes of one target are chained in the wave
@@ -659,7 +646,7 @@ __global__ __launch_bounds__(kBlock) void k_ll_store(Common c, FragView f, LLBuf
   }
   const uint64_t om = __ballot(ovf);
   if (om && lane_id() == (uint32_t)__builtin_ctzll(om)) atomicAdd(ll.overflow, (uint32_t)__builtin_popcountll(om));
-  if constexpr (kTail) tail_record_own(ovf, t, lid, 0u, col, tb);
+  if constexpr (kTail) tail_queue_push(ovf, t, lid, 0u, col, tb, 0xFFFFFFFFu);
 }
 
 // Link half: the head exchange of oitLinkedList.frag.glsl:72, done as an
@@ -791,7 +778,7 @@ __device__ __forceinline__ void ll_append_wave(const AppendSrc& f, const float* 
     if constexpr (kTail) {  // records exist only where the pool can overflow (node >= cap)
       const uint64_t i = f.first + (uint64_t)u * 64u + lane;
       const uint32_t lid = (uint32_t)(f.lid0 + i);
-      if ((uint64_t)f.node0 + i >= ll.cap) tail_record_own(ovf[u], t[u], lid, 0u, col[u], tb, m[u]);
+      if ((uint64_t)f.node0 + i >= ll.cap) tail_queue_push(ovf[u], t[u], lid, 0u, col[u], tb, m[u]);
     }
   }
 }
@@ -1593,24 +1580,30 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
       tail = true;
     }
   }
-  if constexpr (kTail) tail_record_own(tail, t, lid, 0u, col, tb, m);
+  if constexpr (kTail) tail_queue_push(tail, t, lid, 0u, col, tb, m);
 }
 
 // ---------------------------------------------------------------------------
 // Spinlock: oitSpinlock.frag.glsl
 // ---------------------------------------------------------------------------
 
-// The critical section of oitSpinlock.frag.glsl:95-123 for the fragment
-// (key, col, id, mask) of target t, run by the lane that holds t's lock on a
-// full buffer: replace the furthest element (first index wins, :96-107) when
-// the newcomer is nearer (:109-117). Sets the tail output: the evicted
-// element, or the fragment itself.
+// The critical section of oitSpinlock.frag.glsl:95-123, run by the lane
+// that holds target t's lock on a full buffer, for the fragment a and then
+// for the fragment b (a pair; either may be absent): each replaces the
+// furthest element (first index wins, :96-107) when it is nearer (:109-117).
+// One scan serves both sections: it finds the furthest element and the
+// furthest of the others, so after a replaces the furthest, the furthest is
+// a or that second one, and b's section needs no scan of its own. Each
+// section sets its tail output: the evicted element, or the fragment itself.
 template <int K>
-__device__ __forceinline__ void spin_replace(const Common& c, const KBuf& kb, uint32_t t, uint32_t key, uint32_t col,
-                                             uint32_t id, uint32_t mask, bool& tail, uint32_t& tail_id,
-                                             uint32_t& tail_col, uint32_t& tail_mask) {
-  uint32_t fur = 0;
-  uint64_t mx = 0;
+__device__ __forceinline__ void spin_replace(const Common& c, const KBuf& kb, uint32_t t, bool run_a, uint32_t a_key,
+                                             uint32_t a_col, uint32_t a_id, uint32_t a_mask, bool& a_tail,
+                                             uint32_t& a_tail_id, uint32_t& a_tail_col, uint32_t& a_tail_mask,
+                                             bool run_b, uint32_t b_key, uint32_t b_col, uint32_t b_id,
+                                             uint32_t b_mask, bool& b_tail, uint32_t& b_tail_id, uint32_t& b_tail_col,
+                                             uint32_t& b_tail_mask) {
+  uint32_t fur = 0, sec = 0;
+  uint64_t mx = 0, mx2 = 0;
   // the scan as a loop of eight loads per trip (K/2 below 16): unrolled in
   // full it held 100-256 VGPRs (three or four waves per SIMD at K = 8, 16),
   // as a loop 49-63 (eight waves)
@@ -1620,25 +1613,73 @@ __device__ __forceinline__ void spin_replace(const Common& c, const KBuf& kb, ui
     const uint64_t idx = (uint64_t)j * c.T + t;
     const uint64_t v = ((uint64_t)ld_relaxed(&kb.key[idx]) << 32) | ld_relaxed(&kb.id[idx]);
     if (v > mx) {
+      mx2 = mx;
+      sec = fur;
       mx = v;
       fur = (uint32_t)j;
+    } else if (v > mx2) {
+      mx2 = v;
+      sec = (uint32_t)j;
     }
   }
-  const uint64_t me = ((uint64_t)key << 32) | id;
-  tail = true;
-  if (mx > me) {
-    const uint64_t idx = (uint64_t)fur * c.T + t;
-    tail_col = ld_relaxed(&kb.col[idx]);
-    tail_id = (uint32_t)mx;
-    if (kb.mask) {
-      tail_mask = ld_relaxed(&kb.mask[idx]);
-      st_relaxed(&kb.mask[idx], mask);
-    }
-    st_relaxed(&kb.col[idx], col);
-    st_relaxed(&kb.key[idx], key);
-    st_relaxed(&kb.id[idx], id);
-    st_relaxed(&kb.depth[t], (uint32_t)(mx >> 32));
+  // the colours (and masks) the sections can evict, in one round trip
+  const uint64_t i_fur = (uint64_t)fur * c.T + t, i_sec = (uint64_t)sec * c.T + t;
+  const uint32_t col_fur = ld_relaxed(&kb.col[i_fur]), col_sec = ld_relaxed(&kb.col[i_sec]);
+  uint32_t m_fur = 0u, m_sec = 0u;
+  if (kb.mask) {
+    m_fur = ld_relaxed(&kb.mask[i_fur]);
+    m_sec = ld_relaxed(&kb.mask[i_sec]);
   }
+  // the furthest element now: value, slot, colour, mask
+  uint64_t cur = mx, i_cur = i_fur;
+  uint32_t c_cur = col_fur, m_cur = m_fur;
+  uint32_t bound = 0u;
+  bool moved = false;
+  if (run_a) {
+    a_tail = true;
+    const uint64_t me = ((uint64_t)a_key << 32) | a_id;
+    if (mx > me) {
+      a_tail_id = (uint32_t)mx;
+      a_tail_col = col_fur;
+      if (kb.mask) {
+        a_tail_mask = m_fur;
+        st_relaxed(&kb.mask[i_fur], a_mask);
+      }
+      st_relaxed(&kb.col[i_fur], a_col);
+      st_relaxed(&kb.key[i_fur], a_key);
+      st_relaxed(&kb.id[i_fur], a_id);
+      bound = (uint32_t)(mx >> 32);
+      moved = true;
+      if (me > mx2) {  // a is now the furthest
+        cur = me;
+        c_cur = a_col;
+        m_cur = a_mask;
+      } else {
+        cur = mx2;
+        i_cur = i_sec;
+        c_cur = col_sec;
+        m_cur = m_sec;
+      }
+    }
+  }
+  if (run_b) {
+    b_tail = true;
+    const uint64_t me = ((uint64_t)b_key << 32) | b_id;
+    if (cur > me) {
+      b_tail_id = (uint32_t)cur;
+      b_tail_col = c_cur;
+      if (kb.mask) {
+        b_tail_mask = m_cur;
+        st_relaxed(&kb.mask[i_cur], b_mask);
+      }
+      st_relaxed(&kb.col[i_cur], b_col);
+      st_relaxed(&kb.key[i_cur], b_key);
+      st_relaxed(&kb.id[i_cur], b_id);
+      bound = (uint32_t)(cur >> 32);
+      moved = true;
+    }
+  }
+  if (moved) st_relaxed(&kb.depth[t], bound);
 }
 
 // Colour pass (oitSpinlock.frag.glsl:49-129). The count of the reference
@@ -1744,9 +1785,8 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
       const uint32_t stored = ld_relaxed(&kb.filled[t]);
       if (lock_try(&kb.lock[t])) {
         if (stored >= (uint32_t)K) {
-          if (me_pass) spin_replace<K>(c, kb, t, my_key, my_col, lid, my_mask, tail, tail_id, tail_col, tail_mask);
-          if (p_pass)
-            spin_replace<K>(c, kb, t, p_key, p_col, lid + 1u, p_mask, p_tail, p_tail_id, p_tail_col, p_tail_mask);
+          spin_replace<K>(c, kb, t, me_pass, my_key, my_col, lid, my_mask, tail, tail_id, tail_col, tail_mask, p_pass,
+                          p_key, p_col, lid + 1u, p_mask, p_tail, p_tail_id, p_tail_col, p_tail_mask);
           pending = false;
         }
         lock_release(&kb.lock[t]);
@@ -1770,7 +1810,7 @@ __global__ __launch_bounds__(kBlock) void k_spin_store(Common c, FragView f, KBu
     // for both keeps the image independent of it
     float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
     if (tail) payload = decode_premult(tail_col, c.tab);
-    tail_record_evict(tail, valid ? t : kEmpty, tail_id, 0u, payload, tb, tail_mask);
+    tail_queue_push(tail, valid ? t : kEmpty, tail_id, 0u, payload, tb, tail_mask);
   }
 }
 
@@ -2082,7 +2122,7 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color_m(Common c, FragView f,
                                      __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if constexpr (kTail) tail_record_own(tail, t[u], lid, 0u, valid[u] ? cv[u] : make_float4(0.f, 0.f, 0.f, 0.f), tb);
+    if constexpr (kTail) tail_queue_push(tail, t[u], lid, 0u, valid[u] ? cv[u] : make_float4(0.f, 0.f, 0.f, 0.f), tb, 0xFFFFFFFFu);
   }
 }
 
