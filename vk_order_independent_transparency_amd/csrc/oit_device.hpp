@@ -279,6 +279,18 @@ __device__ __forceinline__ void store_fence() {
   __atomic_signal_fence(__ATOMIC_SEQ_CST);
 }
 
+// Waits for every memory operation the lane has in flight, at a point that
+// the whole wave reaches. After a step of conditional atomics or loads
+// (`if (live) x = atomic(...)`), the compiler otherwise keeps the returns
+// pending on the paths that skip their consumers and then waits for every
+// operation before each conditional access of the next step, which runs a
+// wave's independent chains one after another instead of together.
+__device__ __forceinline__ void returns_done() {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
 // Per-target lock for critical sections whose shared data is read and
 // written only with agent-scope relaxed atomics (coherent across the XCDs'
 // L2 caches, so the section needs ordering but no L2 writeback or

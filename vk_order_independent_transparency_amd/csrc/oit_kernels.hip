@@ -1587,6 +1587,11 @@ __global__ __launch_bounds__(kBlock) void k_simple_store(Common c, FragView f, K
 // Spinlock: oitSpinlock.frag.glsl
 // ---------------------------------------------------------------------------
 
+// The word at byte offset `off` of a wave-uniform base.
+__device__ __forceinline__ uint32_t* byte_at(uint32_t* base, uint64_t off) {
+  return reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(base) + off);
+}
+
 // The critical section of oitSpinlock.frag.glsl:95-123, run by the lane
 // that holds target t's lock on a full buffer, for the fragment a and then
 // for the fragment b (a pair; either may be absent): each replaces the
@@ -1604,22 +1609,33 @@ __device__ __forceinline__ void spin_replace(const Common& c, const KBuf& kb, ui
                                              uint32_t& b_tail_mask) {
   uint32_t fur = 0, sec = 0;
   uint64_t mx = 0, mx2 = 0;
-  // the scan as a loop of eight loads per trip (K/2 below 16): unrolled in
-  // full it held 100-256 VGPRs (three or four waves per SIMD at K = 8, 16),
-  // as a loop 49-63 (eight waves)
-  constexpr int kScanUnroll = K >= 16 ? 8 : (K >= 2 ? K / 2 : 1);
-#pragma unroll kScanUnroll
-  for (int j = 0; j < K; ++j) {
-    const uint64_t idx = (uint64_t)j * c.T + t;
-    const uint64_t v = ((uint64_t)ld_relaxed(&kb.key[idx]) << 32) | ld_relaxed(&kb.id[idx]);
-    if (v > mx) {
-      mx2 = mx;
-      sec = fur;
-      mx = v;
-      fur = (uint32_t)j;
-    } else if (v > mx2) {
-      mx2 = v;
-      sec = (uint32_t)j;
+  // the scan, eight slots per round trip: a batch's loads are issued
+  // together and compared after (branch-free), since a compare between two
+  // loads made every slot a round trip of its own
+  constexpr int kBatch = K <= 16 ? K : 8;
+  // the lane's byte offset, made opaque here so that each address is formed
+  // at its load (a scalar row base plus this offset) rather than hoisted out
+  // of the lock loop and held, one 64-bit address per slot
+  uint64_t off = (uint64_t)t * 4u;
+  asm volatile("" : "+v"(off));
+#pragma unroll 1
+  for (int j0 = 0; j0 < K; j0 += kBatch) {
+    uint32_t kk[kBatch], ii[kBatch];
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {  // a wave-uniform row base and the lane's byte offset
+      const uint64_t row = (uint64_t)(j0 + u) * c.T;
+      kk[u] = ld_relaxed(byte_at(kb.key + row, off));
+      ii[u] = ld_relaxed(byte_at(kb.id + row, off));
+    }
+#pragma unroll
+    for (int u = 0; u < kBatch; ++u) {
+      const uint64_t v = ((uint64_t)kk[u] << 32) | ii[u];
+      const uint32_t j = (uint32_t)(j0 + u);
+      const bool top = v > mx, second = !top && v > mx2;
+      mx2 = top ? mx : (second ? v : mx2);
+      sec = top ? fur : (second ? j : sec);
+      mx = top ? v : mx;
+      fur = top ? j : fur;
     }
   }
   // the colours (and masks) the sections can evict, in one round trip
@@ -2595,6 +2611,7 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f,
         any = true;
       }
     }
+    returns_done();
     if (!any) break;
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
