@@ -120,6 +120,45 @@ __device__ __forceinline__ void tail_list_push(bool tail, uint32_t t, uint32_t l
     atomicAdd(&tb.qn[lid >> kTailRegionShift], (uint32_t)__builtin_popcountll(m));
 }
 
+// tail_list_push over kU chunks of a wave (chunk u: tail[u], t[u], lid[u],
+// ...), with every chunk's exchanges in flight together: the wave waits one
+// round trip for all of them. Must be reached by all 64 lanes.
+template <int kU>
+__device__ __forceinline__ void tail_list_push_n(const bool (&tail)[kU], const uint32_t (&t)[kU],
+                                                 const uint32_t (&lid)[kU], const uint32_t (&flag)[kU],
+                                                 const float4 (&payload)[kU], const TailBuf& tb) {
+  const uint32_t lane = lane_id();
+  bool ex[kU];
+  uint32_t lo[kU], old[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const bool tl = tail[u] && lid[u] < tb.cap;
+    if (tl) {
+      tb.rgba[lid[u]] = payload[u];
+      if (tb.mask) tb.mask[lid[u]] = 0xFFFFFFFFu;
+    }
+    const uint32_t v = tail_value(lid[u], flag[u]);
+    const Run r = same_target_run(tl ? t[u] : kEmpty);
+    const uint32_t v_prev = __shfl_up(v, 1u);
+    lo[u] = __shfl(v, (int)r.start_lane);
+    if (tl && !r.start) st_relaxed(&tb.next[lid[u]], v_prev);
+    ex[u] = tl && r.end;
+    const uint64_t m = __ballot(tl);
+    if (m != 0ull && lane == (uint32_t)__builtin_ctzll(m))
+      atomicAdd(&tb.qn[lid[u] >> kTailRegionShift], (uint32_t)__builtin_popcountll(m));
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u) old[u] = 0u;
+#pragma unroll
+  for (int u = 0; u < kU; ++u)
+    if (ex[u]) old[u] = __hip_atomic_exchange(&tb.head[t[u]], tail_value(lid[u], flag[u]), __ATOMIC_RELAXED,
+                                              __HIP_MEMORY_SCOPE_AGENT);
+  returns_done();
+#pragma unroll
+  for (int u = 0; u < kU; ++u)
+    if (ex[u]) st_relaxed(&tb.next[tail_index(lo[u])], old[u]);
+}
+
 // Sorted register array of K (sort key, colour) pairs, ascending; inserting x
 // returns the element pushed out of the back (x itself if it does not fit):
 // insertionSortTail, oitCompositeDefines.glsl:116-139, unrolled into selects.
@@ -2651,7 +2690,10 @@ template <int K, int kU>
 __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f, const uint64_t* __restrict__ slots,
                                                             const uint64_t* __restrict__ bigm, TailBuf tb) {
   __shared__ float s_thr[257];
-  stage_thresholds(s_thr, c.tab);
+  __shared__ float s_dec[256];
+  __shared__ float s_alpha[256];
+  stage_decode(s_dec, s_alpha, c.tab);
+  stage_thresholds(s_thr, c.tab);  // its barrier also publishes the decode tables
   const uint32_t lane = lane_id();
   const uint64_t first = ((uint64_t)blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6)) * (64u * kU);
   bool big[kU];
@@ -2671,37 +2713,46 @@ __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f
     t[u] = big[u] ? ld_stream(&f.target[i]) : 0u;
     dv[u] = big[u] ? ld_stream(&f.depth[i]) : 0.f;
   }
+  // every chunk's K-th slots in one round trip, loaded by every lane (a lane
+  // outside such a pixel reads target 0's: one line for all of them), so no
+  // load waits behind another
   uint64_t sk[kU];
 #pragma unroll
-  for (int u = 0; u < kU; ++u) sk[u] = big[u] ? slots[(uint64_t)(K - 1) * c.T + t[u]] : 0ull;
+  for (int u = 0; u < kU; ++u) sk[u] = slots[(uint64_t)(K - 1) * c.T + t[u]];
   uint32_t key[kU];
   bool cand[kU];
   float4 cv[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    key[u] = big[u] ? depth_key(dv[u]) : 0u;
+    key[u] = depth_key(dv[u]);
     // the key's high half is the depth: a nearer fragment stays in the slots
     cand[u] = big[u] && key[u] >= (uint32_t)(sk[u] >> 32);
-    const uint64_t i = first + (uint64_t)u * 64u + lane;
-    cv[u] = cand[u] ? ld_stream(&f.rgba[i]) : make_float4(0.f, 0.f, 0.f, 0.f);
+    cv[u] = make_float4(0.f, 0.f, 0.f, 0.f);
   }
 #pragma unroll
+  for (int u = 0; u < kU; ++u)  // the candidates' colours, one round trip
+    if (cand[u]) cv[u] = ld_stream(&f.rgba[first + (uint64_t)u * 64u + lane]);
+  returns_done();
+  bool tail[kU];
+  uint32_t flag[kU], lid[kU];
+  float4 payload[kU];
+#pragma unroll
   for (int u = 0; u < kU; ++u) {
-    bool tail = false;
-    uint32_t flag = 0;
-    float4 payload = make_float4(0.f, 0.f, 0.f, 0.f);
+    tail[u] = false;
+    flag[u] = 0u;
+    payload[u] = make_float4(0.f, 0.f, 0.f, 0.f);
     if (cand[u]) {
       const uint32_t col = encode_color_lds(cv[u], s_thr);
       const uint64_t z = ((uint64_t)key[u] << 32) | col;
       if (z >= sk[u]) {
-        tail = true;
-        flag = z == sk[u] ? 1u : 0u;
-        payload = decode_premult(col, c.tab);
+        tail[u] = true;
+        flag[u] = z == sk[u] ? 1u : 0u;
+        payload[u] = decode_premult_lds(col, s_dec, s_alpha);
       }
     }
-    const uint64_t i = first + (uint64_t)u * 64u + lane;
-    tail_list_push(tail, t[u], f.id_base + (uint32_t)i, flag, payload, tb);
+    lid[u] = f.id_base + (uint32_t)(first + (uint64_t)u * 64u + lane);
   }
+  tail_list_push_n<kU>(tail, t, lid, flag, payload, tb);
 }
 
 // Composite (oitLoop64.frag.glsl:162-183): blend slots front to back until
