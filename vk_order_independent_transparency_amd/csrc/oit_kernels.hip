@@ -2083,6 +2083,7 @@ __global__ __launch_bounds__(kBlock) void k_loop32_depth_m(Common c, FragView f,
         any = true;
       }
     }
+    returns_done();
     if (!any) break;
 #pragma unroll
     for (int u = 0; u < kU; ++u) {
@@ -2146,21 +2147,43 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color_m(Common c, FragView f,
     if (valid[u] && !opaque_visible(c.opq_z, c.T, 1u, t[u], z[u], 1u)) valid[u] = false;
     if (!valid[u]) t[u] = kEmpty;
   }
+  // the first half of every chunk's depth slots in one round trip, loaded by
+  // every lane (one without a fragment reads target 0's), so no load waits
+  // behind another; the second half only where the first is all nearer
   constexpr int kHalf = K / 2 > 0 ? K / 2 : K;
-  uint32_t below[kU];
+  uint32_t below[kU], ts[kU];
+  {
+    uint32_t zk[kU][kHalf];
 #pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    below[u] = 0;
-    if (valid[u]) {
+    for (int u = 0; u < kU; ++u) {
+      ts[u] = valid[u] ? t[u] : 0u;
 #pragma unroll
-      for (int k = 0; k < kHalf; ++k) below[u] += zs[(uint64_t)k * c.T + t[u]] < z[u] ? 1u : 0u;
+      for (int k = 0; k < kHalf; ++k) zk[u][k] = zs[(uint64_t)k * c.T + ts[u]];
+    }
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      below[u] = 0;
+#pragma unroll
+      for (int k = 0; k < kHalf; ++k) below[u] += valid[u] && zk[u][k] < z[u] ? 1u : 0u;
     }
   }
+  if constexpr (kHalf < K) {
+    bool more[kU], any = false;
 #pragma unroll
-  for (int u = 0; u < kU; ++u) {
-    if (valid[u] && below[u] == (uint32_t)kHalf) {
+    for (int u = 0; u < kU; ++u) {
+      more[u] = valid[u] && below[u] == (uint32_t)kHalf;
+      any = any || more[u];
+    }
+    if (__ballot(any)) {  // wave-uniform: then every lane loads, as above
+      uint32_t zk[kU][K - kHalf];
 #pragma unroll
-      for (int k = kHalf; k < K; ++k) below[u] += zs[(uint64_t)k * c.T + t[u]] < z[u] ? 1u : 0u;
+      for (int u = 0; u < kU; ++u)
+#pragma unroll
+        for (int k = kHalf; k < K; ++k) zk[u][k - kHalf] = zs[(uint64_t)k * c.T + ts[u]];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+#pragma unroll
+        for (int k = kHalf; k < K; ++k) below[u] += more[u] && zk[u][k - kHalf] < z[u] ? 1u : 0u;
     }
   }
 #pragma unroll
@@ -2565,21 +2588,31 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f,
     }
     t[u] = tt;
   }
-  // every chunk's pixel counts in one round trip
+  // every chunk's pixel counts in one round trip, loaded by every lane (one
+  // without a fragment reads target 0's), so no load waits behind another
   uint32_t n[kU];
 #pragma unroll
-  for (int u = 0; u < kU; ++u) n[u] = t[u] != kEmpty ? cnt[t[u]] : 0u;
-  // pixels of at most K fragments: a slot index per fragment from the count
-  uint32_t base[kU];
-  bool small[kU];
+  for (int u = 0; u < kU; ++u) n[u] = cnt[t[u] != kEmpty ? t[u] : 0u];
+  // pixels of at most K fragments: a slot index per fragment from the count,
+  // every chunk's atomics in flight together
+  uint32_t base[kU], start_lane[kU], rank[kU];
+  bool small[kU], sub[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
     small[u] = t[u] != kEmpty && n[u] <= (uint32_t)K;
     const Run r = same_target_run(small[u] ? t[u] : kEmpty);
+    sub[u] = small[u] && r.start;
+    rank[u] = r.end_lane - r.lane + 1u;  // the run's length, at its first lane
+    start_lane[u] = r.start_lane;
     base[u] = 0u;
-    if (small[u] && r.start) base[u] = atomicSub(&cnt[t[u]], r.end_lane - r.lane + 1u);
-    base[u] = __shfl(base[u], (int)r.start_lane) - (r.lane - r.start_lane);  // this lane's index + 1
   }
+#pragma unroll
+  for (int u = 0; u < kU; ++u)
+    if (sub[u]) base[u] = atomicSub(&cnt[t[u]], rank[u]);
+  returns_done();
+#pragma unroll
+  for (int u = 0; u < kU; ++u)  // this lane's index + 1
+    base[u] = __shfl(base[u], (int)start_lane[u]) - (lane_id() - start_lane[u]);
   // pixels of more than K: their fragments' lanes, per 64-fragment chunk,
   // for the classification (which skips the rest); pairs and chains
 #pragma unroll
