@@ -94,35 +94,13 @@ __device__ __forceinline__ void tail_queue_push(bool tail, uint32_t t, uint32_t 
 
 // A converged wave's tail fragments straight onto their targets' lists,
 // for a pass that runs after every fragment of the frame is stored (no link
-// pass follows): neighbouring lanes of one target are chained in the wave
-// and published with one exchange on the list head (k_tail_link does the
-// same for queued records); the wave's count goes to its region's counter,
-// which the queue would otherwise have filled. Must be reached by all 64
-// lanes (lanes without a tail fragment pass tail = false).
-__device__ __forceinline__ void tail_list_push(bool tail, uint32_t t, uint32_t lid, uint32_t flag, float4 payload,
-                                               const TailBuf& tb) {
-  tail = tail && lid < tb.cap;
-  if (tail) {
-    tb.rgba[lid] = payload;
-    if (tb.mask) tb.mask[lid] = 0xFFFFFFFFu;
-  }
-  const uint32_t v = tail_value(lid, flag);
-  const Run r = same_target_run(tail ? t : kEmpty);
-  const uint32_t v_prev = __shfl_up(v, 1u);
-  const uint32_t lo = __shfl(v, (int)r.start_lane);
-  if (tail && !r.start) st_relaxed(&tb.next[lid], v_prev);
-  if (tail && r.end) {
-    const uint32_t old = __hip_atomic_exchange(&tb.head[t], v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    st_relaxed(&tb.next[tail_index(lo)], old);
-  }
-  const uint64_t m = __ballot(tail);
-  if (m != 0ull && lane_id() == (uint32_t)__builtin_ctzll(m))
-    atomicAdd(&tb.qn[lid >> kTailRegionShift], (uint32_t)__builtin_popcountll(m));
-}
-
-// tail_list_push over kU chunks of a wave (chunk u: tail[u], t[u], lid[u],
-// ...), with every chunk's exchanges in flight together: the wave waits one
-// round trip for all of them. Must be reached by all 64 lanes.
+// pass follows), over kU chunks of the wave (chunk u: tail[u], t[u], lid[u],
+// ...): neighbouring lanes of one target are chained in the wave and
+// published with one exchange on the list head (k_tail_link does the same
+// for queued records), every chunk's exchanges in flight together; each
+// chunk's count goes to its region's counter, which the queue would
+// otherwise have filled. Must be reached by all 64 lanes (lanes without a
+// tail fragment pass tail = false).
 template <int kU>
 __device__ __forceinline__ void tail_list_push_n(const bool (&tail)[kU], const uint32_t (&t)[kU],
                                                  const uint32_t (&lid)[kU], const uint32_t (&flag)[kU],
