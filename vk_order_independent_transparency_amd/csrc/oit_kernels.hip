@@ -1626,17 +1626,20 @@ __device__ __forceinline__ void spin_replace(const Common& c, const KBuf& kb, ui
                                              uint32_t& b_tail_mask) {
   uint32_t fur = 0, sec = 0;
   uint64_t mx = 0, mx2 = 0;
-  // the scan, eight slots per round trip: a batch's loads are issued
-  // together and compared after (branch-free), since a compare between two
-  // loads made every slot a round trip of its own
-  constexpr int kBatch = K <= 16 ? K : 8;
+  // the scan, a batch of up to eight slots at a time: a batch's loads are
+  // issued together and compared after (branch-free), since a compare
+  // between two loads made every slot a round trip of its own. At K = 16
+  // sixteen slots in one batch held 70 VGPRs (seven waves per SIMD), two
+  // batches 62 (eight), which ran faster
+  constexpr int kBatch = K < 16 ? K : 8;
   // the lane's byte offset, made opaque here so that each address is formed
   // at its load (a scalar row base plus this offset) rather than hoisted out
   // of the lock loop and held, one 64-bit address per slot
   uint64_t off = (uint64_t)t * 4u;
   asm volatile("" : "+v"(off));
-#pragma unroll 1
+#pragma unroll
   for (int j0 = 0; j0 < K; j0 += kBatch) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);  // a compiler barrier between batches
     uint32_t kk[kBatch], ii[kBatch];
 #pragma unroll
     for (int u = 0; u < kBatch; ++u) {  // a wave-uniform row base and the lane's byte offset
