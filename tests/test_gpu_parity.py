@@ -132,7 +132,7 @@ def test_linkedlist_pool_overflow_bitexact(oracle, cap_div):
     _assert_same("out", g["out"], o["out"])
 
 
-@pytest.mark.parametrize("K", [1, 4, 8, 16, 32])
+@pytest.mark.parametrize("K", [1, 2, 4, 8, 16, 32])
 @pytest.mark.parametrize("tail", [0, 1])
 @pytest.mark.parametrize("variant", [0, 0x10, 0x20])  # store: 2 chunks per wave, 1 fragment per lane, 4 chunks
 def test_loop64_bitexact(oracle, K, tail, variant):
@@ -146,7 +146,7 @@ def test_loop64_bitexact(oracle, K, tail, variant):
     _assert_same("out", g["out"], o["out"])
 
 
-@pytest.mark.parametrize("K", [1, 4, 8, 16, 32])
+@pytest.mark.parametrize("K", [1, 2, 4, 8, 16, 32])  # every scan batch shape: K slots, and 8 per batch above 8
 @pytest.mark.parametrize("tail", [0, 1])
 def test_spinlock_bitexact(oracle, K, tail):
     p, tg, d, c = deep_scene(seed=9, spheres=160 if K <= 16 else 400)
