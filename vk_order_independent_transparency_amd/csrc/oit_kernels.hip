@@ -2697,9 +2697,10 @@ __global__ __launch_bounds__(kBlock) void k_loop64_store_m(Common c, FragView f,
 // Only pixels of more than K fragments can have tails (the others keep every
 // fragment): the store left, per 64-fragment chunk, the mask of the lanes
 // whose fragment belongs to such a pixel, and a chunk with an empty mask is
-// skipped. Each wave takes kU chunks and loads, one round trip each, for
-// the masked lanes: target, depth and then the K-th slot; for those at or
-// past the K-th depth, the colour.
+// skipped. Each wave takes kU chunks and loads, one round trip each: the
+// masked lanes' targets and depths, then every lane's K-th slot, then the
+// colours of the lanes at or past the K-th depth; the tail fragments go
+// onto their lists with every chunk's exchanges in flight together.
 template <int K, int kU>
 __global__ __launch_bounds__(kBlock) void k_loop64_classify(Common c, FragView f, const uint64_t* __restrict__ slots,
                                                             const uint64_t* __restrict__ bigm, TailBuf tb) {
