@@ -92,6 +92,56 @@ __device__ __forceinline__ void tail_queue_push(bool tail, uint32_t t, uint32_t 
     tb.rec[((uint64_t)r << kTailRegionShift) + q + lanes_below(my_group)] = ((uint64_t)t << 32) | tail_value(lid, flag);
 }
 
+// tail_queue_push over kU chunks of a wave (chunk u: tail[u], t[u], lid[u],
+// payload[u]; no boundary flag, full coverage), with every chunk's region
+// counter atomics in flight together: the wave waits one round trip for all
+// of them. Must be reached by all 64 lanes.
+template <int kU>
+__device__ __forceinline__ void tail_queue_push_n(const bool (&tail_in)[kU], const uint32_t (&t)[kU],
+                                                  const uint32_t (&lid)[kU], const float4 (&payload)[kU],
+                                                  const TailBuf& tb) {
+  const uint32_t lane = lane_id();
+  bool tail[kU];
+  uint32_t r[kU], leader[kU], q[kU];
+  uint64_t group[kU];
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    tail[u] = tail_in[u] && lid[u] < tb.cap && tb.qn != nullptr;
+    if (tail[u]) {
+      tb.rgba[lid[u]] = payload[u];
+      if (tb.mask) tb.mask[lid[u]] = 0xFFFFFFFFu;
+    }
+    r[u] = lid[u] >> kTailRegionShift;
+    // group the lanes by region (cross-lane work only)
+    uint64_t rem = __ballot(tail[u]);
+    leader[u] = 0;
+    group[u] = 0;
+    while (rem) {
+      const uint32_t l = (uint32_t)__builtin_ctzll(rem);
+      const uint32_t rl = __builtin_amdgcn_readlane(r[u], l);
+      const uint64_t m = __ballot(tail[u] && r[u] == rl);
+      if ((m >> lane) & 1ull) {
+        leader[u] = l;
+        group[u] = m;
+      }
+      rem &= ~m;
+    }
+    q[u] = 0u;
+    asm volatile("" : "+v"(q[u]));  // the zeros are written before any atomic goes out
+  }
+#pragma unroll
+  for (int u = 0; u < kU; ++u)
+    if (tail[u] && lane == leader[u]) q[u] = atomicAdd(&tb.qn[r[u]], (uint32_t)__builtin_popcountll(group[u]));
+  returns_done();
+#pragma unroll
+  for (int u = 0; u < kU; ++u) {
+    const uint32_t base = __shfl(q[u], (int)leader[u]);
+    if (tail[u])
+      tb.rec[((uint64_t)r[u] << kTailRegionShift) + base + lanes_below(group[u])] =
+          ((uint64_t)t[u] << 32) | tail_value(lid[u], 0u);
+  }
+}
+
 // A converged wave's tail fragments straight onto their targets' lists,
 // for a pass that runs after every fragment of the frame is stored (no link
 // pass follows), over kU chunks of the wave (chunk u: tail[u], t[u], lid[u],
@@ -126,7 +176,10 @@ __device__ __forceinline__ void tail_list_push_n(const bool (&tail)[kU], const u
       atomicAdd(&tb.qn[lid[u] >> kTailRegionShift], (uint32_t)__builtin_popcountll(m));
   }
 #pragma unroll
-  for (int u = 0; u < kU; ++u) old[u] = 0u;
+  for (int u = 0; u < kU; ++u) {
+    old[u] = 0u;
+    asm volatile("" : "+v"(old[u]));  // the zeros are written before any exchange goes out
+  }
 #pragma unroll
   for (int u = 0; u < kU; ++u)
     if (ex[u]) old[u] = __hip_atomic_exchange(&tb.head[t[u]], tail_value(lid[u], flag[u]), __ATOMIC_RELAXED,
@@ -2167,22 +2220,25 @@ __global__ __launch_bounds__(kBlock) void k_loop32_color_m(Common c, FragView f,
         for (int k = kHalf; k < K; ++k) below[u] += more[u] && zk[u][k - kHalf] < z[u] ? 1u : 0u;
     }
   }
+  bool tail[kU];
+  uint32_t lid[kU];
+  float4 payload[kU];
 #pragma unroll
   for (int u = 0; u < kU; ++u) {
-    const uint64_t i = first + (uint64_t)u * 64u + lane;
-    const uint32_t lid = f.id_base + (uint32_t)i;
-    bool tail = false;
+    lid[u] = f.id_base + (uint32_t)(first + (uint64_t)u * 64u + lane);
+    tail[u] = false;
+    payload[u] = valid[u] ? cv[u] : make_float4(0.f, 0.f, 0.f, 0.f);
     if (valid[u]) {
       if (below[u] == (uint32_t)K) {
-        tail = true;
+        tail[u] = true;
       } else {
-        const uint64_t v = ((uint64_t)(lid + 1u) << 32) | encode_color_lds(cv[u], s_thr);
+        const uint64_t v = ((uint64_t)(lid[u] + 1u) << 32) | encode_color_lds(cv[u], s_thr);
         (void)__hip_atomic_fetch_max(&cs[(uint64_t)below[u] * c.T + t[u]], v, __ATOMIC_RELAXED,
                                      __HIP_MEMORY_SCOPE_AGENT);
       }
     }
-    if constexpr (kTail) tail_queue_push(tail, t[u], lid, 0u, valid[u] ? cv[u] : make_float4(0.f, 0.f, 0.f, 0.f), tb, 0xFFFFFFFFu);
   }
+  if constexpr (kTail) tail_queue_push_n<kU>(tail, t, lid, payload, tb);
 }
 
 // Composite (:186-222): the layers up to the first empty depth, front to back,
